@@ -1,0 +1,237 @@
+"""Benchmark: amplitude updates/s & adjoint grads/s vs the plain per-gate kernel,
+and % of HBM peak, on one MI355X.
+
+Workload of the headline line (``value``): 20 qubits, bsz 16, complex64;
+20 layers of [RY(theta ~ U[0, 2pi)) on every wire, CZ on even then odd pairs];
+<Z_i> per wire; per-gate HIP kernels, forward only.  One step = one forward
+pass of that circuit over the batch plus the <Z> read-out.
+
+Extra fields in the same JSON line:
+  * fused:   the same circuit through the fused-stage engine;
+  * adjoint: 24 qubits, bsz 64, 10 x [RX, RY, RZ all wires (N(0, 0.1)), CNOT
+    ring], loss = sum <Z_i Z_i+1> + 0.5 sum <X_i>; gradient evaluations/s for
+    the fused+adjoint engine and for the plain per-gate adjoint path;
+  * roofline of the dominant kernel (1-qubit per-gate kernel, HIP events);
+  * cpu_baseline: the reference's per-gate torch bmm path (restated) on the
+    host cores, on a bounded sample of the headline circuit.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np
+import torch
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_config2(q, ops, tqf):
+    for op in ops:
+        p = op["params"]
+        tqf.func_name_dict[op["name"]](q, op["wires"], params=None if p is None else
+                                       torch.tensor(p, dtype=torch.float32, device=q.device))
+
+
+def time_forward(n, bsz, ops, mode, steps, warmup, tq):
+    import torchquantum_amd.functional as tqf
+    params = [None if op["params"] is None else torch.tensor(op["params"], dtype=torch.float32, device="cuda")
+              for op in ops]
+
+    def step():
+        q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
+        q.engine_options = tq.EngineOptions(mode=mode)
+        for op, p in zip(ops, params):
+            tqf.func_name_dict[op["name"]](q, op["wires"], params=p)
+        return tq.MeasureAll(tq.PauliZ)(q)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return dt, out
+
+
+def time_gradient(n, bsz, layers, mode, steps, warmup, tq):
+    from circuits import config3_ops, config3_observables
+    import torchquantum_amd.functional as tqf
+    ops = config3_ops(seed=0, n=n, layers=layers)
+    params = [torch.tensor(op["params"], dtype=torch.float32, device="cuda", requires_grad=True) for op in ops
+              if op["params"] is not None]
+    obs, coef = config3_observables(n)
+    # per-row input layer makes the batch rows distinct: RY(phi_b) on every wire
+    g = torch.Generator().manual_seed(1)
+    phi = (torch.rand(bsz, 1, generator=g) * np.pi).to("cuda")
+
+    def step():
+        for p in params:
+            p.grad = None
+        q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
+        q.engine_options = tq.EngineOptions(mode=mode)
+        for w in range(n):
+            tqf.ry(q, w, params=phi)
+        k = 0
+        for op in ops:
+            if op["params"] is None:
+                tqf.func_name_dict[op["name"]](q, op["wires"])
+            else:
+                tqf.func_name_dict[op["name"]](q, op["wires"], params=params[k])
+                k += 1
+        loss = tq.expval_pauli_sum(q, obs, coef).mean()
+        loss.backward()
+        return loss
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    grads = torch.cat([p.grad.reshape(-1) for p in params])
+    return dt, float(loss), grads
+
+
+def kernel_roofline(n, bsz, iters=50):
+    """Average duration of the 1-qubit per-gate kernel (RY on a mid wire) with
+    HIP events on the launch stream; algorithmic bytes = 16 B x bsz x 2^n."""
+    from torchquantum_amd import _native as N
+    st = torch.zeros((bsz, 1 << n), dtype=torch.complex64, device="cuda")
+    st[:, 0] = 1
+    th = 0.7
+    m = torch.tensor([[np.cos(th / 2), -np.sin(th / 2)], [np.sin(th / 2), np.cos(th / 2)]],
+                     dtype=torch.complex64, device="cuda")
+    stream = N.stream_of(st)
+    bit = n // 2
+    for _ in range(5):
+        N.call("tqsv_apply_1q", stream, st.data_ptr(), bsz, n, bit, 0, 0, m.data_ptr(), 0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for i in range(iters):
+        N.call("tqsv_apply_1q", stream, st.data_ptr(), bsz, n, (bit + i) % n, 0, 0, m.data_ptr(), 0)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / iters
+    nbytes = 16.0 * bsz * (1 << n)
+    return ms, nbytes
+
+
+def cpu_baseline(ops_sample, n, bsz, n_gates_total_per_step):
+    from oracle import refpath
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    t0 = time.perf_counter()
+    refpath.run(ops_sample, n, bsz, device="cpu")
+    dt = time.perf_counter() - t0
+    upd = len(ops_sample) * bsz * (1 << n)
+    return upd / dt, threads, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--adj-qubits", type=int, default=24)
+    ap.add_argument("--adj-bsz", type=int, default=64)
+    ap.add_argument("--adj-steps", type=int, default=2)
+    ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-layers", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md but not built)"}))
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no ROCm GPU visible"}))
+        sys.exit(2)
+    import torchquantum_amd as tq
+    from circuits import config2_ops
+
+    n, bsz = 20, 16
+    ops = config2_ops(seed=0, n=n, layers=20)
+    upd_per_step = len(ops) * bsz * (1 << n)
+
+    log("[bench] config2 per-gate forward")
+    dt_pg, ev_pg = time_forward(n, bsz, ops, "pergate", args.steps, args.warmup, tq)
+    log(f"[bench] per-gate {dt_pg * 1e3:.2f} ms/step")
+    log("[bench] config2 fused forward")
+    dt_f, ev_f = time_forward(n, bsz, ops, "fused", args.steps, args.warmup, tq)
+    log(f"[bench] fused {dt_f * 1e3:.2f} ms/step")
+    parity = float((ev_pg - ev_f).abs().max())
+
+    log("[bench] kernel roofline")
+    ms, nbytes = kernel_roofline(n, bsz)
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    ms_big, nbytes_big = kernel_roofline(26, 16, iters=20)
+    achieved_big = nbytes_big / (ms_big * 1e-3) / 1e9
+
+    extra = {}
+    if not args.skip_adjoint:
+        an, ab = args.adj_qubits, args.adj_bsz
+        log(f"[bench] adjoint gradient fused n={an} bsz={ab}")
+        dt_af, loss_f, g_f = time_gradient(an, ab, 10, "fused", args.adj_steps, 1, tq)
+        log(f"[bench] adjoint fused {dt_af:.3f} s/grad")
+        log(f"[bench] adjoint per-gate n={an} bsz={ab}")
+        dt_ap, loss_p, g_p = time_gradient(an, ab, 10, "pergate", args.adj_steps, 1, tq)
+        log(f"[bench] adjoint per-gate {dt_ap:.3f} s/grad")
+        extra["adjoint"] = {
+            "workload": f"{an} qubits, bsz {ab}, 10 x [RX,RY,RZ all wires, CNOT ring], 720 params",
+            "fused_grads_per_s": 1.0 / dt_af, "pergate_grads_per_s": 1.0 / dt_ap,
+            "speedup_vs_pergate": dt_ap / dt_af,
+            "grad_max_abs_diff_fused_vs_pergate": float((g_f - g_p).abs().max()),
+            "loss": loss_f,
+        }
+
+    cpu = None
+    if not args.skip_cpu:
+        sample = ops[: 39 * args.cpu_layers]
+        log(f"[bench] cpu baseline ({len(sample)} gates)")
+        rate, threads, cdt = cpu_baseline(sample, n, bsz, len(ops))
+        cpu = {"value": rate, "unit": "amplitude updates/s", "cores": threads, "kind": "port",
+               "sample": f"first {len(sample)} gates of the 780-gate circuit, 20 qubits bsz 16, torch CPU "
+                         f"bmm path restated from gate_wrapper.py:96-140 ({cdt:.1f} s)"}
+
+    line = {
+        "metric": "amplitude updates/s & adjoint grads/s vs plain per-gate kernel, % HBM peak",
+        "value": upd_per_step / dt_pg,
+        "unit": "amplitude updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt_pg * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex64 (fp32)", "data": "synthetic (seeded angles)",
+        "config": {"workload": "20 qubits, bsz 16, 20 x [RY all wires, CZ even+odd pairs], <Z_i>, per-gate "
+                               "kernels, forward", "n_qubits": n, "bsz": bsz, "gates": len(ops)},
+        "fused": {"amp_updates_per_s": upd_per_step / dt_f, "ms_per_step": dt_f * 1e3,
+                  "speedup_vs_pergate": dt_pg / dt_f, "expval_max_abs_diff_vs_pergate": parity},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_1q_v2 (RY, 20 qubits bsz 16: 256 MiB/launch)", "avg_launch_ms": ms,
+                     "at_26q_bsz16": {"achieved": achieved_big, "frac": achieved_big / HBM_PEAK_GBS,
+                                      "avg_launch_ms": ms_big}},
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,116 @@
+/*
+ * tqsv.h -- C ABI of libtqsv.so, the MI355X (gfx950) statevector kernels
+ * behind torchquantum_amd.
+ *
+ * The reference (mit-han-lab/torchquantum 0.1.8) is pure Python over torch;
+ * its hot path has no native interface.  Each entry point below replaces the
+ * torch computation named in its comment (file:line in the reference tree).
+ *
+ * Conventions (all entry points):
+ *   - state: device pointer to bsz rows of 2^n complex64 amplitudes
+ *     (interleaved re, im float32), row b at [b << n, (b + 1) << n).
+ *     Wire w of the reference API is bit position (n - 1 - w) of the in-row
+ *     index (torchquantum/device/devices.py:64-71); every argument named
+ *     *bit* / *mask* below is in bit positions.
+ *   - matrices: device pointers to row-major complex64; a 2-qubit matrix is
+ *     indexed 2*b(bit_hi) + b(bit_lo) (gate_wrapper.py:120-127: wires[0] is
+ *     the most significant bit).  mat_bstride: complex elements between the
+ *     matrices of consecutive batch rows, 0 = one matrix shared by all rows.
+ *   - controls: cmask selects control bits, cval their required values.
+ *   - stream: hipStream_t (passed as void*); every call is asynchronous on it.
+ *   - return value: 0 on success, negative on error (-1 invalid argument,
+ *     -2 HIP launch/runtime error, -3 a documented kernel limit exceeded);
+ *     tqsv_last_error() describes the last error of the calling thread.
+ *   - no entry point allocates device memory; workspaces come from the caller.
+ */
+#ifndef TQSV_H
+#define TQSV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* tqsv_stream_t; /* hipStream_t */
+
+int tqsv_version(void);
+const char* tqsv_last_error(void);
+int tqsv_num_cu(void);
+
+/* |index> in every row.  Replaces QuantumDevice.__init__ / reset_states
+ * (torchquantum/device/devices.py:64-71, 89-92). */
+int tqsv_init_basis(tqsv_stream_t s, void* state, int64_t bsz, int n, int64_t index);
+
+/* ---- per-gate kernels: replace apply_unitary_bmm / apply_unitary_einsum
+ *      (torchquantum/functional/gate_wrapper.py:18-140) ------------------- */
+
+/* (controlled) 2x2 on bit tbit. */
+int tqsv_apply_1q(tqsv_stream_t s, void* state, int64_t bsz, int n, int tbit, uint64_t cmask, uint64_t cval,
+                  const void* mat, int64_t mat_bstride);
+/* (controlled) 4x4 on bits (bit_hi, bit_lo). */
+int tqsv_apply_2q(tqsv_stream_t s, void* state, int64_t bsz, int n, int bit_hi, int bit_lo, uint64_t cmask,
+                  uint64_t cval, const void* mat, int64_t mat_bstride);
+/* diagonal: amplitudes j with (j & cmask) == cval are multiplied by
+ * phases[popcount(j & pmask) & 1] (phases: 2 complex; ph_bstride as above).
+ * Covers RZ, RZZ, MultiRZ, CRZ, Z, S, T, PhaseShift, CZ, CCZ, CU1 ... */
+int tqsv_apply_diag(tqsv_stream_t s, void* state, int64_t bsz, int n, uint64_t cmask, uint64_t cval,
+                    uint64_t pmask, const void* phases, int64_t ph_bstride);
+/* (controlled) exchange of bits a and b: SWAP, CSWAP. */
+int tqsv_apply_swap(tqsv_stream_t s, void* state, int64_t bsz, int n, int bit_a, int bit_b, uint64_t cmask,
+                    uint64_t cval);
+/* dense 2^k x 2^k (k <= 10) on host-listed bits (bits[0] = matrix MSB):
+ * qubitunitary / qubitunitaryfast (functional/qubit_unitary.py:19-69), QFT, RCCX ... */
+int tqsv_apply_kq(tqsv_stream_t s, void* state, int64_t bsz, int n, int k, const int* bits, uint64_t cmask,
+                  uint64_t cval, const void* mat, int64_t mat_bstride);
+
+/* ---- fused stages: replace the static-mode block application
+ *      (torchquantum/graph/graphs.py:384-532, 836-860) and runs of per-gate
+ *      calls.  prog_host/prog_dev: the same stage program (int32 words, format
+ *      in tqsv_stage.hip) on host and device; mats: matrix pool.  state_in may
+ *      equal state_out, or be NULL with init_zero = 1 (start from |0..0>).
+ *      grid_cap: workgroups per launch, 0 = automatic.  n <= 32. ---------- */
+int tqsv_stage_forward(tqsv_stream_t s, void* state_out, const void* state_in, int64_t bsz, int n,
+                       const int* prog_host, const void* prog_dev, const void* mats, int init_zero, int grid_cap);
+/* Reverse sweep of one stage for the adjoint method: psi <- U^dag psi,
+ * lam <- U^dag lam, grad[row * grad_stride + slot] += Re<lam|dU/dtheta psi>.
+ * Replaces torch autograd through gate_wrapper.py:127-136. */
+int tqsv_stage_adjoint(tqsv_stream_t s, void* psi, void* lam, int64_t bsz, int n, const int* prog_host,
+                       const void* prog_dev, const void* mats, float* grad, int64_t grad_stride, int grid_cap);
+
+/* ---- per-gate adjoint steps (the unfused gradient path); mat holds U then
+ *      P derivative matrices dU/dtheta_p (P <= 4); grad slots gslot..gslot+P-1.
+ *      Replace autograd through apply_unitary_bmm (gate_wrapper.py:127-136). */
+int tqsv_adjoint_1q(tqsv_stream_t s, void* psi, void* lam, int64_t bsz, int n, int tbit, uint64_t cmask,
+                    uint64_t cval, const void* mat, int64_t mat_bstride, int P, float* grad, int64_t grad_stride,
+                    int gslot);
+int tqsv_adjoint_2q(tqsv_stream_t s, void* psi, void* lam, int64_t bsz, int n, int bit_hi, int bit_lo,
+                    uint64_t cmask, uint64_t cval, const void* mat, int64_t mat_bstride, int P, float* grad,
+                    int64_t grad_stride, int gslot);
+int tqsv_adjoint_diag(tqsv_stream_t s, void* psi, void* lam, int64_t bsz, int n, uint64_t cmask, uint64_t cval,
+                      uint64_t pmask, const void* phases, int64_t ph_bstride, int P, float* grad,
+                      int64_t grad_stride, int gslot);
+
+/* ---- expectations ------------------------------------------------------ */
+/* bytes of float workspace needed by the calls below (-1: unsupported n). */
+int64_t tqsv_expval_workspace_bytes(int64_t bsz, int n, int nterms);
+/* <Z_w> for every wire: out[bsz, n].  Replaces expval / MeasureAll with
+ * PauliZ (torchquantum/measurement/measurements.py:282-343). */
+int tqsv_expval_z_all(tqsv_stream_t s, const void* state, int64_t bsz, int n, float* out, void* workspace,
+                      int64_t ws_bytes);
+/* Re<psi|P_t|psi> for Pauli strings given by host masks (x: X/Y positions,
+ * z: Z/Y positions): out[row * out_stride + t].  Replaces
+ * expval_joint_analytical (measurements.py:232-279). */
+int tqsv_expval_pauli(tqsv_stream_t s, const void* state, int64_t bsz, int n, int nterms, const uint64_t* xmask,
+                      const uint64_t* zmask, float* out, int64_t out_stride, void* workspace, int64_t ws_bytes);
+/* out (+)= sum_t coef[row, t] X^x_t Z^z_t psi  (coef: device complex64
+ * [bsz, nterms]); the backward of the expectations (torch autograd through
+ * measurements.py:271-279 and :299-312). */
+int tqsv_pauli_apply(tqsv_stream_t s, void* out, const void* in, int64_t bsz, int n, int nterms,
+                     const uint64_t* xmask, const uint64_t* zmask, const void* coef, int accumulate);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TQSV_H */

@@ -1,0 +1,190 @@
+"""GPU parity: the HIP engine (through the C ABI) against the complex128 oracle.
+
+Bars (written in each test): amplitudes and expectations within 1e-5
+max-abs (complex64 engine vs complex128 oracle, circuits of <= 160 gates);
+gradients against central finite differences of the float64 oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+from circuits import ALL_GATES, apply_ops_tq, oracle_ops, random_ops, config1_ops, config3_observables
+from oracle import statevector as O
+
+gpu = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _qdev(n, bsz, mode, **kw):
+    import torchquantum_amd as tq
+    q = tq.QuantumDevice(n_wires=n, bsz=bsz, device=DEV)
+    q.engine_options = tq.EngineOptions(mode=mode, **kw)
+    return q
+
+
+def _run(n, bsz, ops, mode, init=None, **kw):
+    q = _qdev(n, bsz, mode, **kw)
+    if init is not None:
+        q.set_states(torch.tensor(init, dtype=torch.complex64, device=DEV))
+    apply_ops_tq(q, ops, DEV)
+    return q.get_states_1d().detach().cpu().numpy()
+
+
+@gpu
+@pytest.mark.parametrize("mode", ["pergate", "fused"])
+@pytest.mark.parametrize("name", ALL_GATES)
+def test_each_gate(name, mode):
+    """Every gate, random wires, params, inverse flag, per-row params; 1e-5."""
+    rng = np.random.default_rng(abs(hash((name, mode))) % (2 ** 32))
+    n, bsz = 6, 3
+    init = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+    init /= np.linalg.norm(init, axis=1, keepdims=True)
+    ops = random_ops(n, 4, bsz, rng, names=[name])
+    got = _run(n, bsz, ops, mode, init)
+    ref = O.run_ops(oracle_ops(ops, bsz), n, bsz, state=init)
+    assert np.abs(got - ref).max() < 1e-5, (name, mode)
+
+
+@gpu
+@pytest.mark.parametrize("mode,stage_bits", [("pergate", 0), ("fused", 0), ("fused", 4), ("fused", 6),
+                                             ("fused", 9), ("fused", 13)])
+@pytest.mark.parametrize("n", [4, 7, 11, 14])
+def test_random_circuits(n, mode, stage_bits):
+    """Random mixed circuits (all gate kinds), several stage widths; 1e-5."""
+    rng = np.random.default_rng(1000 * n + stage_bits)
+    bsz = 2
+    ops = random_ops(n, 60, bsz, rng)
+    got = _run(n, bsz, ops, mode, stage_bits=stage_bits)
+    ref = O.run_ops(oracle_ops(ops, bsz), n, bsz)
+    assert np.abs(got - ref).max() < 1e-5
+
+
+@gpu
+@pytest.mark.parametrize("mode", ["pergate", "fused"])
+def test_config1_parity(mode):
+    """10 qubits, bsz 32, 8 x (U3 all + CNOT ring), <Z_i>: amplitudes and expvals 1e-5."""
+    import torchquantum_amd as tq
+    n, bsz = 10, 32
+    ops = config1_ops(seed=0)
+    q = _qdev(n, bsz, mode)
+    apply_ops_tq(q, ops, DEV)
+    ev = tq.MeasureAll(tq.PauliZ)(q).cpu().numpy()
+    ref = O.run_ops(oracle_ops(ops, 1), n, 1)
+    assert np.abs(q.get_states_1d().cpu().numpy() - ref).max() < 1e-5
+    assert np.abs(ev - O.expval_z_all(ref, n)).max() < 1e-5
+
+
+@gpu
+def test_expvals():
+    """MeasureAll(X/Y/Z/H/I), expval_joint_analytical (incl. Y strings) vs oracle, 1e-5."""
+    import torchquantum_amd as tq
+    rng = np.random.default_rng(7)
+    n, bsz = 9, 3
+    ops = random_ops(n, 40, bsz, rng)
+    for obs_cls in (tq.PauliZ, tq.PauliX, tq.PauliY, tq.Hadamard, tq.I):
+        q = _qdev(n, bsz, "fused")
+        apply_ops_tq(q, ops, DEV)
+        ev = tq.MeasureAll(obs_cls)(q).cpu().numpy()
+        st = O.run_ops(oracle_ops(ops, bsz), n, bsz)
+        ref = np.zeros((bsz, n))
+        for w in range(n):
+            ch = {"PauliZ": "Z", "PauliX": "X", "PauliY": "Y", "I": "I"}.get(obs_cls.__name__)
+            if ch is None:   # Hadamard observable: (X + Z)/sqrt(2)
+                sx = ["I"] * n
+                sx[w] = "X"
+                sz = ["I"] * n
+                sz[w] = "Z"
+                ref[:, w] = (O.expval_pauli(st, "".join(sx)) + O.expval_pauli(st, "".join(sz))) / np.sqrt(2)
+            else:
+                s = ["I"] * n
+                s[w] = ch
+                ref[:, w] = O.expval_pauli(st, "".join(s))
+        assert np.abs(ev - ref).max() < 1e-5, obs_cls.__name__
+    q = _qdev(n, bsz, "fused")
+    apply_ops_tq(q, ops, DEV)
+    st = O.run_ops(oracle_ops(ops, bsz), n, bsz)
+    for _ in range(12):
+        obs = "".join(rng.choice(list("IXYZ"), n))
+        got = tq.expval_joint_analytical(q, obs).cpu().numpy()
+        assert np.abs(got - O.expval_pauli(st, obs)).max() < 1e-5, obs
+
+
+@gpu
+@pytest.mark.parametrize("mode", ["pergate", "fused"])
+def test_adjoint_gradients_fd(mode):
+    """d(sum ZZ + 0.5 sum X)/dtheta through random trainable gates vs central FD
+    of the float64 oracle (h = 1e-4); tolerance 2e-4 abs + 1e-3 rel."""
+    import torchquantum_amd as tq
+    rng = np.random.default_rng(11)
+    n, bsz = 8, 2
+    names = ["rx", "ry", "rz", "u3", "rot", "cnot", "cz", "crx", "cry", "crz", "rzz", "rxx", "ryy", "rzx", "cu3",
+             "u1", "phaseshift", "cu1", "h", "swap", "toffoli", "multirz", "xxplusyy", "singleexcitation", "cu",
+             "u2", "r", "crot"]
+    ops = random_ops(n, 50, bsz, rng, names=names)
+    obs, coef = config3_observables(n)
+    q = _qdev(n, bsz, mode)
+    ts = apply_ops_tq(q, ops, DEV, requires_grad=True)
+    loss = tq.expval_pauli_sum(q, obs, coef).sum()
+    loss.backward()
+
+    def f(flat):
+        k = 0
+        o2 = []
+        for op in ops:
+            p = op["params"]
+            if p is not None and not np.iscomplexobj(p):
+                p = flat[k:k + p.size].reshape(p.shape)
+                k += p.size
+            o2.append(dict(op, params=p))
+        st = O.run_ops(oracle_ops(o2, bsz), n, bsz)
+        return float(sum(c * O.expval_pauli(st, o).sum() for o, c in zip(obs, coef)))
+
+    flat = np.concatenate([op["params"].reshape(-1) for op in ops
+                           if op["params"] is not None and not np.iscomplexobj(op["params"])])
+    fd = O.central_fd(f, flat.copy(), h=1e-4)
+    got = np.concatenate([t.grad.detach().cpu().numpy().reshape(-1) for t in ts])
+    assert np.all(np.abs(got - fd) <= 2e-4 + 1e-3 * np.abs(fd)), np.abs(got - fd).max()
+
+
+@gpu
+def test_state_gradient():
+    """Gradient w.r.t. an input state set with set_states (encoder path) vs FD."""
+    import torchquantum_amd as tq
+    rng = np.random.default_rng(3)
+    n, bsz = 6, 2
+    ops = random_ops(n, 20, bsz, rng, names=["rx", "cnot", "u3", "cz", "swap"])
+    init = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+    x = torch.tensor(init, dtype=torch.complex64, device=DEV, requires_grad=True)
+    q = _qdev(n, bsz, "fused")
+    q.set_states(x)
+    apply_ops_tq(q, ops, DEV)
+    obs = "ZXIYZI"
+    tq.expval_joint_analytical(q, obs).sum().backward()
+    g = x.grad.cpu().numpy()
+
+    def f(v):
+        st = O.run_ops(oracle_ops(ops, bsz), n, bsz, state=v)
+        return float(O.expval_pauli(st, obs).sum())
+
+    h = 1e-4
+    for (b, j) in [(0, 3), (1, 17), (0, 40)]:
+        for comp in (1.0, 1j):
+            vp, vm = init.copy(), init.copy()
+            vp[b, j] += h * comp
+            vm[b, j] -= h * comp
+            d = (f(vp) - f(vm)) / (2 * h)
+            gc = g[b, j].real if comp == 1.0 else g[b, j].imag
+            assert abs(gc - d) < 1e-3 * (1 + abs(d))
+
+
+@gpu
+def test_fused_matches_pergate_large():
+    """n = 18, bsz 2, 200 random gates: fused stages vs per-gate kernels, 1e-5."""
+    rng = np.random.default_rng(21)
+    n, bsz = 18, 2
+    names = ["h", "rx", "ry", "rz", "u3", "cnot", "cz", "swap", "crx", "rzz", "toffoli", "cswap", "rxx", "s"]
+    ops = random_ops(n, 200, bsz, rng, names=names)
+    a = _run(n, bsz, ops, "fused")
+    b = _run(n, bsz, ops, "pergate")
+    assert np.abs(a - b).max() < 1e-5

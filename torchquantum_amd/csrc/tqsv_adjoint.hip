@@ -1,0 +1,257 @@
+// Per-gate adjoint-differentiation step kernels (the unfused gradient path).
+//
+// For gate k (walking the circuit backwards) one pass reads psi_k and
+// lam_k and writes
+//     psi_{k-1} = U^dag psi_k,   lam_{k-1} = U^dag lam_k,
+// while accumulating, for every gate parameter p,
+//     g_p += Re < lam_k | (dU/dtheta_p) psi_{k-1} >
+// (lam = torch's gradient of the loss w.r.t. the state, i.e. dL/dRe + i dL/dIm).
+// Gates without trainable parameters need no fused step: their U^dag is two
+// ordinary per-gate launches (tqsv_gates.hip) on psi and lam.
+//
+// Gradients land in grad[row * grad_stride + gslot + p] (one row per batch
+// row; the caller sums rows for parameters shared across the batch).  Each
+// workgroup covers part of one row (grid.y = row), reduces its partials with
+// a wave shuffle tree + LDS and adds them with one atomic per parameter.
+#include "tqsv_common.h"
+
+namespace tqsv {
+int num_cu();
+}
+
+namespace tqsv {
+
+constexpr int AJ_THREADS = 256;
+
+__device__ __forceinline__ void block_accum(float (&g)[4], int P, float* __restrict__ dst) {
+  __shared__ float red[4][AJ_THREADS / 64];
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (q < P) {
+      const float s = wave_sum(g[q], 64);
+      if ((threadIdx.x & 63) == 0) red[q][w] = s;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < (unsigned)P) {
+    float s = 0.f;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[threadIdx.x][i];
+    atomicAdd(dst + threadIdx.x, s);
+  }
+}
+
+template <bool BATCHED>
+__global__ __launch_bounds__(AJ_THREADS) void k_adj_1q(float2* __restrict__ psi, float2* __restrict__ lam, int n,
+                                                      uint64_t items_row, InsertSpec ins, uint64_t cval,
+                                                      uint64_t tmask, const float2* __restrict__ mat,
+                                                      int64_t bstride, int P, float* __restrict__ grad,
+                                                      int64_t gstride, int gslot) {
+  const uint64_t row = blockIdx.y;
+  const float2* M = BATCHED ? mat + row * bstride : mat;
+  const float2 u0 = M[0], u1 = M[1], u2 = M[2], u3 = M[3];
+  float g[4] = {0.f, 0.f, 0.f, 0.f};
+  const uint64_t rbase = row << n;
+  for (uint64_t it = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; it < items_row;
+       it += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i0 = rbase + (insert_zeros(it, ins) | cval);
+    const uint64_t i1 = i0 | tmask;
+    const float2 x0 = psi[i0], x1 = psi[i1];
+    const float2 l0 = lam[i0], l1 = lam[i1];
+    const float2 y0 = cfmac(u2, x1, cmulc(u0, x0));
+    const float2 y1 = cfmac(u3, x1, cmulc(u1, x0));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < P) {
+        const float2* D = M + 4 * (q + 1);
+        const float2 v0 = cfma(D[1], y1, cmul(D[0], y0));
+        const float2 v1 = cfma(D[3], y1, cmul(D[2], y0));
+        g[q] += redot(l0, v0) + redot(l1, v1);
+      }
+    }
+    psi[i0] = y0;
+    psi[i1] = y1;
+    lam[i0] = cfmac(u2, l1, cmulc(u0, l0));
+    lam[i1] = cfmac(u3, l1, cmulc(u1, l0));
+  }
+  block_accum(g, P, grad + row * gstride + gslot);
+}
+
+template <bool BATCHED>
+__global__ __launch_bounds__(AJ_THREADS) void k_adj_2q(float2* __restrict__ psi, float2* __restrict__ lam, int n,
+                                                      uint64_t items_row, InsertSpec ins, uint64_t cval,
+                                                      uint64_t mhi, uint64_t mlo, const float2* __restrict__ mat,
+                                                      int64_t bstride, int P, float* __restrict__ grad,
+                                                      int64_t gstride, int gslot) {
+  const uint64_t row = blockIdx.y;
+  const float2* M = BATCHED ? mat + row * bstride : mat;
+  float g[4] = {0.f, 0.f, 0.f, 0.f};
+  const uint64_t rbase = row << n;
+  for (uint64_t it = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; it < items_row;
+       it += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = rbase + (insert_zeros(it, ins) | cval);
+    const uint64_t idx[4] = {b, b | mlo, b | mhi, b | mhi | mlo};
+    float2 x[4], l[4], y[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { x[r] = psi[idx[r]]; l[r] = lam[idx[r]]; }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float2 acc = cmulc(M[r], x[0]);
+#pragma unroll
+      for (int c = 1; c < 4; ++c) acc = cfmac(M[4 * c + r], x[c], acc);
+      y[r] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < P) {
+        const float2* D = M + 16 * (q + 1);
+        float t = 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float2 v = cmul(D[4 * r], y[0]);
+#pragma unroll
+          for (int c = 1; c < 4; ++c) v = cfma(D[4 * r + c], y[c], v);
+          t += redot(l[r], v);
+        }
+        g[q] += t;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float2 acc = cmulc(M[r], l[0]);
+#pragma unroll
+      for (int c = 1; c < 4; ++c) acc = cfmac(M[4 * c + r], l[c], acc);
+      psi[idx[r]] = y[r];
+      lam[idx[r]] = acc;
+    }
+  }
+  block_accum(g, P, grad + row * gstride + gslot);
+}
+
+template <bool BATCHED>
+__global__ __launch_bounds__(AJ_THREADS) void k_adj_diag(float2* __restrict__ psi, float2* __restrict__ lam, int n,
+                                                        uint64_t items_row, InsertSpec ins, uint64_t cval,
+                                                        uint64_t pmask, const float2* __restrict__ ph,
+                                                        int64_t bstride, int P, float* __restrict__ grad,
+                                                        int64_t gstride, int gslot) {
+  const uint64_t row = blockIdx.y;
+  const float2* M = BATCHED ? ph + row * bstride : ph;
+  const float2 p0 = M[0], p1 = M[1];
+  float g[4] = {0.f, 0.f, 0.f, 0.f};
+  const uint64_t rbase = row << n;
+  for (uint64_t it = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; it < items_row;
+       it += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t j = insert_zeros(it, ins) | cval;
+    const uint64_t i = rbase + j;
+    const int par = __popcll(j & pmask) & 1;
+    const float2 f = par ? p1 : p0;
+    const float2 y = cmulc(f, psi[i]);
+    const float2 l = lam[i];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < P) {
+        const float2 df = par ? M[2 * (q + 1) + 1] : M[2 * (q + 1)];
+        g[q] += redot(l, cmul(df, y));
+      }
+    }
+    psi[i] = y;
+    lam[i] = cmulc(f, l);
+  }
+  block_accum(g, P, grad + row * gstride + gslot);
+}
+
+static int build_insert_adj(InsertSpec& s, uint64_t mask, int n) {
+  s.nb = 0;
+  for (int p = 0; p < n; ++p)
+    if ((mask >> p) & 1ull) {
+      if (s.nb >= 12) return 0;
+      s.pos[s.nb++] = p;
+    }
+  return 1;
+}
+
+static dim3 adj_grid(uint64_t items_row, int64_t bsz) {
+  uint64_t want = ((uint64_t)num_cu() * 8 + bsz - 1) / bsz;
+  uint64_t need = (items_row + AJ_THREADS - 1) / AJ_THREADS;
+  uint64_t gx = want < need ? want : need;
+  if (gx == 0) gx = 1;
+  return dim3((unsigned)gx, (unsigned)bsz, 1);
+}
+
+}  // namespace tqsv
+
+using namespace tqsv;
+
+static int adj_common(void* psi, void* lam, int64_t bsz, int n, const void* mat, int P, float* grad) {
+  if (!psi || !lam || !mat || (P > 0 && !grad)) { set_error("adjoint: null pointer"); return E_ARG; }
+  if (bsz < 1 || bsz > 65535 || n < 1 || n > 40) { set_error("adjoint: bad bsz/n (bsz <= 65535)"); return E_ARG; }
+  if (P < 0 || P > 4) { set_error("adjoint: 0..4 parameters per gate (got %d)", P); return E_LIMIT; }
+  return OK;
+}
+
+// mat: U (4 complex) followed by P derivative matrices dU/dtheta_p (4 each).
+TQSV_API int tqsv_adjoint_1q(hipStream_t s, void* psi, void* lam, int64_t bsz, int n, int tbit, uint64_t cmask,
+                             uint64_t cval, const void* mat, int64_t mat_bstride, int P, float* grad,
+                             int64_t grad_stride, int gslot) {
+  int rc = adj_common(psi, lam, bsz, n, mat, P, grad);
+  if (rc) return rc;
+  if (tbit < 0 || tbit >= n) { set_error("adjoint_1q: target out of range"); return E_ARG; }
+  const uint64_t tmask = 1ull << tbit;
+  if ((cmask & tmask) || (n < 64 && (cmask >> n)) || (cval & ~cmask)) { set_error("adjoint_1q: bad control mask"); return E_ARG; }
+  InsertSpec ins;
+  if (!build_insert_adj(ins, cmask | tmask, n)) { set_error("adjoint_1q: too many controls"); return E_LIMIT; }
+  const uint64_t items_row = 1ull << (n - 1 - __builtin_popcountll(cmask));
+  const dim3 grid = adj_grid(items_row, bsz);
+  if (mat_bstride)
+    k_adj_1q<true><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, tmask,
+                                               (const float2*)mat, mat_bstride, P, grad, grad_stride, gslot);
+  else
+    k_adj_1q<false><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, tmask,
+                                                (const float2*)mat, 0, P, grad, grad_stride, gslot);
+  return check_launch("tqsv_adjoint_1q");
+}
+
+// mat: U (16 complex, index 2*b(hi)+b(lo)) followed by P derivative matrices.
+TQSV_API int tqsv_adjoint_2q(hipStream_t s, void* psi, void* lam, int64_t bsz, int n, int bit_hi, int bit_lo,
+                             uint64_t cmask, uint64_t cval, const void* mat, int64_t mat_bstride, int P,
+                             float* grad, int64_t grad_stride, int gslot) {
+  int rc = adj_common(psi, lam, bsz, n, mat, P, grad);
+  if (rc) return rc;
+  if (bit_hi < 0 || bit_hi >= n || bit_lo < 0 || bit_lo >= n || bit_hi == bit_lo) {
+    set_error("adjoint_2q: bad target bits");
+    return E_ARG;
+  }
+  const uint64_t mhi = 1ull << bit_hi, mlo = 1ull << bit_lo;
+  if ((cmask & (mhi | mlo)) || (n < 64 && (cmask >> n)) || (cval & ~cmask)) { set_error("adjoint_2q: bad control mask"); return E_ARG; }
+  InsertSpec ins;
+  if (!build_insert_adj(ins, cmask | mhi | mlo, n)) { set_error("adjoint_2q: too many controls"); return E_LIMIT; }
+  const uint64_t items_row = 1ull << (n - 2 - __builtin_popcountll(cmask));
+  const dim3 grid = adj_grid(items_row, bsz);
+  if (mat_bstride)
+    k_adj_2q<true><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, mhi, mlo,
+                                               (const float2*)mat, mat_bstride, P, grad, grad_stride, gslot);
+  else
+    k_adj_2q<false><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, mhi, mlo,
+                                                (const float2*)mat, 0, P, grad, grad_stride, gslot);
+  return check_launch("tqsv_adjoint_2q");
+}
+
+// phases: (phi_even, phi_odd) followed by P derivative pairs.
+TQSV_API int tqsv_adjoint_diag(hipStream_t s, void* psi, void* lam, int64_t bsz, int n, uint64_t cmask,
+                               uint64_t cval, uint64_t pmask, const void* phases, int64_t ph_bstride, int P,
+                               float* grad, int64_t grad_stride, int gslot) {
+  int rc = adj_common(psi, lam, bsz, n, phases, P, grad);
+  if (rc) return rc;
+  if ((n < 64 && ((cmask | pmask) >> n)) || (cval & ~cmask) || (cmask & pmask)) { set_error("adjoint_diag: bad masks"); return E_ARG; }
+  InsertSpec ins;
+  if (!build_insert_adj(ins, cmask, n)) { set_error("adjoint_diag: too many controls"); return E_LIMIT; }
+  const uint64_t items_row = 1ull << (n - __builtin_popcountll(cmask));
+  const dim3 grid = adj_grid(items_row, bsz);
+  if (ph_bstride)
+    k_adj_diag<true><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, pmask,
+                                                 (const float2*)phases, ph_bstride, P, grad, grad_stride, gslot);
+  else
+    k_adj_diag<false><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, pmask,
+                                                  (const float2*)phases, 0, P, grad, grad_stride, gslot);
+  return check_launch("tqsv_adjoint_diag");
+}

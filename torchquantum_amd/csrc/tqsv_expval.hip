@@ -1,0 +1,347 @@
+// Pauli expectation kernels (one read pass of the state per distinct flip mask).
+//
+// Replaces:
+//   * expval / MeasureAll (torchquantum/measurement/measurements.py:282-343):
+//     |psi|^2 then one full reduction per wire -> here one pass computes
+//     <Z_w> for every wire: <Z_w> = sum_j p_j - 2 * sum_{j: bit(w)=1} p_j;
+//   * expval_joint_analytical (measurements.py:232-279), which builds a dense
+//     2^n x 2^n Kronecker Hamiltonian (O(4^n) memory) -> here a Pauli string
+//     P = i^{nY} X^x Z^z is applied through its flip mask x and phase mask z:
+//     <psi|P|psi> = i^{nY} sum_j (-1)^{popcount(j & z)} conj(psi_{j^x}) psi_j.
+// Reductions: per-thread partial sums -> 64-lane shuffle tree -> LDS across
+// waves -> one partial per (row, chunk, term) -> a second tiny kernel sums
+// the chunks in float64.  No float atomics, so results are deterministic.
+#include "tqsv_common.h"
+
+namespace tqsv {
+int num_cu();
+}
+
+namespace tqsv {
+
+constexpr int EV_THREADS = 256;
+constexpr int EV_MAXQ = 32;      // wires per <Z_w> launch (n <= 32)
+constexpr int EV_MAXT = 16;      // terms per diagonal-term launch
+
+// chunks per row so that the grid has enough blocks; chunk = power of two amps
+static int ev_chunk_log2(int64_t bsz, int n) {
+  // at least 2 amps per thread per chunk, at most the whole row
+  int lg = n;
+  const int min_lg = 9 + 2;  // 256 threads * 2 amps * 4 iterations
+  const int64_t want_blocks = (int64_t)num_cu() * 8;
+  while (lg > min_lg && (bsz << (n - lg)) < want_blocks) --lg;
+  if (lg > n) lg = n;
+  return lg;
+}
+
+__device__ __forceinline__ float block_sum(float v, float* red) {
+  // red: >= EV_THREADS / 64 floats of LDS
+  v = wave_sum(v, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  float s = 0.f;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+  return s;  // valid in thread 0
+}
+
+// <Z_w> partials: part[(row * nchunks + chunk) * (n + 1) + q], q < n: sum of p
+// over amplitudes with bit q set; q = n: total probability.
+__global__ __launch_bounds__(EV_THREADS) void k_ez_part(const float4* __restrict__ st, int n, int lg_chunk,
+                                                        float* __restrict__ part) {
+  __shared__ float red[EV_THREADS / 64];
+  const int nchunks = 1 << (n - lg_chunk);
+  const uint64_t row = blockIdx.x >> (n - lg_chunk);
+  const uint64_t chunk = blockIdx.x & (nchunks - 1);
+  const uint64_t first = (row << n) + (chunk << lg_chunk);  // amplitude index
+  const uint64_t npairs = (1ull << lg_chunk) >> 1;
+  float S[EV_MAXQ];
+#pragma unroll
+  for (int q = 0; q < EV_MAXQ; ++q) S[q] = 0.f;
+  float tot = 0.f;
+  for (uint64_t i = threadIdx.x; i < npairs; i += blockDim.x) {
+    const float4 v = st[(first >> 1) + i];
+    const float p0 = v.x * v.x + v.y * v.y;
+    const float p1 = v.z * v.z + v.w * v.w;
+    const uint32_t j = (uint32_t)(((chunk << lg_chunk) + 2 * i));  // in-row index of p0 (n <= 32)
+    const float pp = p0 + p1;
+    tot += pp;
+    S[0] += p1;
+#pragma unroll
+    for (int q = 1; q < EV_MAXQ; ++q)
+      if (q < n) S[q] += ((j >> q) & 1u) ? pp : 0.f;
+  }
+  float* out = part + (uint64_t)blockIdx.x * (n + 1);
+#pragma unroll
+  for (int q = 0; q < EV_MAXQ; ++q) {
+    if (q < n) {
+      const float s = block_sum(S[q], red);
+      if (threadIdx.x == 0) out[q] = s;
+    }
+  }
+  const float s = block_sum(tot, red);
+  if (threadIdx.x == 0) out[n] = s;
+}
+
+// out[row * n + w] = total - 2 * S[n - 1 - w]
+__global__ void k_ez_final(const float* __restrict__ part, int n, int nchunks, int64_t bsz, float* __restrict__ out) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= bsz * n) return;
+  const int64_t row = idx / n;
+  const int w = (int)(idx % n);
+  const int q = n - 1 - w;
+  double s = 0.0, t = 0.0;
+  for (int c = 0; c < nchunks; ++c) {
+    const float* p = part + (row * nchunks + c) * (n + 1);
+    s += p[q];
+    t += p[n];
+  }
+  out[idx] = (float)(t - 2.0 * s);
+}
+
+// Diagonal Pauli strings (x = 0): sum_j p_j (-1)^{popcount(j & z_t)}, up to
+// EV_MAXT terms per launch.  part[(row*nchunks+chunk)*nt + t].
+struct ZMasks {
+  int nt;
+  uint32_t z[EV_MAXT];
+};
+
+__global__ __launch_bounds__(EV_THREADS) void k_ediag_part(const float2* __restrict__ st, int n, int lg_chunk,
+                                                           ZMasks zm, float* __restrict__ part) {
+  __shared__ float red[EV_THREADS / 64];
+  const int nchunks = 1 << (n - lg_chunk);
+  const uint64_t row = blockIdx.x >> (n - lg_chunk);
+  const uint64_t chunk = blockIdx.x & (nchunks - 1);
+  const uint64_t first = (row << n) + (chunk << lg_chunk);
+  const uint64_t cnt = 1ull << lg_chunk;
+  float acc[EV_MAXT];
+#pragma unroll
+  for (int t = 0; t < EV_MAXT; ++t) acc[t] = 0.f;
+  for (uint64_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+    const float2 v = st[first + i];
+    const float p = v.x * v.x + v.y * v.y;
+    const uint32_t j = (uint32_t)((chunk << lg_chunk) + i);
+#pragma unroll
+    for (int t = 0; t < EV_MAXT; ++t)
+      if (t < zm.nt) acc[t] += (__popc(j & zm.z[t]) & 1) ? -p : p;
+  }
+  float* out = part + (uint64_t)blockIdx.x * zm.nt;
+#pragma unroll
+  for (int t = 0; t < EV_MAXT; ++t) {
+    if (t < zm.nt) {
+      const float s = block_sum(acc[t], red);
+      if (threadIdx.x == 0) out[t] = s;
+    }
+  }
+}
+
+// One off-diagonal string (x != 0): pairs (j, j ^ x) with j's top x-bit clear.
+// Partial of sum_j s_j conj(psi_{j^x}) psi_j (complex; times i^{nY} on host).
+__global__ __launch_bounds__(EV_THREADS) void k_ex_part(const float2* __restrict__ st, int n, int lg_chunk,
+                                                        uint32_t x, uint32_t z, int top, float* __restrict__ part) {
+  __shared__ float red[EV_THREADS / 64];
+  // items: in-row indices with bit `top` clear; chunk over items
+  const int lg_items = n - 1;
+  const int lgc = lg_chunk > lg_items ? lg_items : lg_chunk;
+  const int nchunks = 1 << (lg_items - lgc);
+  const uint64_t row = blockIdx.x >> (lg_items - lgc);
+  const uint64_t chunk = blockIdx.x & (nchunks - 1);
+  const uint64_t cnt = 1ull << lgc;
+  const float2* r = st + (row << n);
+  float ar = 0.f, ai = 0.f;
+  for (uint64_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+    const uint32_t it = (uint32_t)((chunk << lgc) + i);
+    const uint32_t lo = it & ((1u << top) - 1u);
+    const uint32_t j = ((it >> top) << (top + 1)) | lo;
+    const uint32_t k = j ^ x;
+    const float2 a = r[j], b = r[k];
+    const float sj = (__popc(j & z) & 1) ? -1.f : 1.f;
+    const float sk = (__popc(k & z) & 1) ? -1.f : 1.f;
+    // sj * conj(b) * a + sk * conj(a) * b
+    const float2 c1 = cmulc(b, a);
+    const float2 c2 = cmulc(a, b);
+    ar += sj * c1.x + sk * c2.x;
+    ai += sj * c1.y + sk * c2.y;
+  }
+  float* out = part + (uint64_t)blockIdx.x * 2;
+  const float sr = block_sum(ar, red);
+  if (threadIdx.x == 0) out[0] = sr;
+  const float si = block_sum(ai, red);
+  if (threadIdx.x == 0) out[1] = si;
+}
+
+// Sum per-chunk partials of one term: partial of (row, chunk) sits at
+// part[(row * nchunks + chunk) * cstride] (complex: two floats re, im).
+// out[row * ostride + ooff] = scale_re * Re(sum) - scale_im * Im(sum).
+__global__ void k_sum_chunks(const float* __restrict__ part, int nchunks, int cstride, int is_complex,
+                             int64_t bsz, float scale_re, float scale_im, float* __restrict__ out,
+                             int64_t ostride, int ooff) {
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= bsz) return;
+  double sr = 0.0, si = 0.0;
+  for (int c = 0; c < nchunks; ++c) {
+    const float* p = part + (row * nchunks + c) * cstride;
+    sr += p[0];
+    if (is_complex) si += p[1];
+  }
+  out[row * ostride + ooff] = (float)(scale_re * sr - scale_im * si);
+}
+
+// out_j (+)= sum_t coef[row, t] * (-1)^{popcount((j^x_t) & z_t)} * psi_{j ^ x_t}
+struct PTerms {
+  int nt;
+  uint32_t x[EV_MAXT];
+  uint32_t z[EV_MAXT];
+};
+
+__global__ __launch_bounds__(EV_THREADS) void k_pauli_apply(float2* __restrict__ out, const float2* __restrict__ in,
+                                                            int n, uint64_t total, PTerms pt,
+                                                            const float2* __restrict__ coef, int64_t coef_stride,
+                                                            int coef_off, int accumulate) {
+  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t row = g >> n;
+    const uint32_t j = (uint32_t)(g & ((1ull << n) - 1ull));
+    const float2* r = in + (row << n);
+    const float2* cf = coef + row * coef_stride + coef_off;
+    float2 acc = accumulate ? out[g] : make_float2(0.f, 0.f);
+    for (int t = 0; t < pt.nt; ++t) {
+      const uint32_t k = j ^ pt.x[t];
+      const float2 v = r[k];
+      const float2 c = cf[t];
+      const float2 cv = cmul(c, v);
+      const bool neg = __popc(k & pt.z[t]) & 1;
+      acc.x += neg ? -cv.x : cv.x;
+      acc.y += neg ? -cv.y : cv.y;
+    }
+    out[g] = acc;
+  }
+}
+
+}  // namespace tqsv
+
+using namespace tqsv;
+
+static unsigned grid_cap(uint64_t work) {
+  uint64_t b = (work + EV_THREADS - 1) / EV_THREADS;
+  uint64_t cap = (uint64_t)num_cu() * 16;
+  if (b > cap) b = cap;
+  return b ? (unsigned)b : 1u;
+}
+
+// Bytes of float workspace the expectation entry points need.
+TQSV_API int64_t tqsv_expval_workspace_bytes(int64_t bsz, int n, int nterms) {
+  if (bsz < 1 || n < 1 || n > 32) return -1;
+  const int lg = ev_chunk_log2(bsz, n);
+  const int64_t blocks = bsz << (n - lg);
+  int64_t per = (int64_t)(n + 1);
+  if (nterms * 2 > per) per = nterms * 2;
+  if (EV_MAXT * 2 > per) per = EV_MAXT * 2;
+  return blocks * per * (int64_t)sizeof(float) * 2;
+}
+
+// <Z_w> for all wires w: out[bsz, n] (wire order as in the reference: out[:, w]).
+TQSV_API int tqsv_expval_z_all(hipStream_t s, const void* state, int64_t bsz, int n, float* out, void* workspace,
+                               int64_t ws_bytes) {
+  if (!state || !out || !workspace) { set_error("tqsv_expval_z_all: null pointer"); return E_ARG; }
+  if (bsz < 1 || n < 1 || n > EV_MAXQ) { set_error("tqsv_expval_z_all: need 1 <= n <= %d", EV_MAXQ); return E_ARG; }
+  const int lg = ev_chunk_log2(bsz, n);
+  const int64_t blocks = bsz << (n - lg);
+  if (ws_bytes < blocks * (n + 1) * (int64_t)sizeof(float)) { set_error("tqsv_expval_z_all: workspace too small"); return E_ARG; }
+  k_ez_part<<<(unsigned)blocks, EV_THREADS, 0, s>>>((const float4*)state, n, lg, (float*)workspace);
+  int rc = check_launch("tqsv_expval_z_all/part");
+  if (rc) return rc;
+  const int64_t outs = bsz * n;
+  k_ez_final<<<(unsigned)((outs + 255) / 256), 256, 0, s>>>((const float*)workspace, n, 1 << (n - lg), bsz, out);
+  return check_launch("tqsv_expval_z_all/final");
+}
+
+// Re <psi|P_t|psi> for nterms Pauli strings given by bit masks (in-row bit
+// positions; host arrays).  out[row * out_stride + t].
+TQSV_API int tqsv_expval_pauli(hipStream_t s, const void* state, int64_t bsz, int n, int nterms,
+                               const uint64_t* xmask, const uint64_t* zmask, float* out, int64_t out_stride,
+                               void* workspace, int64_t ws_bytes) {
+  if (!state || !out || !workspace || (nterms > 0 && (!xmask || !zmask))) {
+    set_error("tqsv_expval_pauli: null pointer");
+    return E_ARG;
+  }
+  if (bsz < 1 || n < 1 || n > 32 || nterms < 0) { set_error("tqsv_expval_pauli: need 1 <= n <= 32"); return E_ARG; }
+  const int lg = ev_chunk_log2(bsz, n);
+  const int64_t nchunks = 1ll << (n - lg);
+  const int64_t blocks = bsz * nchunks;
+  // diagonal terms, batched EV_MAXT at a time (gather indices first)
+  int idx[EV_MAXT];
+  int cnt = 0;
+  for (int t = 0; t <= nterms; ++t) {
+    const bool flush = (t == nterms) || cnt == EV_MAXT;
+    if (flush && cnt > 0) {
+      if (ws_bytes < blocks * cnt * (int64_t)sizeof(float)) { set_error("tqsv_expval_pauli: workspace too small"); return E_ARG; }
+      ZMasks zm;
+      zm.nt = cnt;
+      for (int q = 0; q < cnt; ++q) zm.z[q] = (uint32_t)zmask[idx[q]];
+      k_ediag_part<<<(unsigned)blocks, EV_THREADS, 0, s>>>((const float2*)state, n, lg, zm, (float*)workspace);
+      int rc = check_launch("tqsv_expval_pauli/diag");
+      if (rc) return rc;
+      // contiguous output indices? scatter one term at a time
+      for (int q = 0; q < cnt; ++q) {
+        // sum chunk partials of term q: treat part as [row, chunk, cnt]
+        k_sum_chunks<<<(unsigned)((bsz + 255) / 256), 256, 0, s>>>(
+            (const float*)workspace + q, (int)nchunks, cnt, 0, bsz, 1.f, 0.f, out, out_stride, idx[q]);
+        if ((rc = check_launch("tqsv_expval_pauli/sum"))) return rc;
+      }
+      cnt = 0;
+    }
+    if (t == nterms) break;
+    if (n < 64 && ((xmask[t] | zmask[t]) >> n)) { set_error("tqsv_expval_pauli: mask beyond n bits"); return E_ARG; }
+    if (xmask[t] == 0) idx[cnt++] = t;
+  }
+  // off-diagonal terms, one pass each
+  for (int t = 0; t < nterms; ++t) {
+    if (xmask[t] == 0) continue;
+    const uint32_t x = (uint32_t)xmask[t], z = (uint32_t)zmask[t];
+    const int top = 31 - __builtin_clz(x);
+    const int ny = __builtin_popcount(x & z);
+    // i^{ny}: (re, im) scale
+    float sre = 0.f, sim = 0.f;
+    switch (ny & 3) { case 0: sre = 1.f; break; case 1: sim = 1.f; break; case 2: sre = -1.f; break; default: sim = -1.f; }
+    const int lgi = n - 1;
+    const int lgc = lg > lgi ? lgi : lg;
+    const int64_t nch = 1ll << (lgi - lgc);
+    const int64_t blk = bsz * nch;
+    if (ws_bytes < blk * 2 * (int64_t)sizeof(float)) { set_error("tqsv_expval_pauli: workspace too small"); return E_ARG; }
+    k_ex_part<<<(unsigned)blk, EV_THREADS, 0, s>>>((const float2*)state, n, lg, x, z, top, (float*)workspace);
+    int rc = check_launch("tqsv_expval_pauli/x");
+    if (rc) return rc;
+    // Re(i^ny * S) = sre*Re(S) - sim*Im(S)
+    k_sum_chunks<<<(unsigned)((bsz + 255) / 256), 256, 0, s>>>((const float*)workspace, (int)nch, 2, 1, bsz, sre,
+                                                               sim, out, out_stride, t);
+    if ((rc = check_launch("tqsv_expval_pauli/xsum"))) return rc;
+  }
+  return OK;
+}
+
+// out (+)= sum_t coef[row, t] P_t psi   (coef: device complex64 [bsz, nterms],
+// already including i^{nY}); used for d/d psi of Pauli expectations.
+TQSV_API int tqsv_pauli_apply(hipStream_t s, void* out, const void* in, int64_t bsz, int n, int nterms,
+                              const uint64_t* xmask, const uint64_t* zmask, const void* coef, int accumulate) {
+  if (!out || !in || !coef || (nterms > 0 && (!xmask || !zmask))) { set_error("tqsv_pauli_apply: null pointer"); return E_ARG; }
+  if (out == in) { set_error("tqsv_pauli_apply: out must not alias in"); return E_ARG; }
+  if (bsz < 1 || n < 1 || n > 32 || nterms < 1) { set_error("tqsv_pauli_apply: bad sizes"); return E_ARG; }
+  const uint64_t total = (uint64_t)bsz << n;
+  for (int t0 = 0; t0 < nterms; t0 += EV_MAXT) {
+    PTerms pt;
+    pt.nt = nterms - t0 < EV_MAXT ? nterms - t0 : EV_MAXT;
+    for (int q = 0; q < pt.nt; ++q) {
+      if (n < 64 && ((xmask[t0 + q] | zmask[t0 + q]) >> n)) { set_error("tqsv_pauli_apply: mask beyond n bits"); return E_ARG; }
+      pt.x[q] = (uint32_t)xmask[t0 + q];
+      pt.z[q] = (uint32_t)zmask[t0 + q];
+    }
+    k_pauli_apply<<<grid_cap(total), EV_THREADS, 0, s>>>((float2*)out, (const float2*)in, n, total, pt,
+                                                         (const float2*)coef, nterms, t0, (accumulate || t0 > 0) ? 1 : 0);
+    int rc = check_launch("tqsv_pauli_apply");
+    if (rc) return rc;
+  }
+  return OK;
+}

@@ -1,0 +1,601 @@
+// Fused "stage" kernels: one launch applies a whole run of gates to every
+// batch row in a single read + write of the state.
+//
+// The reference applies gates one at a time (gate_wrapper.py:96-140); its own
+// fusion ("static mode", torchquantum/graph/graphs.py:384-532, 836-860)
+// multiplies a k-wire block into one 2^k x 2^k matrix and then still applies
+// it with the per-gate bmm.  Here instead:
+//
+//   * a stage owns m "local" bits of the in-row index (m <= 13); a workgroup
+//     processes one group of 2^m amplitudes (all values of the local bits,
+//     the other bits fixed) at a time, in a persistent loop over groups;
+//   * each thread keeps NS = 16 amplitudes in VGPRs: R = 4 local bits are
+//     "register bits" (the slot index), the other T = m - 4 bits are the
+//     thread index.  Gates whose mixing targets are register bits run with no
+//     communication at all; control bits and diagonal (phase) gates may sit
+//     on any local bit;
+//   * a REMAP op re-assigns which local bits are register bits by one
+//     LDS write + read of the group (XOR-swizzled to spread banks);
+//   * the canonical layout used for the global load/store puts the lowest
+//     local bits on the thread index, so lanes read consecutive addresses.
+//
+// The adjoint kernel walks the same program backwards on two states at once
+// (psi: the forward output being un-computed, lam: d loss / d conj(psi) in
+// torch's convention), applying U^dagger to both and accumulating, for every
+// parameter p of a gate, Re <lam | dU/dtheta_p psi_prev> -- the adjoint
+// differentiation method, with O(1) extra states instead of autograd's saved
+// per-gate tensors (the reference's backward, gate_wrapper.py:127-136).
+#include "tqsv_common.h"
+
+namespace tqsv {
+int num_cu();
+}
+
+namespace tqsv {
+
+constexpr int R = 4;
+constexpr int NS = 1 << R;
+constexpr int MAX_GP = 1024;  // grad params per stage (LDS accumulators)
+
+enum StageOp { OP_MAT1 = 1, OP_MAT2 = 2, OP_DIAG = 3, OP_MATK = 4, OP_REMAP = 5 };
+
+// stage header words
+enum Hdr {
+  H_NOPS = 0, H_M = 1, H_T = 2, H_NNL = 3, H_OFF_OPS = 4, H_NGP = 5, H_OFF_GSLOT = 6, H_FLAGS = 7, H_POS = 8
+};
+
+__device__ __forceinline__ int swz(int l) { return l ^ ((l >> 5) & 31) ^ ((l >> 10) & 31); }
+
+// ------------------------------------------------------------------ forward
+template <int K>
+__device__ __forceinline__ void f_mat1(float2 (&a)[NS], const float2* __restrict__ M, bool tok, int rm, int rv) {
+  const float2 u0 = M[0], u1 = M[1], u2 = M[2], u3 = M[3];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    if (j & (1 << K)) continue;
+    const int j1 = j | (1 << K);
+    const bool ok = tok && ((j & rm) == rv);
+    const float2 x0 = a[j], x1 = a[j1];
+    const float2 y0 = cfma(u1, x1, cmul(u0, x0));
+    const float2 y1 = cfma(u3, x1, cmul(u2, x0));
+    a[j] = csel(ok, y0, x0);
+    a[j1] = csel(ok, y1, x1);
+  }
+}
+
+template <int KH, int KL>
+__device__ __forceinline__ void f_mat2(float2 (&a)[NS], const float2* __restrict__ M, bool tok, int rm, int rv) {
+  float2 u[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) u[q] = M[q];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    if (j & ((1 << KH) | (1 << KL))) continue;
+    const int idx[4] = {j, j | (1 << KL), j | (1 << KH), j | (1 << KH) | (1 << KL)};
+    const bool ok = tok && ((j & rm) == rv);
+    float2 x[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x[r] = a[idx[r]];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float2 acc = cmul(u[4 * r], x[0]);
+#pragma unroll
+      for (int c = 1; c < 4; ++c) acc = cfma(u[4 * r + c], x[c], acc);
+      a[idx[r]] = csel(ok, acc, x[r]);
+    }
+  }
+}
+
+// dense op on KB = 3 or 4 register bits; EX = the excluded register bit for
+// KB = 3 (matrix index bit q <-> q-th remaining register bit, ascending).
+template <int KB, int EX>
+__device__ __forceinline__ int matk_slot(int base, int r) {
+  int s = base;
+  int q = 0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if (KB == 3 && k == EX) continue;
+    if ((r >> q) & 1) s |= 1 << k;
+    ++q;
+  }
+  return s;
+}
+
+template <int KB, int EX>
+__device__ __forceinline__ void f_matk(float2 (&a)[NS], const float2* __restrict__ M, bool tok) {
+  constexpr int D = 1 << KB;
+#pragma unroll
+  for (int base = 0; base < NS; ++base) {
+    if (KB == 4 && base != 0) continue;
+    if (KB == 3 && (base != 0 && base != (1 << EX))) continue;
+    float2 x[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) x[r] = a[matk_slot<KB, EX>(base, r)];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      float2 acc = cmul(M[r * D], x[0]);
+#pragma unroll
+      for (int c = 1; c < D; ++c) acc = cfma(M[r * D + c], x[c], acc);
+      a[matk_slot<KB, EX>(base, r)] = csel(tok, acc, x[r]);
+    }
+  }
+}
+
+__device__ __forceinline__ void f_diag(float2 (&a)[NS], const float2* __restrict__ M, bool tok, int rm, int rv,
+                                       int tpar, int rpm) {
+  const float2 p0 = M[0], p1 = M[1];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const bool ok = tok && ((j & rm) == rv);
+    const int par = (tpar + __popc(j & rpm)) & 1;
+    const float2 f = par ? p1 : p0;
+    a[j] = csel(ok, cmul(f, a[j]), a[j]);
+  }
+}
+
+// ------------------------------------------------------------------ adjoint
+// psi <- U^dag psi ; g_p += Re<lam | D_p psi> ; lam <- U^dag lam
+template <int K>
+__device__ __forceinline__ void b_mat1(float2 (&p)[NS], float2 (&l)[NS], const float2* __restrict__ M, int P,
+                                       bool tok, int rm, int rv, float (&g)[4]) {
+  const float2 u0 = M[0], u1 = M[1], u2 = M[2], u3 = M[3];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    if (j & (1 << K)) continue;
+    const int j1 = j | (1 << K);
+    const bool ok = tok && ((j & rm) == rv);
+    const float2 x0 = p[j], x1 = p[j1];
+    const float2 y0 = cfmac(u2, x1, cmulc(u0, x0));
+    const float2 y1 = cfmac(u3, x1, cmulc(u1, x0));
+    const float2 l0 = l[j], l1 = l[j1];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < P) {
+        const float2* D = M + 4 * (q + 1);
+        const float2 v0 = cfma(D[1], y1, cmul(D[0], y0));
+        const float2 v1 = cfma(D[3], y1, cmul(D[2], y0));
+        const float t = redot(l0, v0) + redot(l1, v1);
+        g[q] += ok ? t : 0.f;
+      }
+    }
+    const float2 m0 = cfmac(u2, l1, cmulc(u0, l0));
+    const float2 m1 = cfmac(u3, l1, cmulc(u1, l0));
+    p[j] = csel(ok, y0, x0);
+    p[j1] = csel(ok, y1, x1);
+    l[j] = csel(ok, m0, l0);
+    l[j1] = csel(ok, m1, l1);
+  }
+}
+
+template <int KH, int KL>
+__device__ __forceinline__ void b_mat2(float2 (&p)[NS], float2 (&l)[NS], const float2* __restrict__ M, int P,
+                                       bool tok, int rm, int rv, float (&g)[4]) {
+  float2 u[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) u[q] = M[q];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    if (j & ((1 << KH) | (1 << KL))) continue;
+    const int idx[4] = {j, j | (1 << KL), j | (1 << KH), j | (1 << KH) | (1 << KL)};
+    const bool ok = tok && ((j & rm) == rv);
+    float2 x[4], y[4], lm[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { x[r] = p[idx[r]]; lm[r] = l[idx[r]]; }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float2 acc = cmulc(u[r], x[0]);
+#pragma unroll
+      for (int c = 1; c < 4; ++c) acc = cfmac(u[4 * c + r], x[c], acc);
+      y[r] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < P) {
+        const float2* D = M + 16 * (q + 1);
+        float t = 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float2 v = cmul(D[4 * r], y[0]);
+#pragma unroll
+          for (int c = 1; c < 4; ++c) v = cfma(D[4 * r + c], y[c], v);
+          t += redot(lm[r], v);
+        }
+        g[q] += ok ? t : 0.f;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float2 acc = cmulc(u[r], lm[0]);
+#pragma unroll
+      for (int c = 1; c < 4; ++c) acc = cfmac(u[4 * c + r], lm[c], acc);
+      p[idx[r]] = csel(ok, y[r], x[r]);
+      l[idx[r]] = csel(ok, acc, lm[r]);
+    }
+  }
+}
+
+template <int KB, int EX>
+__device__ __forceinline__ void b_matk1(float2 (&a)[NS], const float2* __restrict__ M, bool tok) {
+  constexpr int D = 1 << KB;
+#pragma unroll
+  for (int base = 0; base < NS; ++base) {
+    if (KB == 4 && base != 0) continue;
+    if (KB == 3 && (base != 0 && base != (1 << EX))) continue;
+    float2 x[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) x[r] = a[matk_slot<KB, EX>(base, r)];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      float2 acc = cmulc(M[r], x[0]);
+#pragma unroll
+      for (int c = 1; c < D; ++c) acc = cfmac(M[c * D + r], x[c], acc);
+      a[matk_slot<KB, EX>(base, r)] = csel(tok, acc, x[r]);
+    }
+  }
+}
+
+__device__ __forceinline__ void b_diag(float2 (&p)[NS], float2 (&l)[NS], const float2* __restrict__ M, int P,
+                                       bool tok, int rm, int rv, int tpar, int rpm, float (&g)[4]) {
+  const float2 p0 = M[0], p1 = M[1];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const bool ok = tok && ((j & rm) == rv);
+    const int par = (tpar + __popc(j & rpm)) & 1;
+    const float2 f = par ? p1 : p0;
+    const float2 y = cmulc(f, p[j]);
+    const float2 lj = l[j];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < P) {
+        const float2 df = par ? M[2 * (q + 1) + 1] : M[2 * (q + 1)];
+        g[q] += ok ? redot(lj, cmul(df, y)) : 0.f;
+      }
+    }
+    p[j] = csel(ok, y, p[j]);
+    l[j] = csel(ok, cmulc(f, lj), lj);
+  }
+}
+
+// ------------------------------------------------------------------ remap
+__device__ __forceinline__ int layout_thread(const int* __restrict__ tl, int T, int tid) {
+  int lt = 0;
+  for (int i = 0; i < T; ++i) lt |= ((tid >> i) & 1) ? tl[i] : 0;
+  return lt;
+}
+
+__device__ __forceinline__ void remap(float2 (&a)[NS], float2* __restrict__ lds, const int* __restrict__ from,
+                                      const int* __restrict__ to, int T, int tid) {
+  // record: tl[T], rl[R] (local-index value of each thread / register bit).
+  // swz is linear over XOR, so address = swz(thread part) ^ swz(slot part),
+  // the slot part being wave-uniform.
+  int st0 = swz(layout_thread(from, T, tid));
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    int lr = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) lr |= ((j >> k) & 1) ? from[T + k] : 0;
+    asm volatile("" : "+v"(st0));
+    lds[st0 ^ swz(lr)] = a[j];
+  }
+  __syncthreads();
+  int st1 = swz(layout_thread(to, T, tid));
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    int lr = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) lr |= ((j >> k) & 1) ? to[T + k] : 0;
+    asm volatile("" : "+v"(st1));
+    a[j] = lds[st1 ^ swz(lr)];
+  }
+}
+
+// ------------------------------------------------------------------ dispatch
+__device__ __forceinline__ void fwd_op(float2 (&a)[NS], const int* __restrict__ w, const float2* __restrict__ M,
+                                       int tid) {
+  const int type = w[0] & 15;
+  const bool tok = (tid & w[4]) == w[5];
+  const int rm = w[6] & 255, rv = (w[6] >> 8) & 255;
+  if (type == OP_MAT1) {
+    switch (w[3]) {
+      case 0: f_mat1<0>(a, M, tok, rm, rv); break;
+      case 1: f_mat1<1>(a, M, tok, rm, rv); break;
+      case 2: f_mat1<2>(a, M, tok, rm, rv); break;
+      default: f_mat1<3>(a, M, tok, rm, rv); break;
+    }
+  } else if (type == OP_MAT2) {
+    switch (w[3]) {  // kh | kl << 4, kh > kl
+      case 0x01: f_mat2<1, 0>(a, M, tok, rm, rv); break;
+      case 0x02: f_mat2<2, 0>(a, M, tok, rm, rv); break;
+      case 0x03: f_mat2<3, 0>(a, M, tok, rm, rv); break;
+      case 0x12: f_mat2<2, 1>(a, M, tok, rm, rv); break;
+      case 0x13: f_mat2<3, 1>(a, M, tok, rm, rv); break;
+      default: f_mat2<3, 2>(a, M, tok, rm, rv); break;
+    }
+  } else if (type == OP_DIAG) {
+    const int tpar = __popc(tid & w[3]);
+    f_diag(a, M, tok, rm, rv, tpar, (w[6] >> 16) & 255);
+  } else if (type == OP_MATK) {
+    switch (w[3]) {  // 4 = all register bits; 0..3 = k=3 excluding that bit
+      case 0: f_matk<3, 0>(a, M, tok); break;
+      case 1: f_matk<3, 1>(a, M, tok); break;
+      case 2: f_matk<3, 2>(a, M, tok); break;
+      case 3: f_matk<3, 3>(a, M, tok); break;
+      default: f_matk<4, 0>(a, M, tok); break;
+    }
+  }
+}
+
+__device__ __forceinline__ void bwd_op(float2 (&p)[NS], float2 (&l)[NS], const int* __restrict__ w,
+                                       const float2* __restrict__ M, int tid, float (&g)[4]) {
+  const int type = w[0] & 15;
+  const int P = (w[7] >= 0) ? ((w[0] >> 4) & 15) : 0;
+  const bool tok = (tid & w[4]) == w[5];
+  const int rm = w[6] & 255, rv = (w[6] >> 8) & 255;
+  if (type == OP_MAT1) {
+    switch (w[3]) {
+      case 0: b_mat1<0>(p, l, M, P, tok, rm, rv, g); break;
+      case 1: b_mat1<1>(p, l, M, P, tok, rm, rv, g); break;
+      case 2: b_mat1<2>(p, l, M, P, tok, rm, rv, g); break;
+      default: b_mat1<3>(p, l, M, P, tok, rm, rv, g); break;
+    }
+  } else if (type == OP_MAT2) {
+    switch (w[3]) {
+      case 0x01: b_mat2<1, 0>(p, l, M, P, tok, rm, rv, g); break;
+      case 0x02: b_mat2<2, 0>(p, l, M, P, tok, rm, rv, g); break;
+      case 0x03: b_mat2<3, 0>(p, l, M, P, tok, rm, rv, g); break;
+      case 0x12: b_mat2<2, 1>(p, l, M, P, tok, rm, rv, g); break;
+      case 0x13: b_mat2<3, 1>(p, l, M, P, tok, rm, rv, g); break;
+      default: b_mat2<3, 2>(p, l, M, P, tok, rm, rv, g); break;
+    }
+  } else if (type == OP_DIAG) {
+    const int tpar = __popc(tid & w[3]);
+    b_diag(p, l, M, P, tok, rm, rv, tpar, (w[6] >> 16) & 255, g);
+  } else if (type == OP_MATK) {
+    switch (w[3]) {
+      case 0: b_matk1<3, 0>(p, M, tok); b_matk1<3, 0>(l, M, tok); break;
+      case 1: b_matk1<3, 1>(p, M, tok); b_matk1<3, 1>(l, M, tok); break;
+      case 2: b_matk1<3, 2>(p, M, tok); b_matk1<3, 2>(l, M, tok); break;
+      case 3: b_matk1<3, 3>(p, M, tok); b_matk1<3, 3>(l, M, tok); break;
+      default: b_matk1<4, 0>(p, M, tok); b_matk1<4, 0>(l, M, tok); break;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ kernels
+struct StageArgs {
+  const int* prog;        // this stage's words (device)
+  const float2* mats;     // matrix pool (device)
+  int n;                  // qubits
+  int init_zero;          // forward: 1 = start from |0..0> instead of reading psi_in
+  int64_t n_groups;       // bsz << (n - m)
+  int64_t groups_per_wg;
+  float* grad;            // adjoint: [bsz, grad_stride] accumulated with atomics
+  int64_t grad_stride;
+};
+
+__device__ __forceinline__ uint32_t group_offset(const int* __restrict__ nlpos, int nnl, uint64_t rest) {
+  uint32_t o = 0;
+  for (int i = 0; i < nnl; ++i) o |= (uint32_t)((rest >> i) & 1ull) << nlpos[i];
+  return o;
+}
+
+__global__ __launch_bounds__(512) void k_stage_fwd(float2* __restrict__ out, const float2* __restrict__ in,
+                                                   StageArgs A) {
+  extern __shared__ float2 lds[];
+  const int* __restrict__ h = A.prog;
+  const int n_ops = h[H_NOPS], T = h[H_T], nnl = h[H_NNL];
+  const int* __restrict__ tpos = h + H_POS;
+  const int* __restrict__ rpos = tpos + T;
+  const int* __restrict__ nlpos = rpos + R;
+  const int* __restrict__ ops = h + h[H_OFF_OPS];
+  const int tid = threadIdx.x;
+  uint32_t toff = 0;
+  for (int i = 0; i < T; ++i) toff |= (uint32_t)((tid >> i) & 1) << tpos[i];
+  uint32_t roff[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) o |= (uint32_t)((j >> k) & 1) << rpos[k];
+    roff[j] = o;
+  }
+  const int64_t g0 = (int64_t)blockIdx.x * A.groups_per_wg;
+  int64_t g1 = g0 + A.groups_per_wg;
+  if (g1 > A.n_groups) g1 = A.n_groups;
+  const uint64_t rest_mask = (1ull << nnl) - 1ull;
+  for (int64_t g = g0; g < g1; ++g) {
+    const uint64_t row = (uint64_t)g >> nnl;
+    const uint32_t goff = group_offset(nlpos, nnl, (uint64_t)g & rest_mask);
+    const uint64_t base = (row << A.n) + goff + toff;
+    float2 a[NS];
+    if (A.init_zero) {
+#pragma unroll
+      for (int j = 0; j < NS; ++j) a[j] = make_float2((goff + toff + roff[j]) == 0 ? 1.f : 0.f, 0.f);
+    } else {
+      const float2* src = in + base;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) a[j] = src[roff[j]];
+    }
+    for (int o = 0; o < n_ops; ++o) {
+      const int* __restrict__ w = ops + 8 * o;
+      if ((w[0] & 15) == OP_REMAP) {
+        const int* rec = h + w[7];
+        remap(a, lds, rec, rec + T + R, T, tid);
+        continue;
+      }
+      const float2* M = A.mats + w[1] + ((w[0] & 256) ? (int64_t)row * w[2] : 0);
+      fwd_op(a, w, M, tid);
+    }
+    float2* dst = out + base;
+    asm volatile("" : "+v"(dst));
+#pragma unroll
+    for (int j = 0; j < NS; ++j) dst[roff[j]] = a[j];
+  }
+}
+
+__global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, float2* __restrict__ lam,
+                                                   StageArgs A) {
+  extern __shared__ float2 lds[];
+  __shared__ float gacc[MAX_GP];
+  const int* __restrict__ h = A.prog;
+  const int n_ops = h[H_NOPS], T = h[H_T], nnl = h[H_NNL], n_gp = h[H_NGP];
+  const int* __restrict__ tpos = h + H_POS;
+  const int* __restrict__ rpos = tpos + T;
+  const int* __restrict__ nlpos = rpos + R;
+  const int* __restrict__ ops = h + h[H_OFF_OPS];
+  const int* __restrict__ gslot = h + h[H_OFF_GSLOT];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wwidth = blockDim.x < 64 ? (int)blockDim.x : 64;
+  for (int t = tid; t < n_gp; t += blockDim.x) gacc[t] = 0.f;
+  uint32_t toff = 0;
+  for (int i = 0; i < T; ++i) toff |= (uint32_t)((tid >> i) & 1) << tpos[i];
+  uint32_t roff[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) o |= (uint32_t)((j >> k) & 1) << rpos[k];
+    roff[j] = o;
+  }
+  const int64_t g0 = (int64_t)blockIdx.x * A.groups_per_wg;
+  int64_t g1 = g0 + A.groups_per_wg;
+  if (g1 > A.n_groups) g1 = A.n_groups;
+  const uint64_t rest_mask = (1ull << nnl) - 1ull;
+  int64_t cur_row = -1;
+  __syncthreads();
+  for (int64_t g = g0; g < g1; ++g) {
+    const uint64_t row = (uint64_t)g >> nnl;
+    if (n_gp > 0 && (int64_t)row != cur_row) {
+      if (cur_row >= 0) {
+        __syncthreads();
+        for (int t = tid; t < n_gp; t += blockDim.x) {
+          const float v = gacc[t];
+          if (v != 0.f) atomicAdd(A.grad + cur_row * A.grad_stride + gslot[t], v);
+          gacc[t] = 0.f;
+        }
+        __syncthreads();
+      }
+      cur_row = (int64_t)row;
+    }
+    const uint32_t goff = group_offset(nlpos, nnl, (uint64_t)g & rest_mask);
+    const uint64_t base = (row << A.n) + goff + toff;
+    float2 p[NS], l[NS];
+    {
+      const float2* sp = psi + base;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) p[j] = sp[roff[j]];
+      const float2* sl = lam + base;
+      asm volatile("" : "+v"(sl));
+#pragma unroll
+      for (int j = 0; j < NS; ++j) l[j] = sl[roff[j]];
+    }
+    for (int o = n_ops - 1; o >= 0; --o) {
+      const int* __restrict__ w = ops + 8 * o;
+      if ((w[0] & 15) == OP_REMAP) {
+        const int* rec = h + w[7];
+        remap(p, lds, rec + T + R, rec, T, tid);
+        remap(l, lds, rec + T + R, rec, T, tid);
+        continue;
+      }
+      const float2* M = A.mats + w[1] + ((w[0] & 256) ? (int64_t)row * w[2] : 0);
+      float gq[4] = {0.f, 0.f, 0.f, 0.f};
+      bwd_op(p, l, w, M, tid, gq);
+      if (w[7] >= 0) {
+        const int P = (w[0] >> 4) & 15;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (q < P) {
+            const float s = wave_sum(gq[q], wwidth);
+            if (lane == 0) atomicAdd(&gacc[w[7] + q], s);
+          }
+        }
+      }
+    }
+    {
+      float2* dp = psi + base;
+      asm volatile("" : "+v"(dp));
+#pragma unroll
+      for (int j = 0; j < NS; ++j) dp[roff[j]] = p[j];
+      float2* dl = lam + base;
+      asm volatile("" : "+v"(dl));
+#pragma unroll
+      for (int j = 0; j < NS; ++j) dl[roff[j]] = l[j];
+    }
+  }
+  if (n_gp > 0 && cur_row >= 0) {
+    __syncthreads();
+    for (int t = tid; t < n_gp; t += blockDim.x) {
+      const float v = gacc[t];
+      if (v != 0.f) atomicAdd(A.grad + cur_row * A.grad_stride + gslot[t], v);
+    }
+  }
+}
+
+}  // namespace tqsv
+
+using namespace tqsv;
+
+static int stage_launch_shape(const int* prog_host, int n, int64_t bsz, int grid_cap, int& T, int64_t& n_groups,
+                              int& grid, int64_t& gpw, size_t& shm) {
+  const int m = prog_host[H_M];
+  T = prog_host[H_T];
+  if (m < R || m > 13 || T != m - R || m > n || prog_host[H_NNL] != n - m) {
+    set_error("stage header inconsistent: m=%d T=%d n=%d nnl=%d", m, T, n, prog_host[H_NNL]);
+    return E_ARG;
+  }
+  if (prog_host[H_NGP] > MAX_GP) {
+    set_error("stage has %d gradient parameters (max %d)", prog_host[H_NGP], MAX_GP);
+    return E_LIMIT;
+  }
+  n_groups = bsz << (n - m);
+  int64_t cap = grid_cap > 0 ? grid_cap : (int64_t)num_cu() * (T >= 8 ? 4 : 8);
+  int64_t g = n_groups < cap ? n_groups : cap;
+  gpw = (n_groups + g - 1) / g;
+  grid = (int)((n_groups + gpw - 1) / gpw);
+  shm = sizeof(float2) << m;
+  return OK;
+}
+
+// prog_host: the same stage words as prog_dev, readable on the host (only the
+// header is read, to size the launch).  mats: matrix pool.  state_in may
+// equal state_out (in-place) or be null with init_zero = 1.
+TQSV_API int tqsv_stage_forward(hipStream_t s, void* state_out, const void* state_in, int64_t bsz, int n,
+                                const int* prog_host, const int* prog_dev, const void* mats, int init_zero,
+                                int grid_cap) {
+  if (!state_out || !prog_host || !prog_dev || !mats || (!state_in && !init_zero)) {
+    set_error("tqsv_stage_forward: null pointer argument");
+    return E_ARG;
+  }
+  if (bsz < 1 || n < R || n > 32) { set_error("tqsv_stage_forward: bad bsz/n (%lld, %d)", (long long)bsz, n); return E_ARG; }
+  int T, grid;
+  int64_t n_groups, gpw;
+  size_t shm;
+  int rc = stage_launch_shape(prog_host, n, bsz, grid_cap, T, n_groups, grid, gpw, shm);
+  if (rc) return rc;
+  StageArgs A{prog_dev, (const float2*)mats, n, init_zero, n_groups, gpw, nullptr, 0};
+  k_stage_fwd<<<grid, 1 << T, shm, s>>>((float2*)state_out, (const float2*)state_in, A);
+  return check_launch("tqsv_stage_forward");
+}
+
+// psi / lam are updated in place; grad ([bsz, grad_stride] float) is
+// accumulated into (the caller zeroes it once per backward pass).
+TQSV_API int tqsv_stage_adjoint(hipStream_t s, void* psi, void* lam, int64_t bsz, int n, const int* prog_host,
+                                const int* prog_dev, const void* mats, float* grad, int64_t grad_stride,
+                                int grid_cap) {
+  if (!psi || !lam || !prog_host || !prog_dev || !mats) {
+    set_error("tqsv_stage_adjoint: null pointer argument");
+    return E_ARG;
+  }
+  if (bsz < 1 || n < R || n > 32) { set_error("tqsv_stage_adjoint: bad bsz/n"); return E_ARG; }
+  if (prog_host[H_NGP] > 0 && !grad) { set_error("tqsv_stage_adjoint: grad buffer required"); return E_ARG; }
+  int T, grid;
+  int64_t n_groups, gpw;
+  size_t shm;
+  int rc = stage_launch_shape(prog_host, n, bsz, grid_cap, T, n_groups, grid, gpw, shm);
+  if (rc) return rc;
+  StageArgs A{prog_dev, (const float2*)mats, n, 0, n_groups, gpw, grad, grad_stride};
+  k_stage_adj<<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, A);
+  return check_launch("tqsv_stage_adjoint");
+}

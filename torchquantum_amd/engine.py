@@ -1,0 +1,837 @@
+"""Circuit engine: tape -> kernel ops -> plan -> launches (forward and adjoint).
+
+The reference dispatches every gate eagerly through ``gate_wrapper``
+(torchquantum/functional/gate_wrapper.py:324-446) into a permute + bmm +
+permute (``apply_unitary_bmm``, :96-140).  Here gate calls are recorded on a
+per-device tape; when the state is needed the tape is compiled once per
+circuit *structure* into a plan and executed by the HIP kernels in
+``libtqsv.so``:
+
+* ``pergate``  -- one strided kernel per gate (the plain baseline);
+* ``fused``    -- greedy partition of the gate DAG into stages of <= m local
+  bits, each stage one read + write of the state (tqsv_stage.hip), gates
+  that fit no stage run as single per-gate launches in between.
+
+Matrices and their parameter derivatives are evaluated on the host in
+float64 (vectorised per gate type, derivatives by forward-mode AD) and
+uploaded as one pool per call; the plan's program words are uploaded once.
+
+Bit positions: wire w of an n-wire device is bit ``n - 1 - w`` of the in-row
+amplitude index (torchquantum/device/devices.py:64-71).
+"""
+from __future__ import annotations
+
+import math
+import os
+import threading
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import gates as G
+from . import _native as N
+
+R_BITS = 4            # register bits per thread in the stage kernel
+MAX_STAGE_BITS = 13   # LDS: 2^13 * 8 B = 64 KiB per workgroup
+MAX_GP_STAGE = 1024
+MAX_P_PER_OP = 4
+
+# stage header layout (must match tqsv_stage.hip)
+H_NOPS, H_M, H_T, H_NNL, H_OFF_OPS, H_NGP, H_OFF_GSLOT, H_FLAGS, H_POS = range(9)
+OP_MAT1, OP_MAT2, OP_DIAG, OP_MATK, OP_REMAP = 1, 2, 3, 4, 5
+
+
+# ----------------------------------------------------------------------------
+# options
+# ----------------------------------------------------------------------------
+@dataclass
+class EngineOptions:
+    """Execution options of one device (``QuantumDevice.engine_options``).
+
+    mode:        "fused" (default) or "pergate".
+    stage_bits:  m, local bits per stage (R_BITS + log2 threads per group); 0 = auto.
+    low_bits:    lowest bit positions always kept local (coalescing).
+    merge_1q:    multiply runs of single-wire gates into one 2x2 (fused mode).
+    grid_cap:    workgroups per stage launch (0 = auto).
+    """
+    mode: str = "fused"
+    stage_bits: int = 0
+    low_bits: int = 3
+    merge_1q: bool = True
+    grid_cap: int = 0
+
+    def key(self):
+        return (self.mode, self.stage_bits, self.low_bits, self.merge_1q, self.grid_cap)
+
+
+DEFAULT_OPTIONS = EngineOptions()
+
+
+# ----------------------------------------------------------------------------
+# tape
+# ----------------------------------------------------------------------------
+@dataclass
+class TapeOp:
+    name: str                       # canonical gate name
+    wires: Tuple[int, ...]
+    params: Optional[torch.Tensor]  # [rows, P] real, or complex matrix [d,d] / [B,d,d]
+    inverse: bool = False
+    version: int = 0
+
+    @property
+    def is_matrix_param(self) -> bool:
+        return self.params is not None and self.params.is_complex()
+
+    def rows(self) -> int:
+        if self.params is None:
+            return 1
+        if self.is_matrix_param:
+            return self.params.shape[0] if self.params.dim() == 3 else 1
+        return self.params.shape[0]
+
+    def n_params(self) -> int:
+        if self.params is None or self.is_matrix_param:
+            return 0
+        return self.params.shape[-1]
+
+    def wants_grad(self) -> bool:
+        return self.params is not None and self.params.requires_grad
+
+
+# ----------------------------------------------------------------------------
+# kernel-level ops
+# ----------------------------------------------------------------------------
+@dataclass
+class KOp:
+    kind: str                       # mat1 | mat2 | matk | diag | swap
+    bits: Tuple[int, ...]           # mat1: (t,); mat2: (hi, lo); matk: MSB..LSB; diag: parity bits; swap: (a, b)
+    cbits: Tuple[int, ...] = ()     # control bit positions
+    cvals: Tuple[int, ...] = ()     # required values
+    src: Tuple[int, ...] = ()       # tape indices (in application order)
+    extract: str = "full"           # how to take this op's matrix from the tape op matrix
+    diag_idx: Tuple[int, int] = (0, 0)
+    grad_slots: Tuple[int, ...] = ()  # flat parameter slots (one per kernel parameter)
+    batched: bool = False
+
+    @property
+    def cmask(self) -> int:
+        m = 0
+        for b in self.cbits:
+            m |= 1 << b
+        return m
+
+    @property
+    def cval(self) -> int:
+        v = 0
+        for b, x in zip(self.cbits, self.cvals):
+            if x:
+                v |= 1 << b
+        return v
+
+    @property
+    def allbits(self) -> frozenset:
+        return frozenset(self.bits) | frozenset(self.cbits)
+
+    @property
+    def P(self) -> int:
+        return len(self.grad_slots)
+
+    def dim(self) -> int:
+        if self.kind == "diag":
+            return 0
+        if self.kind == "mat1":
+            return 2
+        if self.kind in ("mat2", "swap"):
+            return 4
+        return 1 << len(self.bits)
+
+
+def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]) -> List[KOp]:
+    """Tape ops -> kernel ops (no fusion).  Identity gates vanish."""
+    out: List[KOp] = []
+    for ti, op in enumerate(tape):
+        s = G.spec(op.name)
+        k = len(op.wires)
+        pos = [n - 1 - w for w in op.wires]
+        rows = op.rows()
+        if rows not in (1, bsz):
+            raise ValueError(f"gate '{op.name}': parameter batch {rows} does not match bsz {bsz}")
+        batched = rows > 1
+        slots = ()
+        if op.wants_grad():
+            slots = tuple(param_offsets[ti] + p for p in range(op.n_params()))
+        if s.name == "i":
+            continue
+        if s.kind == "diag":
+            cidx, pidx = G.diag_form(s, k)
+            out.append(KOp("diag", tuple(pos[i] for i in pidx), tuple(pos[i] for i in cidx),
+                           tuple(1 for _ in cidx), (ti,), "diag",
+                           G.diag_phase_indices(cidx, pidx, k), slots, batched))
+        elif s.kind == "ctrl1":
+            ci, cv, t = G.ctrl_values(s, k)
+            out.append(KOp("mat1", (pos[t],), tuple(pos[i] for i in ci), tuple(cv), (ti,),
+                           "ctrl0" if s.cval0 else "ctrl1", (0, 0), slots, batched))
+        elif s.kind == "swap":
+            out.append(KOp("swap", (pos[-2], pos[-1]), tuple(pos[:-2]), tuple(1 for _ in pos[:-2]), (ti,),
+                           "swap", (0, 0), (), False))
+        else:
+            if op.is_matrix_param and op.params.requires_grad:
+                raise NotImplementedError("gradients through a matrix-valued gate parameter "
+                                          "(qubitunitary*) are not supported")
+            kind = "mat1" if k == 1 else ("mat2" if k == 2 else "matk")
+            out.append(KOp(kind, tuple(pos), (), (), (ti,), "full", (0, 0), slots, batched))
+    return out
+
+
+def merge_single_wire(kops: List[KOp], tape: Sequence[TapeOp]) -> List[KOp]:
+    """Fuse runs of uncontrolled single-wire ops on the same bit into one mat1.
+
+    A later op on bit b joins the pending run on b if nothing touched b in
+    between; the run's matrix is the product of its members.  Runs are capped
+    at MAX_P_PER_OP trainable parameters."""
+    out: List[Optional[KOp]] = []
+    pending: Dict[int, int] = {}   # bit -> index in out of the open run
+
+    def single(k: KOp) -> bool:
+        if k.cbits:
+            return False
+        if k.kind == "mat1":
+            return True
+        return k.kind == "diag" and len(k.bits) == 1
+
+    for k in kops:
+        if single(k):
+            b = k.bits[0]
+            j = pending.get(b)
+            if j is not None:
+                run = out[j]
+                if run.P + k.P <= MAX_P_PER_OP:
+                    out[j] = KOp("mat1", (b,), (), (), run.src + k.src, "chain", (0, 0),
+                                 run.grad_slots + k.grad_slots, run.batched or k.batched)
+                    continue
+            out.append(KOp("mat1", (b,), (), (), k.src, "chain", (0, 0), k.grad_slots, k.batched)
+                       if k.kind == "diag" else k)
+            pending[b] = len(out) - 1
+        else:
+            for b in k.allbits:
+                pending.pop(b, None)
+            out.append(k)
+    return [k for k in out if k is not None]
+
+
+# ----------------------------------------------------------------------------
+# matrix values (host, float64)
+# ----------------------------------------------------------------------------
+class GateValues:
+    """Per tape op: U [rows, d, d] complex128 and, if trainable, D [rows, P, d, d]."""
+
+    def __init__(self, tape: Sequence[TapeOp], want_grad: bool):
+        self.U: List[np.ndarray] = [None] * len(tape)
+        self.D: List[Optional[np.ndarray]] = [None] * len(tape)
+        groups: Dict[tuple, List[int]] = {}
+        for i, op in enumerate(tape):
+            if op.is_matrix_param:
+                m = op.params.detach().to("cpu", torch.complex128).numpy()
+                m = m.reshape((-1,) + m.shape[-2:]) if m.ndim == 3 else m[None]
+                if op.inverse:
+                    m = np.conj(np.swapaxes(m, -1, -2))
+                self.U[i] = m
+                continue
+            key = (op.name, len(op.wires), op.n_params() > 0)
+            groups.setdefault(key, []).append(i)
+        for (name, k, has_p), idx in groups.items():
+            builder = G.width_builder(name, k)
+            if has_p:
+                ps = [tape[i].params.detach().to("cpu", torch.float64) for i in idx]
+                rows = [p.shape[0] for p in ps]
+                P = ps[0].shape[-1]
+                allp = torch.cat([p.reshape(-1, P) for p in ps], dim=0)
+                U = builder(allp).numpy()
+                need_d = want_grad and any(tape[i].wants_grad() for i in idx)
+                if need_d:
+                    def one(q, _b=builder):
+                        return torch.view_as_real(_b(q.unsqueeze(0))[0])
+                    J = torch.func.vmap(torch.func.jacfwd(one))(allp)        # [N, d, d, 2, P]
+                    D = torch.complex(J[..., 0, :], J[..., 1, :]).movedim(-1, 1).numpy()  # [N,P,d,d]
+                off = 0
+                for i, r in zip(idx, rows):
+                    u = U[off:off + r]
+                    d = D[off:off + r] if need_d and tape[i].wants_grad() else None
+                    if tape[i].inverse:
+                        u = np.conj(np.swapaxes(u, -1, -2))
+                        if d is not None:
+                            d = np.conj(np.swapaxes(d, -1, -2))
+                    self.U[i] = u
+                    self.D[i] = d
+                    off += r
+            else:
+                u = builder(torch.zeros(1, 1, dtype=torch.float64)).numpy()
+                for i in idx:
+                    self.U[i] = np.conj(np.swapaxes(u, -1, -2)) if tape[i].inverse else u
+
+
+def _block(u: np.ndarray, extract: str) -> np.ndarray:
+    """Take this op's matrix from a tape-op matrix [..., d, d]."""
+    if extract == "ctrl1":
+        return u[..., -2:, -2:]
+    if extract == "ctrl0":
+        return u[..., :2, :2]
+    return u
+
+
+def kop_data(k: KOp, gv: GateValues, tape: Sequence[TapeOp], with_grad: bool) -> np.ndarray:
+    """Data of one kernel op: [rows, 1 + P, d*d] (or [rows, 1 + P, 2] for diag)."""
+    if k.kind == "swap":
+        m = G.FIXED_NP["swap"][None, None]
+        return m.reshape(1, 1, 16)
+    if k.kind == "diag":
+        ti = k.src[0]
+        u = gv.U[ti]
+        a, b = k.diag_idx
+        ph = np.stack([u[:, a, a], u[:, b, b]], axis=-1)[:, None]          # [rows, 1, 2]
+        if with_grad and k.P:
+            d = gv.D[ti]
+            dph = np.stack([d[:, :, a, a], d[:, :, b, b]], axis=-1)         # [rows, P, 2]
+            ph = np.concatenate([np.broadcast_to(ph, (dph.shape[0],) + ph.shape[1:]), dph], axis=1)
+        return ph
+    if k.extract == "chain":
+        return _chain_data(k, gv, tape, with_grad)
+    ti = k.src[0]
+    u = _block(gv.U[ti], k.extract)
+    d2 = u.shape[-1] ** 2
+    parts = [u.reshape(u.shape[0], 1, d2)]
+    if with_grad and k.P:
+        d = _block(gv.D[ti], k.extract)
+        rows = max(u.shape[0], d.shape[0])
+        parts = [np.broadcast_to(parts[0], (rows, 1, d2)), d.reshape(d.shape[0], d.shape[1], d2)]
+    return np.concatenate(parts, axis=1)
+
+
+def _as2x2(k_src: int, gv: GateValues, tape, with_grad):
+    """(U [rows,2,2], D list of [rows,2,2] or []) of a single-wire tape op."""
+    op = tape[k_src]
+    u = gv.U[k_src]
+    d = gv.D[k_src] if with_grad and op.wants_grad() else None
+    ds = [] if d is None else [d[:, p] for p in range(d.shape[1])]
+    return u, ds
+
+
+def _chain_data(k: KOp, gv: GateValues, tape, with_grad) -> np.ndarray:
+    mats = []
+    ders = []   # list of (member index, derivative matrix)
+    for mi, ti in enumerate(k.src):
+        u, ds = _as2x2(ti, gv, tape, with_grad)
+        mats.append(u)
+        for d in ds:
+            ders.append((mi, d))
+    rows = max(m.shape[0] for m in mats)
+    ident = np.broadcast_to(np.eye(2, dtype=np.complex128), (rows, 2, 2))
+    # prefix[i] = U_{i-1} ... U_0 ; total = U_{m-1} ... U_0
+    prefix = [ident]
+    for u in mats:
+        prefix.append(np.matmul(u, prefix[-1]))
+    total = prefix[-1]
+    parts = [total.reshape(rows, 1, 4)]
+    if with_grad and ders:
+        suffix = [ident] * (len(mats) + 1)
+        for i in range(len(mats) - 1, -1, -1):
+            suffix[i] = np.matmul(suffix[i + 1], mats[i]) if i < len(mats) else ident
+        # suffix[i] = U_{m-1} ... U_i ; derivative of member i: suffix[i+1] @ dU_i @ prefix[i]
+        for mi, d in ders:
+            dd = np.matmul(np.matmul(suffix[mi + 1], d), prefix[mi])
+            parts.append(np.broadcast_to(dd, (rows, 2, 2)).reshape(rows, 1, 4))
+    return np.concatenate([np.broadcast_to(p, (rows,) + p.shape[1:]) for p in parts], axis=1)
+
+
+# ----------------------------------------------------------------------------
+# plans
+# ----------------------------------------------------------------------------
+@dataclass
+class Stage:
+    words: np.ndarray           # int32 program words of this stage
+    ops: List[int]              # kernel-op indices, program order
+    m: int
+    enc: List[tuple]            # per encoded op: (kop index or -1 for remap, word offset, perm)
+
+
+@dataclass
+class Plan:
+    n: int
+    bsz: int
+    options: EngineOptions
+    kops: List[KOp]
+    steps: List[tuple]          # ("stage", Stage) | ("solo", kop index)
+    n_slots: int                # flat trainable-parameter count
+    prog_dev: Optional[torch.Tensor] = None
+    prog_host: Optional[np.ndarray] = None
+    stage_offsets: List[int] = field(default_factory=list)
+
+
+def _perm_index_bits(d_bits: int, src_of_dst: Sequence[int]) -> np.ndarray:
+    """Index permutation: kernel index r -> gate index r' where kernel bit q
+    takes gate bit src_of_dst[q] (bit numbers, LSB = 0)."""
+    out = np.zeros(1 << d_bits, dtype=np.int64)
+    for r in range(1 << d_bits):
+        rr = 0
+        for q in range(d_bits):
+            if (r >> q) & 1:
+                rr |= 1 << src_of_dst[q]
+        out[r] = rr
+    return out
+
+
+def choose_stage_bits(n: int, opts: EngineOptions) -> int:
+    if opts.stage_bits:
+        return max(R_BITS, min(opts.stage_bits, n, MAX_STAGE_BITS))
+    return max(R_BITS, min(n, 12))
+
+
+def stageable(k: KOp, m: int) -> bool:
+    if len(k.allbits) > m:
+        return False
+    if k.kind == "matk" and len(k.bits) > R_BITS:
+        return False
+    return True
+
+
+def partition(kops: List[KOp], n: int, m: int, low_bits: int) -> List[tuple]:
+    """Greedy DAG partition into stages of <= m local bits (+ solo ops)."""
+    steps = []
+    remaining = list(range(len(kops)))
+    forced = set(range(min(low_bits, n)))
+    while remaining:
+        first = remaining[0]
+        if not stageable(kops[first], m):
+            steps.append(("solo", first))
+            remaining = remaining[1:]
+            continue
+        # keep the low (coalescing) bits local unless the first op cannot fit with them
+        L = set(forced) if len(set(forced) | kops[first].allbits) <= m else set()
+        blocked = set()
+        inc, skip = [], []
+        for idx in remaining:
+            b = kops[idx].allbits
+            if b & blocked or not stageable(kops[idx], m):
+                skip.append(idx)
+                blocked |= b
+                continue
+            nl = L | b
+            if len(nl) <= m:
+                L = nl
+                inc.append(idx)
+            else:
+                skip.append(idx)
+                blocked |= b
+        if not inc:   # cannot happen (the first op always fits); guard against a stall
+            steps.append(("solo", first))
+            remaining = remaining[1:]
+            continue
+        # fill to m local bits with the lowest unused positions
+        p = 0
+        while len(L) < m:
+            if p not in L:
+                L.add(p)
+            p += 1
+        steps.append(("stage", (inc, sorted(L))))
+        remaining = skip
+    return steps
+
+
+def reg_targets(k: KOp) -> Tuple[int, ...]:
+    if k.kind == "mat1":
+        return k.bits
+    if k.kind in ("mat2", "swap", "matk"):
+        return k.bits
+    return ()
+
+
+def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off: Dict[int, tuple],
+                 slot_of_kop: Dict[int, List[int]], stage_base: int) -> Stage:
+    """Program words of one stage (see tqsv_stage.hip for the format)."""
+    m = len(L)
+    T = m - R_BITS
+    lbit = {g: i for i, g in enumerate(L)}           # global position -> local bit
+    canon = list(range(T, m))                        # register local bits, canonical
+    nonlocal_pos = [p for p in range(n) if p not in lbit]
+
+    header = np.zeros(H_POS + n, dtype=np.int32)
+    header[H_M], header[H_T], header[H_NNL] = m, T, n - m
+    header[H_POS:H_POS + T] = [L[i] for i in range(T)]
+    header[H_POS + T:H_POS + m] = [L[i] for i in canon]
+    header[H_POS + m:H_POS + n] = nonlocal_pos
+
+    # register-set schedule with look-ahead
+    needs = [tuple(lbit[b] for b in reg_targets(kops[i])) for i in inc]
+    S = list(canon)
+    seq = []   # ("remap", old, new) | ("op", i, S)
+    for j, i in enumerate(inc):
+        need = needs[j]
+        if need and not set(need) <= set(S):
+            new = list(dict.fromkeys(need))
+            for later in needs[j + 1:]:
+                for b in later:
+                    if len(new) < R_BITS and b not in new:
+                        new.append(b)
+                if len(new) >= R_BITS:
+                    break
+            for b in S + list(range(m)):
+                if len(new) >= R_BITS:
+                    break
+                if b not in new:
+                    new.append(b)
+            new = sorted(new)
+            seq.append(("remap", list(S), new))
+            S = new
+        seq.append(("op", i, list(S)))
+    if S != canon:
+        seq.append(("remap", list(S), list(canon)))
+
+    def layout_vals(regs):
+        thr = [b for b in range(m) if b not in regs]
+        return [1 << b for b in thr] + [1 << b for b in regs]
+
+    gslots: List[int] = []
+    ops_words = []
+    remap_words = []
+    enc = []
+    n_ops = len(seq)
+    off_gslot = H_POS + n
+    # gslot table size known after the loop; ops after it; remap records last
+    for item in seq:
+        w = [0] * 8
+        if item[0] == "remap":
+            rec = layout_vals(item[1]) + layout_vals(item[2])
+            w[0] = OP_REMAP
+            w[7] = len(remap_words)          # patched to an absolute offset below
+            remap_words.extend(rec)
+            ops_words.append(w)
+            enc.append((-1, None))
+            continue
+        _, i, regs = item
+        k = kops[i]
+        thr = [b for b in range(m) if b not in regs]
+        tpos = {b: t for t, b in enumerate(thr)}
+        rpos = {b: r for r, b in enumerate(regs)}
+        tm = tv = rm = rv = 0
+        for b, v in zip(k.cbits, k.cvals):
+            lb = lbit[b]
+            if lb in rpos:
+                rm |= 1 << rpos[lb]
+                rv |= (v & 1) << rpos[lb]
+            else:
+                tm |= 1 << tpos[lb]
+                tv |= (v & 1) << tpos[lb]
+        off, bstride = data_off[i]
+        P = k.P
+        perm = None
+        if k.kind == "mat1":
+            w[0] = OP_MAT1
+            w[3] = rpos[lbit[k.bits[0]]]
+        elif k.kind in ("mat2", "swap"):
+            w[0] = OP_MAT2
+            ph, pl = rpos[lbit[k.bits[0]]], rpos[lbit[k.bits[1]]]
+            if ph > pl:
+                w[3] = ph | (pl << 4)
+            else:
+                w[3] = pl | (ph << 4)
+                perm = _perm_index_bits(2, [1, 0])
+        elif k.kind == "matk":
+            w[0] = OP_MATK
+            kb = len(k.bits)
+            posl = [rpos[lbit[b]] for b in k.bits]         # bits[0] = gate MSB
+            order = sorted(range(kb), key=lambda q: posl[q])
+            # kernel index bit q <-> q-th ascending register position <-> gate bit (kb-1-order[q])
+            perm = _perm_index_bits(kb, [kb - 1 - order[q] for q in range(kb)])
+            if kb == 4:
+                w[3] = 4
+            else:
+                used = set(posl)
+                w[3] = [r for r in range(R_BITS) if r not in used][0]
+        else:  # diag
+            w[0] = OP_DIAG
+            tpm = rpm = 0
+            for b in k.bits:
+                lb = lbit[b]
+                if lb in rpos:
+                    rpm |= 1 << rpos[lb]
+                else:
+                    tpm |= 1 << tpos[lb]
+            w[3] = tpm
+            rm |= 0
+            w[6] = (rpm & 255) << 16
+        w[0] |= (P << 4) | ((256 if k.batched else 0))
+        w[1] = off
+        w[2] = bstride
+        w[4], w[5] = tm, tv
+        w[6] |= (rm & 255) | ((rv & 255) << 8)
+        if P:
+            w[7] = len(gslots)
+            gslots.extend(slot_of_kop[i])
+        else:
+            w[7] = -1
+        ops_words.append(w)
+        enc.append((i, perm))
+    if len(gslots) > MAX_GP_STAGE:
+        raise ValueError("stage exceeds the gradient-parameter limit")
+    off_ops = off_gslot + len(gslots)
+    off_remap = off_ops + 8 * len(ops_words)
+    for w in ops_words:
+        if w[0] == OP_REMAP:
+            w[7] += off_remap
+    header[H_NOPS] = n_ops
+    header[H_OFF_OPS] = off_ops
+    header[H_NGP] = len(gslots)
+    header[H_OFF_GSLOT] = off_gslot
+    words = np.concatenate([header, np.array(gslots, dtype=np.int32),
+                            np.array(ops_words, dtype=np.int32).reshape(-1),
+                            np.array(remap_words, dtype=np.int32)]).astype(np.int32)
+    return Stage(words, list(inc), m, enc)
+
+
+# ----------------------------------------------------------------------------
+# compiling
+# ----------------------------------------------------------------------------
+class Compiled:
+    """A plan plus the per-op data layout inside the matrix pool."""
+
+    def __init__(self, tape: Sequence[TapeOp], n: int, bsz: int, opts: EngineOptions, device):
+        self.n, self.bsz, self.opts, self.device = n, bsz, opts, device
+        self.param_offsets = []
+        off = 0
+        for op in tape:
+            self.param_offsets.append(off)
+            off += op.n_params()
+        self.n_slots = off
+        kops = lower(tape, n, bsz, self.param_offsets)
+        if opts.mode == "fused" and opts.merge_1q:
+            kops = merge_single_wire(kops, tape)
+        self.kops = kops
+        # pool layout: kop data (fwd, with derivative matrices when trainable)
+        self.data_off: Dict[int, tuple] = {}
+        self.data_len: Dict[int, int] = {}
+        pos = 0
+        for i, k in enumerate(kops):
+            per = (2 if k.kind == "diag" else k.dim() ** 2) * (1 + k.P)
+            rows = bsz if k.batched else 1
+            self.data_off[i] = (pos, per if k.batched else 0)
+            self.data_len[i] = per * rows
+            pos += per * rows
+        # adjoint data (U^dagger) for non-trainable ops run by per-gate launches
+        self.adj_off: Dict[int, tuple] = {}
+        for i, k in enumerate(kops):
+            if k.P == 0 and k.kind != "swap":
+                per = 2 if k.kind == "diag" else k.dim() ** 2
+                rows = bsz if k.batched else 1
+                self.adj_off[i] = (pos, per if k.batched else 0)
+                pos += per * rows
+        self.pool_len = max(pos, 1)
+        self.steps: List[tuple] = []
+        self.perms: Dict[int, np.ndarray] = {}
+        if opts.mode == "fused" and n >= R_BITS:
+            m = choose_stage_bits(n, opts)
+            slot_of = {i: list(k.grad_slots) for i, k in enumerate(kops)}
+            words = []
+            base = 0
+            for kind, payload in partition(kops, n, m, opts.low_bits):
+                if kind == "solo":
+                    self.steps.append(("solo", payload))
+                    continue
+                inc, L = payload
+                st = encode_stage(inc, L, kops, n, self.data_off, slot_of, base)
+                for (ki, perm) in st.enc:
+                    if ki >= 0 and perm is not None:
+                        self.perms[ki] = perm
+                self.steps.append(("stage", (base, st)))
+                words.append(st.words)
+                base += len(st.words)
+            self.prog_host = np.concatenate(words).astype(np.int32) if words else np.zeros(1, np.int32)
+            self.prog_dev = torch.from_numpy(self.prog_host.copy()).to(device)
+        else:
+            self.steps = [("solo", i) for i in range(len(kops))]
+            self.prog_host = None
+            self.prog_dev = None
+
+    # ---------------------------------------------------------------- values
+    def pool(self, tape: Sequence[TapeOp], with_grad: bool, with_adj: bool) -> torch.Tensor:
+        host = self.pool_host(tape, with_grad, with_adj)
+        return torch.from_numpy(host).to(self.device, non_blocking=False)
+
+    def pool_host(self, tape: Sequence[TapeOp], with_grad: bool, with_adj: bool) -> np.ndarray:
+        """The matrix pool (complex64) for the current parameter values."""
+        gv = GateValues(tape, with_grad)
+        host = np.zeros(self.pool_len, dtype=np.complex64)
+        for i, k in enumerate(self.kops):
+            d = kop_data(k, gv, tape, with_grad and k.P > 0)       # [rows, 1+P, dd]
+            perm = self.perms.get(i)
+            if perm is not None:
+                dd = int(round(math.sqrt(d.shape[-1])))
+                d = d.reshape(d.shape[0], d.shape[1], dd, dd)[:, :, perm][:, :, :, perm]
+                d = d.reshape(d.shape[0], d.shape[1], dd * dd)
+            rows = self.bsz if k.batched else 1
+            if not (with_grad and k.P > 0) and k.P > 0:
+                # derivative slots unused in a forward-only pass: leave zeros
+                d = d[:, :1]
+            d = np.broadcast_to(d, (rows,) + d.shape[1:])
+            off, _ = self.data_off[i]
+            flat = d.reshape(rows, -1)
+            per = self.data_len[i] // rows
+            blk = host[off:off + per * rows].reshape(rows, per)
+            blk[:, :flat.shape[1]] = flat
+            if with_adj and i in self.adj_off:
+                aoff, _ = self.adj_off[i]
+                if k.kind == "diag":
+                    a = np.conj(d[:, 0])
+                else:
+                    dd = k.dim()
+                    u = d[:, 0].reshape(rows, dd, dd)
+                    if perm is not None:
+                        # adjoint data is used by per-gate kernels: un-permuted
+                        inv = np.argsort(perm)
+                        u = u[:, inv][:, :, inv]
+                    a = np.conj(np.swapaxes(u, -1, -2)).reshape(rows, -1)
+                host[aoff:aoff + a.size] = a.reshape(-1)
+        return host
+
+
+_cache_lock = threading.Lock()
+_plan_cache: Dict[tuple, Compiled] = {}
+
+
+def structure_key(tape: Sequence[TapeOp], n: int, bsz: int, opts: EngineOptions, device) -> tuple:
+    items = []
+    for op in tape:
+        if op.params is None:
+            ps = None
+        elif op.is_matrix_param:
+            ps = ("m", tuple(op.params.shape))
+        else:
+            ps = (tuple(op.params.shape), bool(op.params.requires_grad))
+        items.append((op.name, op.wires, op.inverse, ps))
+    return (n, bsz, opts.key(), str(device), tuple(items))
+
+
+def compile_tape(tape: Sequence[TapeOp], n: int, bsz: int, opts: EngineOptions, device) -> Compiled:
+    key = structure_key(tape, n, bsz, opts, device)
+    with _cache_lock:
+        c = _plan_cache.get(key)
+    if c is None:
+        c = Compiled(tape, n, bsz, opts, device)
+        with _cache_lock:
+            if len(_plan_cache) > 256:
+                _plan_cache.clear()
+            _plan_cache[key] = c
+    return c
+
+
+# ----------------------------------------------------------------------------
+# execution
+# ----------------------------------------------------------------------------
+def _pool_ptr(pool: torch.Tensor, off: int) -> int:
+    return pool.data_ptr() + 8 * off
+
+
+def run_solo_forward(st: torch.Tensor, k: KOp, c: Compiled, pool: torch.Tensor, off: int, bstride: int,
+                     stream: int):
+    n, bsz = c.n, c.bsz
+    p = _pool_ptr(pool, off)
+    sp = st.data_ptr()
+    if k.kind == "mat1":
+        N.call("tqsv_apply_1q", stream, sp, bsz, n, k.bits[0], k.cmask, k.cval, p, bstride)
+    elif k.kind == "mat2":
+        N.call("tqsv_apply_2q", stream, sp, bsz, n, k.bits[0], k.bits[1], k.cmask, k.cval, p, bstride)
+    elif k.kind == "swap":
+        N.call("tqsv_apply_swap", stream, sp, bsz, n, k.bits[0], k.bits[1], k.cmask, k.cval)
+    elif k.kind == "diag":
+        pm = 0
+        for b in k.bits:
+            pm |= 1 << b
+        N.call("tqsv_apply_diag", stream, sp, bsz, n, k.cmask, k.cval, pm, p, bstride)
+    else:
+        bits = N.int_array(k.bits)
+        N.call("tqsv_apply_kq", stream, sp, bsz, n, len(k.bits), bits, k.cmask, k.cval, p, bstride)
+
+
+def forward(c: Compiled, state_in: torch.Tensor, pool: torch.Tensor, init_zero: bool = False) -> torch.Tensor:
+    """Run the plan out of place; returns the new state tensor ([bsz, 2^n])."""
+    n, bsz = c.n, c.bsz
+    out = torch.empty((bsz, 1 << n), dtype=torch.complex64, device=c.device)
+    stream = N.stream_of(out)
+    cur_in = state_in
+    fresh = False           # out holds the current state
+    if init_zero and (not c.steps or c.steps[0][0] != "stage"):
+        N.call("tqsv_init_basis", stream, out.data_ptr(), bsz, n, 0)
+        fresh = True
+    for kind, payload in c.steps:
+        if kind == "stage":
+            base, stg = payload
+            src = out if fresh else cur_in
+            hp = c.prog_host[base:]
+            N.call("tqsv_stage_forward", stream, out.data_ptr(), (src.data_ptr() if src is not None else None),
+                   bsz, n, hp.ctypes.data_as(N.c_int_p), c.prog_dev.data_ptr() + 4 * base, pool.data_ptr(),
+                   1 if (init_zero and not fresh) else 0, c.opts.grid_cap)
+            fresh = True
+        else:
+            if not fresh:
+                if init_zero:
+                    N.call("tqsv_init_basis", stream, out.data_ptr(), bsz, n, 0)
+                else:
+                    out.copy_(cur_in.reshape(bsz, -1))
+                fresh = True
+            k = c.kops[payload]
+            off, bs = c.data_off[payload]
+            run_solo_forward(out, k, c, pool, off, bs, stream)
+    if not fresh:
+        if init_zero:
+            N.call("tqsv_init_basis", stream, out.data_ptr(), bsz, n, 0)
+        else:
+            out.copy_(cur_in.reshape(bsz, -1))
+    return out
+
+
+def adjoint(c: Compiled, psi: torch.Tensor, lam: torch.Tensor, pool: torch.Tensor) -> torch.Tensor:
+    """Reverse sweep in place on psi (forward output) and lam (dL/dpsi).
+    Returns grads [bsz, n_slots] float32."""
+    n, bsz = c.n, c.bsz
+    grad = torch.zeros((bsz, max(c.n_slots, 1)), dtype=torch.float32, device=c.device)
+    stream = N.stream_of(psi)
+    gstride = grad.shape[1]
+    for kind, payload in reversed(c.steps):
+        if kind == "stage":
+            base, stg = payload
+            hp = c.prog_host[base:]
+            N.call("tqsv_stage_adjoint", stream, psi.data_ptr(), lam.data_ptr(), bsz, n,
+                   hp.ctypes.data_as(N.c_int_p), c.prog_dev.data_ptr() + 4 * base, pool.data_ptr(),
+                   N.fptr(grad), gstride, c.opts.grid_cap)
+            continue
+        i = payload
+        k = c.kops[i]
+        if k.P > 0:
+            off, bs = c.data_off[i]
+            p = _pool_ptr(pool, off)
+            slots = k.grad_slots
+            if list(slots) != list(range(slots[0], slots[0] + len(slots))):
+                raise RuntimeError("per-gate adjoint needs contiguous parameter slots")
+            if k.kind == "mat1":
+                N.call("tqsv_adjoint_1q", stream, psi.data_ptr(), lam.data_ptr(), bsz, n, k.bits[0], k.cmask,
+                       k.cval, p, bs, k.P, N.fptr(grad), gstride, slots[0])
+            elif k.kind == "mat2":
+                N.call("tqsv_adjoint_2q", stream, psi.data_ptr(), lam.data_ptr(), bsz, n, k.bits[0], k.bits[1],
+                       k.cmask, k.cval, p, bs, k.P, N.fptr(grad), gstride, slots[0])
+            elif k.kind == "diag":
+                pm = 0
+                for b in k.bits:
+                    pm |= 1 << b
+                N.call("tqsv_adjoint_diag", stream, psi.data_ptr(), lam.data_ptr(), bsz, n, k.cmask, k.cval, pm,
+                       p, bs, k.P, N.fptr(grad), gstride, slots[0])
+            else:
+                raise NotImplementedError(f"adjoint of trainable '{k.kind}' op")
+        else:
+            if k.kind == "swap":
+                off, bs = 0, 0
+            else:
+                off, bs = c.adj_off[i]
+            run_solo_forward(psi, k, c, pool, off, bs, stream)
+            run_solo_forward(lam, k, c, pool, off, bs, stream)
+    return grad
