@@ -69,6 +69,13 @@ def stage_apply(words, st, mats, n, init_zero=False, adjoint=False, lam=None, gr
                         l = _remap(l, fr, to, T, m)
                     continue
                 M = _op_mat(mats, w, row)
+                _check_flags(typ, w, M)
+                # the stage's feature bits select the kernel instantiation: a
+                # missing bit would make the kernel skip the op
+                if typ == E.OP_MAT2:
+                    assert int(h[E.H_FLAGS]) & E.FEAT_MAT2, "MAT2 op in a stage without the MAT2 feature"
+                if typ == E.OP_MATK:
+                    assert int(h[E.H_FLAGS]) & E.FEAT_MATK, "MATK op in a stage without the MATK feature"
                 tok = (tid & w[4]) == w[5]
                 rm, rv = w[6] & 255, (w[6] >> 8) & 255
                 P = (w[0] >> 4) & 15 if w[7] >= 0 else 0
@@ -83,6 +90,19 @@ def stage_apply(words, st, mats, n, init_zero=False, adjoint=False, lam=None, gr
                 lam[row, idx] = l
                 for t, s in enumerate(gslot):
                     grad[row, s] += gacc[t]
+
+
+def _check_flags(typ, w, M):
+    """Fast-path flags must agree with the data the kernel would read."""
+    fl = w[0]
+    if fl & E.F_XPERM:
+        assert typ == E.OP_MAT1 and np.allclose(M[:4], [0, 1, 1, 0]), "XPERM flag on a non-X matrix"
+    if fl & E.F_REAL:
+        assert typ == E.OP_MAT1 and np.allclose(np.imag(M[:4]), 0), "REAL flag on a complex matrix"
+    if fl & E.F_NOCTRL:
+        assert w[4] == 0 and (w[6] & 255) == 0, "NOCTRL flag with controls"
+    if fl & E.F_SIGN:
+        assert typ == E.OP_DIAG and np.isclose(M[0], -1) and w[3] == 0 and ((w[6] >> 16) & 255) == 0
 
 
 def _remap(a, fr, to, T, m):

@@ -274,3 +274,87 @@ def test_refpath_matches_oracle():
     got = refpath.run(ops, 10, 2).numpy()
     ref = O.run_ops(oracle_ops(ops, 1), 10, 1)
     assert np.abs(got - ref).max() < 1e-5
+
+
+@pytest.mark.parametrize("width", [2, 3, 4])
+@pytest.mark.parametrize("seed", range(3))
+def test_block_fusion_emulated(width, seed):
+    """Dense block fusion (fusion width k): same states as the oracle; blocks
+    never exceed k bits; trainable blocks keep gradients exact (k = 2)."""
+    rng = np.random.default_rng(300 + seed)
+    n, bsz = 6, 2
+    ops = random_ops(n, 50, bsz, rng)
+    tape = _tape_from_ops(ops)
+    c = E.Compiled(tape, n, bsz, E.EngineOptions(fusion_width=width, stage_bits=5), "cpu")
+    assert all(len(k.allbits) <= max(width, 5) for k in c.kops)
+    assert len(c.kops) < len(tape)
+    pool = c.pool_host(tape, False, True).astype(np.complex128)
+    got = EM.run_compiled(c, pool, None, init_zero=True)
+    assert np.abs(got - O.run_ops(oracle_ops(ops, bsz), n, bsz)).max() < 5e-6
+
+
+def test_block_fusion_gradients_emulated():
+    rng = np.random.default_rng(17)
+    n, bsz = 5, 2
+    ops = random_ops(n, 30, bsz, rng, names=["rx", "ry", "rz", "cnot", "cz", "crx", "u3", "rzz", "h"])
+    tape = _tape_from_ops(ops, grad=True)
+    c = E.Compiled(tape, n, bsz, E.EngineOptions(fusion_width=2, stage_bits=4), "cpu")
+    assert any(k.extract == "block" and k.P > 0 for k in c.kops)
+    pool = c.pool_host(tape, True, True).astype(np.complex128)
+    init = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+    cvec = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+    psi = EM.run_compiled(c, pool, init)
+    grad = EM.adjoint_compiled(c, pool, psi, cvec.copy())
+
+    def L(flat):
+        k = 0
+        o2 = []
+        for op in ops:
+            p = op["params"]
+            if p is not None:
+                p = flat[k:k + p.size].reshape(p.shape)
+                k += p.size
+            o2.append(dict(op, params=p))
+        return float(np.real(np.sum(np.conj(cvec) * O.run_ops(oracle_ops(o2, bsz), n, bsz, state=init))))
+
+    flat = np.concatenate([op["params"].reshape(-1) for op in ops if op["params"] is not None])
+    fd = O.central_fd(L, flat.copy(), h=1e-5)
+    got = []
+    for ti, op in enumerate(tape):
+        if op.params is None:
+            continue
+        g = grad[:, c.param_offsets[ti]:c.param_offsets[ti] + op.n_params()]
+        got.append(g.sum(0) if op.params.shape[0] == 1 else g.reshape(-1))
+    assert np.all(np.abs(np.concatenate(got) - fd) < 1e-4 * (1 + np.abs(fd)))
+
+
+def test_batch_work_lists_cover_every_group():
+    """CircuitBatch (host side): per level, the items of each circuit cover
+    its groups exactly once; programs relocate matrix offsets into one pool."""
+    from torchquantum_amd import batch as B
+    if not _native.lib_available():
+        pytest.skip("libtqsv.so not built")
+    circs = B.random_circuits(n_circ=12, n_range=(6, 15), depth_range=(2, 6), seed=3)
+    cb = B.CircuitBatch(circs, device="cpu", groups_per_item=3)
+    for L, (items, cnt) in enumerate(cb.levels):
+        arr = items.numpy().view(B.BATCH_ITEM)
+        assert len(arr) == cnt
+        for i, (n, _) in enumerate(circs):
+            if L >= len(cb.stage_offsets[i]):
+                continue
+            mine = arr[arr["state_off"] == cb.state_off[i]]
+            ng = 1 << (n - cb.stage_m[i][L])
+            cover = np.zeros(ng, dtype=int)
+            for it in mine:
+                cover[it["g0"]:it["g1"]] += 1
+                assert it["init_zero"] == (1 if L == 0 else 0)
+            assert np.all(cover == 1)
+    # relocated matrix offsets stay inside the pool
+    words = cb.progs.numpy()
+    for offs in cb.stage_offsets:
+        for o in offs:
+            h = words[o:]
+            for q in range(int(h[E.H_NOPS])):
+                w = h[int(h[E.H_OFF_OPS]) + 8 * q:][:8]
+                if (w[0] & 15) != E.OP_REMAP:
+                    assert 0 <= w[1] < cb.mats.numel()

@@ -46,12 +46,17 @@ SIGNATURES = {
                                   c_void_p, c_int64]),
     "tqsv_pauli_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int, c_u64_p, c_u64_p, c_void_p,
                                  c_int]),
-    "tqsv_batch_forward": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int]),
-    "tqsv_gemm_bias_act": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
-    "tqsv_mlp_backward": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                  c_void_p, c_void_p, c_int]),
-    "tqsv_adam_step": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float,
-                               c_float, c_int]),
+    "tqsv_batch_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int]),
+    "tqsv_batch_item_bytes": (c_int, []),
+    "tqsv_batch_zcirc_bytes": (c_int, []),
+    "tqsv_batch_zexpval": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int64, c_float_p, c_float_p]),
+    "tqsv_gemm": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
+                          c_float, c_void_p, c_int, c_void_p, c_int]),
+    "tqsv_relu_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),
+    "tqsv_colsum": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "tqsv_mse_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "tqsv_adam_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_float, c_float, c_float,
+                               c_float, c_float, c_float]),
 }
 
 

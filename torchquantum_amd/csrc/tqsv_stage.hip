@@ -256,6 +256,61 @@ __device__ __forceinline__ void b_diag(float2 (&p)[NS], float2 (&l)[NS], const f
   }
 }
 
+// ------------------------------------------------------------------ fast paths
+// Op flag bits (word 0): the compiler sets them from the gate structure.
+enum OpFlag { F_BATCHED = 1 << 8, F_NOCTRL = 1 << 9, F_REAL = 1 << 10, F_XPERM = 1 << 11, F_SIGN = 1 << 12 };
+
+// no controls: every pair is updated, no selects
+template <int K>
+__device__ __forceinline__ void f_mat1_nc(float2 (&a)[NS], const float2* __restrict__ M) {
+  const float2 u0 = M[0], u1 = M[1], u2 = M[2], u3 = M[3];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    if (j & (1 << K)) continue;
+    const int j1 = j | (1 << K);
+    const float2 x0 = a[j], x1 = a[j1];
+    a[j] = cfma(u1, x1, cmul(u0, x0));
+    a[j1] = cfma(u3, x1, cmul(u2, x0));
+  }
+}
+
+// real 2x2 (RY, H, ...): 8 FMA per pair
+template <int K>
+__device__ __forceinline__ void f_mat1_real_nc(float2 (&a)[NS], const float2* __restrict__ M) {
+  const float r0 = M[0].x, r1 = M[1].x, r2 = M[2].x, r3 = M[3].x;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    if (j & (1 << K)) continue;
+    const int j1 = j | (1 << K);
+    const float2 x0 = a[j], x1 = a[j1];
+    a[j] = make_float2(fmaf(r0, x0.x, r1 * x1.x), fmaf(r0, x0.y, r1 * x1.y));
+    a[j1] = make_float2(fmaf(r2, x0.x, r3 * x1.x), fmaf(r2, x0.y, r3 * x1.y));
+  }
+}
+
+// X on a register bit under controls (CNOT, Toffoli, ...): a pure exchange
+template <int K>
+__device__ __forceinline__ void f_xperm(float2 (&a)[NS], bool tok, int rm, int rv) {
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    if (j & (1 << K)) continue;
+    const int j1 = j | (1 << K);
+    const bool ok = tok && ((j & rm) == rv);
+    const float2 x0 = a[j], x1 = a[j1];
+    a[j] = csel(ok, x1, x0);
+    a[j1] = csel(ok, x0, x1);
+  }
+}
+
+// sign flip where the controls match (Z, CZ, CCZ): phase -1, no parity
+__device__ __forceinline__ void f_sign(float2 (&a)[NS], bool tok, int rm, int rv) {
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const bool ok = tok && ((j & rm) == rv);
+    a[j] = make_float2(ok ? -a[j].x : a[j].x, ok ? -a[j].y : a[j].y);
+  }
+}
+
 // ------------------------------------------------------------------ remap
 __device__ __forceinline__ int layout_thread(const int* __restrict__ tl, int T, int tid) {
   int lt = 0;
@@ -264,12 +319,13 @@ __device__ __forceinline__ int layout_thread(const int* __restrict__ tl, int T, 
 }
 
 __device__ __forceinline__ void remap(float2 (&a)[NS], float2* __restrict__ lds, const int* __restrict__ from,
-                                      const int* __restrict__ to, int T, int tid) {
+                                      const int* __restrict__ to, int T, int tid, bool active = true) {
   // record: tl[T], rl[R] (local-index value of each thread / register bit).
   // swz is linear over XOR, so address = swz(thread part) ^ swz(slot part),
   // the slot part being wave-uniform.
   int st0 = swz(layout_thread(from, T, tid));
   __syncthreads();
+  if (active) {
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
     int lr = 0;
@@ -278,7 +334,9 @@ __device__ __forceinline__ void remap(float2 (&a)[NS], float2* __restrict__ lds,
     asm volatile("" : "+v"(st0));
     lds[st0 ^ swz(lr)] = a[j];
   }
+  }
   __syncthreads();
+  if (!active) return;
   int st1 = swz(layout_thread(to, T, tid));
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
@@ -291,19 +349,63 @@ __device__ __forceinline__ void remap(float2 (&a)[NS], float2* __restrict__ lds,
 }
 
 // ------------------------------------------------------------------ dispatch
+// FEAT: bit 0 = the stage contains 2-bit dense ops, bit 1 = 3/4-bit dense ops.
+// Stages without them run a leaner instantiation (fewer live registers).
+constexpr int FEAT_MAT2 = 1, FEAT_MATK = 2;
+
+template <int FEAT>
 __device__ __forceinline__ void fwd_op(float2 (&a)[NS], const int* __restrict__ w, const float2* __restrict__ M,
                                        int tid) {
   const int type = w[0] & 15;
-  const bool tok = (tid & w[4]) == w[5];
-  const int rm = w[6] & 255, rv = (w[6] >> 8) & 255;
+  const int fl = w[0];
   if (type == OP_MAT1) {
-    switch (w[3]) {
-      case 0: f_mat1<0>(a, M, tok, rm, rv); break;
-      case 1: f_mat1<1>(a, M, tok, rm, rv); break;
-      case 2: f_mat1<2>(a, M, tok, rm, rv); break;
-      default: f_mat1<3>(a, M, tok, rm, rv); break;
+    if (fl & F_XPERM) {
+      const bool tok = (tid & w[4]) == w[5];
+      const int rm = w[6] & 255, rv = (w[6] >> 8) & 255;
+      switch (w[3]) {
+        case 0: f_xperm<0>(a, tok, rm, rv); break;
+        case 1: f_xperm<1>(a, tok, rm, rv); break;
+        case 2: f_xperm<2>(a, tok, rm, rv); break;
+        default: f_xperm<3>(a, tok, rm, rv); break;
+      }
+    } else if (fl & F_NOCTRL) {
+      if (fl & F_REAL) {
+        switch (w[3]) {
+          case 0: f_mat1_real_nc<0>(a, M); break;
+          case 1: f_mat1_real_nc<1>(a, M); break;
+          case 2: f_mat1_real_nc<2>(a, M); break;
+          default: f_mat1_real_nc<3>(a, M); break;
+        }
+      } else {
+        switch (w[3]) {
+          case 0: f_mat1_nc<0>(a, M); break;
+          case 1: f_mat1_nc<1>(a, M); break;
+          case 2: f_mat1_nc<2>(a, M); break;
+          default: f_mat1_nc<3>(a, M); break;
+        }
+      }
+    } else {
+      const bool tok = (tid & w[4]) == w[5];
+      const int rm = w[6] & 255, rv = (w[6] >> 8) & 255;
+      switch (w[3]) {
+        case 0: f_mat1<0>(a, M, tok, rm, rv); break;
+        case 1: f_mat1<1>(a, M, tok, rm, rv); break;
+        case 2: f_mat1<2>(a, M, tok, rm, rv); break;
+        default: f_mat1<3>(a, M, tok, rm, rv); break;
+      }
     }
-  } else if (type == OP_MAT2) {
+  } else if (type == OP_DIAG) {
+    const bool tok = (tid & w[4]) == w[5];
+    const int rm = w[6] & 255, rv = (w[6] >> 8) & 255;
+    if (fl & F_SIGN) {
+      f_sign(a, tok, rm, rv);
+    } else {
+      const int tpar = __popc(tid & w[3]);
+      f_diag(a, M, tok, rm, rv, tpar, (w[6] >> 16) & 255);
+    }
+  } else if ((FEAT & FEAT_MAT2) && type == OP_MAT2) {
+    const bool tok = (tid & w[4]) == w[5];
+    const int rm = w[6] & 255, rv = (w[6] >> 8) & 255;
     switch (w[3]) {  // kh | kl << 4, kh > kl
       case 0x01: f_mat2<1, 0>(a, M, tok, rm, rv); break;
       case 0x02: f_mat2<2, 0>(a, M, tok, rm, rv); break;
@@ -312,10 +414,8 @@ __device__ __forceinline__ void fwd_op(float2 (&a)[NS], const int* __restrict__ 
       case 0x13: f_mat2<3, 1>(a, M, tok, rm, rv); break;
       default: f_mat2<3, 2>(a, M, tok, rm, rv); break;
     }
-  } else if (type == OP_DIAG) {
-    const int tpar = __popc(tid & w[3]);
-    f_diag(a, M, tok, rm, rv, tpar, (w[6] >> 16) & 255);
-  } else if (type == OP_MATK) {
+  } else if ((FEAT & FEAT_MATK) && type == OP_MATK) {
+    const bool tok = (tid & w[4]) == w[5];
     switch (w[3]) {  // 4 = all register bits; 0..3 = k=3 excluding that bit
       case 0: f_matk<3, 0>(a, M, tok); break;
       case 1: f_matk<3, 1>(a, M, tok); break;
@@ -326,20 +426,39 @@ __device__ __forceinline__ void fwd_op(float2 (&a)[NS], const int* __restrict__ 
   }
 }
 
+template <int FEAT>
 __device__ __forceinline__ void bwd_op(float2 (&p)[NS], float2 (&l)[NS], const int* __restrict__ w,
                                        const float2* __restrict__ M, int tid, float (&g)[4]) {
   const int type = w[0] & 15;
+  const int fl = w[0];
   const int P = (w[7] >= 0) ? ((w[0] >> 4) & 15) : 0;
   const bool tok = (tid & w[4]) == w[5];
   const int rm = w[6] & 255, rv = (w[6] >> 8) & 255;
   if (type == OP_MAT1) {
+    if (fl & F_XPERM) {   // X is its own adjoint and carries no parameters
+      switch (w[3]) {
+        case 0: f_xperm<0>(p, tok, rm, rv); f_xperm<0>(l, tok, rm, rv); break;
+        case 1: f_xperm<1>(p, tok, rm, rv); f_xperm<1>(l, tok, rm, rv); break;
+        case 2: f_xperm<2>(p, tok, rm, rv); f_xperm<2>(l, tok, rm, rv); break;
+        default: f_xperm<3>(p, tok, rm, rv); f_xperm<3>(l, tok, rm, rv); break;
+      }
+      return;
+    }
     switch (w[3]) {
       case 0: b_mat1<0>(p, l, M, P, tok, rm, rv, g); break;
       case 1: b_mat1<1>(p, l, M, P, tok, rm, rv, g); break;
       case 2: b_mat1<2>(p, l, M, P, tok, rm, rv, g); break;
       default: b_mat1<3>(p, l, M, P, tok, rm, rv, g); break;
     }
-  } else if (type == OP_MAT2) {
+  } else if (type == OP_DIAG) {
+    if (fl & F_SIGN) {
+      f_sign(p, tok, rm, rv);
+      f_sign(l, tok, rm, rv);
+      return;
+    }
+    const int tpar = __popc(tid & w[3]);
+    b_diag(p, l, M, P, tok, rm, rv, tpar, (w[6] >> 16) & 255, g);
+  } else if ((FEAT & FEAT_MAT2) && type == OP_MAT2) {
     switch (w[3]) {
       case 0x01: b_mat2<1, 0>(p, l, M, P, tok, rm, rv, g); break;
       case 0x02: b_mat2<2, 0>(p, l, M, P, tok, rm, rv, g); break;
@@ -348,10 +467,7 @@ __device__ __forceinline__ void bwd_op(float2 (&p)[NS], float2 (&l)[NS], const i
       case 0x13: b_mat2<3, 1>(p, l, M, P, tok, rm, rv, g); break;
       default: b_mat2<3, 2>(p, l, M, P, tok, rm, rv, g); break;
     }
-  } else if (type == OP_DIAG) {
-    const int tpar = __popc(tid & w[3]);
-    b_diag(p, l, M, P, tok, rm, rv, tpar, (w[6] >> 16) & 255, g);
-  } else if (type == OP_MATK) {
+  } else if ((FEAT & FEAT_MATK) && type == OP_MATK) {
     switch (w[3]) {
       case 0: b_matk1<3, 0>(p, M, tok); b_matk1<3, 0>(l, M, tok); break;
       case 1: b_matk1<3, 1>(p, M, tok); b_matk1<3, 1>(l, M, tok); break;
@@ -380,6 +496,7 @@ __device__ __forceinline__ uint32_t group_offset(const int* __restrict__ nlpos, 
   return o;
 }
 
+template <int FEAT>
 __global__ __launch_bounds__(512) void k_stage_fwd(float2* __restrict__ out, const float2* __restrict__ in,
                                                    StageArgs A) {
   extern __shared__ float2 lds[];
@@ -424,8 +541,8 @@ __global__ __launch_bounds__(512) void k_stage_fwd(float2* __restrict__ out, con
         remap(a, lds, rec, rec + T + R, T, tid);
         continue;
       }
-      const float2* M = A.mats + w[1] + ((w[0] & 256) ? (int64_t)row * w[2] : 0);
-      fwd_op(a, w, M, tid);
+      const float2* M = A.mats + w[1] + ((w[0] & F_BATCHED) ? (int64_t)row * w[2] : 0);
+      fwd_op<FEAT>(a, w, M, tid);
     }
     float2* dst = out + base;
     asm volatile("" : "+v"(dst));
@@ -434,6 +551,7 @@ __global__ __launch_bounds__(512) void k_stage_fwd(float2* __restrict__ out, con
   }
 }
 
+template <int FEAT>
 __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, float2* __restrict__ lam,
                                                    StageArgs A) {
   extern __shared__ float2 lds[];
@@ -499,9 +617,9 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
         remap(l, lds, rec + T + R, rec, T, tid);
         continue;
       }
-      const float2* M = A.mats + w[1] + ((w[0] & 256) ? (int64_t)row * w[2] : 0);
+      const float2* M = A.mats + w[1] + ((w[0] & F_BATCHED) ? (int64_t)row * w[2] : 0);
       float gq[4] = {0.f, 0.f, 0.f, 0.f};
-      bwd_op(p, l, w, M, tid, gq);
+      bwd_op<FEAT>(p, l, w, M, tid, gq);
       if (w[7] >= 0) {
         const int P = (w[0] >> 4) & 15;
 #pragma unroll
@@ -531,6 +649,129 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
       if (v != 0.f) atomicAdd(A.grad + cur_row * A.grad_stride + gslot[t], v);
     }
   }
+}
+
+// ------------------------------------------------------------------ batch
+// Many circuits of different widths in one launch: each item is one stage
+// program applied to a range of groups of one circuit's state (bsz = 1).
+// The workgroup has BATCH_THREADS threads; an item with T thread bits uses
+// the first 2^T of them, the rest only join the LDS barriers.
+constexpr int BATCH_THREADS = 256;
+
+struct BatchItem {
+  int64_t state_off;   // amplitude offset of the circuit's state
+  int32_t prog_off;    // word offset of the stage program
+  int32_t n;           // qubits of the circuit
+  int32_t init_zero;   // first stage of the circuit: start from |0..0>
+  int32_t pad;
+  int64_t g0, g1;      // group range of this item
+};
+
+__global__ __launch_bounds__(BATCH_THREADS) void k_batch_fwd(float2* __restrict__ states,
+                                                             const BatchItem* __restrict__ items, int n_items,
+                                                             const int* __restrict__ progs,
+                                                             const float2* __restrict__ mats) {
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const BatchItem item = items[it];
+    const int* __restrict__ h = progs + item.prog_off;
+    const int n_ops = h[H_NOPS], T = h[H_T], nnl = h[H_NNL];
+    const int* __restrict__ tpos = h + H_POS;
+    const int* __restrict__ rpos = tpos + T;
+    const int* __restrict__ nlpos = rpos + R;
+    const int* __restrict__ ops = h + h[H_OFF_OPS];
+    const bool active = tid < (1 << T);
+    uint32_t toff = 0;
+    for (int i = 0; i < T; ++i) toff |= (uint32_t)((tid >> i) & 1) << tpos[i];
+    uint32_t roff[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      uint32_t o = 0;
+#pragma unroll
+      for (int k = 0; k < R; ++k) o |= (uint32_t)((j >> k) & 1) << rpos[k];
+      roff[j] = o;
+    }
+    const uint64_t rest_mask = (1ull << nnl) - 1ull;
+    float2* st = states + item.state_off;
+    for (int64_t g = item.g0; g < item.g1; ++g) {
+      const uint32_t goff = group_offset(nlpos, nnl, (uint64_t)g & rest_mask);
+      const uint64_t base = (uint64_t)goff + toff;
+      float2 a[NS];
+      if (item.init_zero) {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) a[j] = make_float2((goff + toff + roff[j]) == 0 ? 1.f : 0.f, 0.f);
+      } else if (active) {
+        const float2* src = st + base;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) a[j] = src[roff[j]];
+      }
+      for (int o = 0; o < n_ops; ++o) {
+        const int* __restrict__ w = ops + 8 * o;
+        if ((w[0] & 15) == OP_REMAP) {
+          const int* rec = h + w[7];
+          remap(a, lds, rec, rec + T + R, T, tid, active);
+          continue;
+        }
+        fwd_op<FEAT_MAT2 | FEAT_MATK>(a, w, mats + w[1], tid);
+      }
+      if (active) {
+        float2* dst = st + base;
+        asm volatile("" : "+v"(dst));
+#pragma unroll
+        for (int j = 0; j < NS; ++j) dst[roff[j]] = a[j];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// <Z-string> per circuit: sum_j |psi_j|^2 (-1)^{popcount(j & zmask)}; one
+// partial per (circuit, chunk of 2^14 amplitudes), then a per-circuit sum.
+struct ZCirc {
+  int64_t state_off;
+  int32_t n;
+  uint32_t zmask;
+  int64_t part_off;    // first partial slot of this circuit
+};
+
+constexpr int ZCHUNK_LG = 14;
+
+__global__ __launch_bounds__(256) void k_batch_zpart(const float2* __restrict__ states,
+                                                     const ZCirc* __restrict__ circ, const int64_t* __restrict__ chunk_circ,
+                                                     int64_t n_chunks, float* __restrict__ part) {
+  __shared__ float red[4];
+  for (int64_t ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
+    const int c = (int)chunk_circ[ch];
+    const ZCirc z = circ[c];
+    const int64_t local = ch - z.part_off;
+    const int64_t first = local << ZCHUNK_LG;
+    const int64_t cnt = ((int64_t)1 << z.n) - first < ((int64_t)1 << ZCHUNK_LG) ? ((int64_t)1 << z.n) - first
+                                                                               : ((int64_t)1 << ZCHUNK_LG);
+    float acc = 0.f;
+    for (int64_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const uint32_t j = (uint32_t)(first + i);
+      const float2 v = states[z.state_off + first + i];
+      const float p = v.x * v.x + v.y * v.y;
+      acc += (__popc(j & z.zmask) & 1) ? -p : p;
+    }
+    acc = wave_sum(acc, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[ch] = red[0] + red[1] + red[2] + red[3];
+  }
+}
+
+__global__ void k_batch_zsum(const ZCirc* __restrict__ circ, int n_circ, const float* __restrict__ part,
+                             float* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_circ) return;
+  const ZCirc z = circ[c];
+  const int64_t nch = (((int64_t)1 << z.n) + ((int64_t)1 << ZCHUNK_LG) - 1) >> ZCHUNK_LG;
+  double s = 0.0;
+  for (int64_t i = 0; i < nch; ++i) s += part[z.part_off + i];
+  out[c] = (float)s;
 }
 
 }  // namespace tqsv
@@ -575,7 +816,13 @@ TQSV_API int tqsv_stage_forward(hipStream_t s, void* state_out, const void* stat
   int rc = stage_launch_shape(prog_host, n, bsz, grid_cap, T, n_groups, grid, gpw, shm);
   if (rc) return rc;
   StageArgs A{prog_dev, (const float2*)mats, n, init_zero, n_groups, gpw, nullptr, 0};
-  k_stage_fwd<<<grid, 1 << T, shm, s>>>((float2*)state_out, (const float2*)state_in, A);
+  const int feat = prog_host[H_FLAGS] & 3;
+  if (feat == 0)
+    k_stage_fwd<0><<<grid, 1 << T, shm, s>>>((float2*)state_out, (const float2*)state_in, A);
+  else if (feat == FEAT_MAT2)
+    k_stage_fwd<FEAT_MAT2><<<grid, 1 << T, shm, s>>>((float2*)state_out, (const float2*)state_in, A);
+  else
+    k_stage_fwd<FEAT_MAT2 | FEAT_MATK><<<grid, 1 << T, shm, s>>>((float2*)state_out, (const float2*)state_in, A);
   return check_launch("tqsv_stage_forward");
 }
 
@@ -596,6 +843,48 @@ TQSV_API int tqsv_stage_adjoint(hipStream_t s, void* psi, void* lam, int64_t bsz
   int rc = stage_launch_shape(prog_host, n, bsz, grid_cap, T, n_groups, grid, gpw, shm);
   if (rc) return rc;
   StageArgs A{prog_dev, (const float2*)mats, n, 0, n_groups, gpw, grad, grad_stride};
-  k_stage_adj<<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, A);
+  const int feat = prog_host[H_FLAGS] & 3;
+  if (feat == 0)
+    k_stage_adj<0><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, A);
+  else if (feat == FEAT_MAT2)
+    k_stage_adj<FEAT_MAT2><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, A);
+  else
+    k_stage_adj<FEAT_MAT2 | FEAT_MATK><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, A);
   return check_launch("tqsv_stage_adjoint");
+}
+
+// One launch over a list of (circuit, stage, group range) items; every item's
+// stage program must have T <= 8 (m <= 12).  items_dev: BatchItem array;
+// progs_dev: concatenated stage programs; mats: matrix pool.
+TQSV_API int tqsv_batch_forward(hipStream_t s, void* states, const void* items_dev, int n_items,
+                                const int* progs_dev, const void* mats, int max_m, int grid) {
+  if (!states || !items_dev || !progs_dev || !mats || n_items < 0) {
+    set_error("tqsv_batch_forward: null pointer argument");
+    return E_ARG;
+  }
+  if (max_m < R || max_m > 12) { set_error("tqsv_batch_forward: max_m must be in [4, 12]"); return E_LIMIT; }
+  if (n_items == 0) return OK;
+  if (grid <= 0) grid = n_items < num_cu() * 4 ? n_items : num_cu() * 4;
+  const size_t shm = sizeof(float2) << max_m;
+  k_batch_fwd<<<grid, BATCH_THREADS, shm, s>>>((float2*)states, (const BatchItem*)items_dev, n_items,
+                                               progs_dev, (const float2*)mats);
+  return check_launch("tqsv_batch_forward");
+}
+
+TQSV_API int tqsv_batch_item_bytes(void) { return (int)sizeof(BatchItem); }
+TQSV_API int tqsv_batch_zcirc_bytes(void) { return (int)sizeof(ZCirc); }
+
+// Z-string expectation per circuit (circ_dev: ZCirc array, chunk_circ_dev:
+// circuit index of every 2^14-amplitude chunk, part: n_chunks floats).
+TQSV_API int tqsv_batch_zexpval(hipStream_t s, const void* states, const void* circ_dev, int n_circ,
+                                const int64_t* chunk_circ_dev, int64_t n_chunks, float* part, float* out) {
+  if (!states || !circ_dev || !chunk_circ_dev || !part || !out) { set_error("tqsv_batch_zexpval: null pointer"); return E_ARG; }
+  if (n_circ <= 0 || n_chunks <= 0) return OK;
+  const int64_t g = n_chunks < (int64_t)num_cu() * 8 ? n_chunks : (int64_t)num_cu() * 8;
+  k_batch_zpart<<<(unsigned)g, 256, 0, s>>>((const float2*)states, (const ZCirc*)circ_dev, chunk_circ_dev, n_chunks,
+                                            part);
+  int rc = check_launch("tqsv_batch_zexpval/part");
+  if (rc) return rc;
+  k_batch_zsum<<<(n_circ + 127) / 128, 128, 0, s>>>((const ZCirc*)circ_dev, n_circ, part, out);
+  return check_launch("tqsv_batch_zexpval/sum");
 }

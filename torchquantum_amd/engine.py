@@ -41,6 +41,8 @@ MAX_P_PER_OP = 4
 # stage header layout (must match tqsv_stage.hip)
 H_NOPS, H_M, H_T, H_NNL, H_OFF_OPS, H_NGP, H_OFF_GSLOT, H_FLAGS, H_POS = range(9)
 OP_MAT1, OP_MAT2, OP_DIAG, OP_MATK, OP_REMAP = 1, 2, 3, 4, 5
+F_BATCHED, F_NOCTRL, F_REAL, F_XPERM, F_SIGN = 1 << 8, 1 << 9, 1 << 10, 1 << 11, 1 << 12
+FEAT_MAT2, FEAT_MATK = 1, 2
 
 
 # ----------------------------------------------------------------------------
@@ -54,16 +56,19 @@ class EngineOptions:
     stage_bits:  m, local bits per stage (R_BITS + log2 threads per group); 0 = auto.
     low_bits:    lowest bit positions always kept local (coalescing).
     merge_1q:    multiply runs of single-wire gates into one 2x2 (fused mode).
+    fusion_width: k; >= 2 pre-multiplies runs of gates acting within k wires
+                 into one dense 2^k x 2^k block (k <= 4; trainable blocks k <= 2).
     grid_cap:    workgroups per stage launch (0 = auto).
     """
     mode: str = "fused"
     stage_bits: int = 0
     low_bits: int = 3
     merge_1q: bool = True
+    fusion_width: int = 1
     grid_cap: int = 0
 
     def key(self):
-        return (self.mode, self.stage_bits, self.low_bits, self.merge_1q, self.grid_cap)
+        return (self.mode, self.stage_bits, self.low_bits, self.merge_1q, self.fusion_width, self.grid_cap)
 
 
 DEFAULT_OPTIONS = EngineOptions()
@@ -114,6 +119,10 @@ class KOp:
     diag_idx: Tuple[int, int] = (0, 0)
     grad_slots: Tuple[int, ...] = ()  # flat parameter slots (one per kernel parameter)
     batched: bool = False
+    n_for_block: int = 0            # qubit count (blocks map tape wires to bit positions)
+    real: bool = False              # matrix entries are real for every parameter value
+    xperm: bool = False             # the 2x2 is Pauli X (a controlled exchange)
+    sign: bool = False              # diagonal phase -1 where the controls match, no parity
 
     @property
     def cmask(self) -> int:
@@ -148,6 +157,12 @@ class KOp:
         return 1 << len(self.bits)
 
 
+# gate structure that lets the stage kernel take a cheaper path
+X_GATES = {"paulix", "cnot", "toffoli", "c3x", "c4x", "multicnot", "multixcnot"}
+SIGN_GATES = {"pauliz", "cz", "ccz"}
+REAL_1Q_BLOCK = X_GATES | {"ry", "hadamard", "shadamard", "cry", "chadamard", "pauliz", "i"}
+
+
 def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]) -> List[KOp]:
     """Tape ops -> kernel ops (no fusion).  Identity gates vanish."""
     out: List[KOp] = []
@@ -168,11 +183,13 @@ def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]
             cidx, pidx = G.diag_form(s, k)
             out.append(KOp("diag", tuple(pos[i] for i in pidx), tuple(pos[i] for i in cidx),
                            tuple(1 for _ in cidx), (ti,), "diag",
-                           G.diag_phase_indices(cidx, pidx, k), slots, batched))
+                           G.diag_phase_indices(cidx, pidx, k), slots, batched,
+                           sign=s.name in SIGN_GATES, real=s.name in SIGN_GATES))
         elif s.kind == "ctrl1":
             ci, cv, t = G.ctrl_values(s, k)
             out.append(KOp("mat1", (pos[t],), tuple(pos[i] for i in ci), tuple(cv), (ti,),
-                           "ctrl0" if s.cval0 else "ctrl1", (0, 0), slots, batched))
+                           "ctrl0" if s.cval0 else "ctrl1", (0, 0), slots, batched,
+                           real=s.name in REAL_1Q_BLOCK, xperm=s.name in X_GATES))
         elif s.kind == "swap":
             out.append(KOp("swap", (pos[-2], pos[-1]), tuple(pos[:-2]), tuple(1 for _ in pos[:-2]), (ti,),
                            "swap", (0, 0), (), False))
@@ -181,7 +198,8 @@ def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]
                 raise NotImplementedError("gradients through a matrix-valued gate parameter "
                                           "(qubitunitary*) are not supported")
             kind = "mat1" if k == 1 else ("mat2" if k == 2 else "matk")
-            out.append(KOp(kind, tuple(pos), (), (), (ti,), "full", (0, 0), slots, batched))
+            out.append(KOp(kind, tuple(pos), (), (), (ti,), "full", (0, 0), slots, batched,
+                           real=k == 1 and s.name in REAL_1Q_BLOCK, xperm=k == 1 and s.name in X_GATES))
     return out
 
 
@@ -209,9 +227,10 @@ def merge_single_wire(kops: List[KOp], tape: Sequence[TapeOp]) -> List[KOp]:
                 run = out[j]
                 if run.P + k.P <= MAX_P_PER_OP:
                     out[j] = KOp("mat1", (b,), (), (), run.src + k.src, "chain", (0, 0),
-                                 run.grad_slots + k.grad_slots, run.batched or k.batched)
+                                 run.grad_slots + k.grad_slots, run.batched or k.batched,
+                                 real=run.real and k.real)
                     continue
-            out.append(KOp("mat1", (b,), (), (), k.src, "chain", (0, 0), k.grad_slots, k.batched)
+            out.append(KOp("mat1", (b,), (), (), k.src, "chain", (0, 0), k.grad_slots, k.batched, real=k.real)
                        if k.kind == "diag" else k)
             pending[b] = len(out) - 1
         else:
@@ -221,55 +240,176 @@ def merge_single_wire(kops: List[KOp], tape: Sequence[TapeOp]) -> List[KOp]:
     return [k for k in out if k is not None]
 
 
+def fuse_blocks(kops: List[KOp], tape: Sequence[TapeOp], n: int, width: int) -> List[KOp]:
+    """Greedy gate fusion into dense blocks of <= ``width`` bits.
+
+    An op joins the block that last touched its bits when exactly one block
+    did (so no op in between shares a bit with it and it commutes forward)
+    and the union stays within ``width`` bits.  Multi-member blocks become
+    one dense op (mat1 / mat2 / matk) whose matrix is the ordered product of
+    its members embedded in the block's bits.  Blocks with trainable members
+    are kept to <= 2 bits and <= MAX_P_PER_OP parameters (the widths whose
+    kernels return parameter gradients)."""
+    width = max(1, min(width, R_BITS))
+    blocks: List[dict] = []
+    last: Dict[int, int] = {}
+    for i, k in enumerate(kops):
+        bs = set(k.allbits)
+        if len(k.src) != 1 or (k.kind == "matk" and len(k.bits) > R_BITS) or len(bs) > width:
+            blocks.append(dict(bits=bs, members=[i], solo=True, P=k.P))
+            for b in bs:
+                last[b] = len(blocks) - 1
+            continue
+        cands = {last[b] for b in bs if b in last}
+        if len(cands) == 1:
+            j = cands.pop()
+            B = blocks[j]
+            nb = B["bits"] | bs
+            P = B["P"] + k.P
+            lim = 2 if P > 0 else width
+            if not B["solo"] and len(nb) <= min(width, lim) and P <= MAX_P_PER_OP:
+                B["bits"] = nb
+                B["members"].append(i)
+                B["P"] = P
+                for b in bs:
+                    last[b] = j
+                continue
+        blocks.append(dict(bits=bs, members=[i], solo=False, P=k.P))
+        for b in bs:
+            last[b] = len(blocks) - 1
+    out = []
+    for B in blocks:
+        if len(B["members"]) == 1:
+            out.append(kops[B["members"][0]])
+            continue
+        bits = tuple(sorted(B["bits"], reverse=True))       # matrix MSB first
+        kind = "mat1" if len(bits) == 1 else ("mat2" if len(bits) == 2 else "matk")
+        src = tuple(kops[m].src[0] for m in B["members"])
+        slots = tuple(s for m in B["members"] for s in kops[m].grad_slots)
+        batched = any(kops[m].batched for m in B["members"])
+        out.append(KOp(kind, bits, (), (), src, "block", (0, 0), slots, batched))
+    return out
+
+
+def embed(u: np.ndarray, op_bits: Sequence[int], block_bits: Sequence[int]) -> np.ndarray:
+    """[rows, 2^k, 2^k] on op_bits (MSB first) -> [rows, 2^K, 2^K] on block_bits."""
+    k, K = len(op_bits), len(block_bits)
+    rest = [b for b in block_bits if b not in op_bits]
+    rows = u.shape[0]
+    full = np.kron(u, np.eye(1 << (K - k))[None]) if K > k else u          # order: op_bits + rest
+    order = list(op_bits) + rest
+    perm = [order.index(b) for b in block_bits]
+    t = full.reshape([rows] + [2] * (2 * K))
+    t = t.transpose([0] + [1 + p for p in perm] + [1 + K + p for p in perm])
+    return t.reshape(rows, 1 << K, 1 << K)
+
+
 # ----------------------------------------------------------------------------
 # matrix values (host, float64)
 # ----------------------------------------------------------------------------
 class GateValues:
-    """Per tape op: U [rows, d, d] complex128 and, if trainable, D [rows, P, d, d]."""
+    """Gate matrices for the current parameter values, evaluated on the host.
+
+    Tape ops are grouped by (gate, width, has-params); each group's matrices
+    are built in one vectorised call into ``UG[key]`` ([rows, d, d]
+    complex128, inverse ops already conjugate-transposed) and, when any
+    member is trainable, derivative matrices ``DG[key]`` ([rows, P, d, d]).
+    ``gkey[i]``, ``grow[i]``, ``rows[i]`` locate tape op i; ``U[i]`` / ``D[i]``
+    are views.  Parameters living on the GPU are fetched with one copy."""
 
     def __init__(self, tape: Sequence[TapeOp], want_grad: bool):
-        self.U: List[np.ndarray] = [None] * len(tape)
-        self.D: List[Optional[np.ndarray]] = [None] * len(tape)
+        T = len(tape)
+        self.U: List[np.ndarray] = [None] * T
+        self.D: List[Optional[np.ndarray]] = [None] * T
+        self.gkey: List[tuple] = [None] * T
+        self.grow = np.zeros(T, dtype=np.int64)
+        self.rows = np.ones(T, dtype=np.int64)
+        self.UG: Dict[tuple, np.ndarray] = {}
+        self.DG: Dict[tuple, Optional[np.ndarray]] = {}
+        host = _host_params(tape)
         groups: Dict[tuple, List[int]] = {}
         for i, op in enumerate(tape):
             if op.is_matrix_param:
-                m = op.params.detach().to("cpu", torch.complex128).numpy()
+                m = host[i].astype(np.complex128)
                 m = m.reshape((-1,) + m.shape[-2:]) if m.ndim == 3 else m[None]
                 if op.inverse:
                     m = np.conj(np.swapaxes(m, -1, -2))
+                key = ("__mat__", i)
+                self.UG[key] = m
+                self.DG[key] = None
+                self.gkey[i] = key
+                self.rows[i] = m.shape[0]
                 self.U[i] = m
                 continue
             key = (op.name, len(op.wires), op.n_params() > 0)
             groups.setdefault(key, []).append(i)
-        for (name, k, has_p), idx in groups.items():
+        for key, idx in groups.items():
+            name, k, has_p = key
             builder = G.width_builder(name, k)
             if has_p:
-                ps = [tape[i].params.detach().to("cpu", torch.float64) for i in idx]
+                ps = [host[i] for i in idx]
                 rows = [p.shape[0] for p in ps]
                 P = ps[0].shape[-1]
-                allp = torch.cat([p.reshape(-1, P) for p in ps], dim=0)
+                allp = torch.from_numpy(np.concatenate([p.reshape(-1, P) for p in ps], axis=0))
                 U = builder(allp).numpy()
-                need_d = want_grad and any(tape[i].wants_grad() for i in idx)
-                if need_d:
+                D = None
+                if want_grad and any(tape[i].wants_grad() for i in idx):
                     def one(q, _b=builder):
                         return torch.view_as_real(_b(q.unsqueeze(0))[0])
                     J = torch.func.vmap(torch.func.jacfwd(one))(allp)        # [N, d, d, 2, P]
                     D = torch.complex(J[..., 0, :], J[..., 1, :]).movedim(-1, 1).numpy()  # [N,P,d,d]
+                inv = np.zeros(U.shape[0], dtype=bool)
                 off = 0
                 for i, r in zip(idx, rows):
-                    u = U[off:off + r]
-                    d = D[off:off + r] if need_d and tape[i].wants_grad() else None
+                    self.grow[i], self.rows[i] = off, r
                     if tape[i].inverse:
-                        u = np.conj(np.swapaxes(u, -1, -2))
-                        if d is not None:
-                            d = np.conj(np.swapaxes(d, -1, -2))
-                    self.U[i] = u
-                    self.D[i] = d
+                        inv[off:off + r] = True
                     off += r
+                if inv.any():
+                    U = U.copy()
+                    U[inv] = np.conj(np.swapaxes(U[inv], -1, -2))
+                    if D is not None:
+                        D = D.copy()
+                        D[inv] = np.conj(np.swapaxes(D[inv], -1, -2))
+                self.UG[key], self.DG[key] = U, D
+                for i in idx:
+                    g, r = self.grow[i], self.rows[i]
+                    self.U[i] = U[g:g + r]
+                    self.D[i] = D[g:g + r] if (D is not None and tape[i].wants_grad()) else None
+                    self.gkey[i] = key
             else:
                 u = builder(torch.zeros(1, 1, dtype=torch.float64)).numpy()
+                U = np.concatenate([u, np.conj(np.swapaxes(u, -1, -2))], axis=0)   # row 0: U, row 1: U^dag
+                self.UG[key], self.DG[key] = U, None
                 for i in idx:
-                    self.U[i] = np.conj(np.swapaxes(u, -1, -2)) if tape[i].inverse else u
+                    self.grow[i] = 1 if tape[i].inverse else 0
+                    self.U[i] = U[self.grow[i]:self.grow[i] + 1]
+                    self.gkey[i] = key
+
+
+def _host_params(tape: Sequence[TapeOp]) -> List[Optional[np.ndarray]]:
+    """Parameter values of every tape op as numpy (float64 or complex128);
+    tensors on a GPU are gathered with one concatenation and one copy."""
+    out: List[Optional[np.ndarray]] = [None] * len(tape)
+    dev: Dict[tuple, List[int]] = {}
+    for i, op in enumerate(tape):
+        if op.params is None:
+            continue
+        p = op.params.detach()
+        if p.device.type == "cpu":
+            out[i] = p.numpy().astype(np.complex128 if p.is_complex() else np.float64)
+        else:
+            dev.setdefault((p.device, p.dtype), []).append(i)
+    for (d, dt), idx in dev.items():
+        flat = torch.cat([tape[i].params.detach().reshape(-1) for i in idx]).cpu().numpy()
+        flat = flat.astype(np.complex128 if np.iscomplexobj(flat) else np.float64)
+        off = 0
+        for i in idx:
+            sh = tuple(tape[i].params.shape)
+            sz = int(np.prod(sh))
+            out[i] = flat[off:off + sz].reshape(sh)
+            off += sz
+    return out
 
 
 def _block(u: np.ndarray, extract: str) -> np.ndarray:
@@ -298,6 +438,8 @@ def kop_data(k: KOp, gv: GateValues, tape: Sequence[TapeOp], with_grad: bool) ->
         return ph
     if k.extract == "chain":
         return _chain_data(k, gv, tape, with_grad)
+    if k.extract == "block":
+        return _block_data(k, gv, tape, with_grad)
     ti = k.src[0]
     u = _block(gv.U[ti], k.extract)
     d2 = u.shape[-1] ** 2
@@ -316,6 +458,38 @@ def _as2x2(k_src: int, gv: GateValues, tape, with_grad):
     d = gv.D[k_src] if with_grad and op.wants_grad() else None
     ds = [] if d is None else [d[:, p] for p in range(d.shape[1])]
     return u, ds
+
+
+def _block_data(k: KOp, gv: GateValues, tape, with_grad) -> np.ndarray:
+    n_bits = len(k.bits)
+    mats, ders = [], []
+    for mi, ti in enumerate(k.src):
+        op = tape[ti]
+        nq = k.n_for_block
+        ob = [nq - 1 - w for w in op.wires]
+        mats.append(embed(gv.U[ti], ob, k.bits))
+        if with_grad and op.wants_grad():
+            d = gv.D[ti]
+            for p in range(d.shape[1]):
+                ders.append((mi, embed(d[:, p], ob, k.bits)))
+    return _product_data(mats, ders, 1 << n_bits, with_grad)
+
+
+def _product_data(mats, ders, dim, with_grad) -> np.ndarray:
+    rows = max(m.shape[0] for m in mats)
+    ident = np.broadcast_to(np.eye(dim, dtype=np.complex128), (rows, dim, dim))
+    prefix = [ident]
+    for u in mats:
+        prefix.append(np.matmul(u, prefix[-1]))
+    parts = [prefix[-1].reshape(rows, 1, dim * dim)]
+    if with_grad and ders:
+        suffix = [ident] * (len(mats) + 1)
+        for i in range(len(mats) - 1, -1, -1):
+            suffix[i] = np.matmul(suffix[i + 1], mats[i])
+        for mi, d in ders:
+            dd = np.matmul(np.matmul(suffix[mi + 1], d), prefix[mi])
+            parts.append(np.broadcast_to(dd, (rows, dim, dim)).reshape(rows, 1, dim * dim))
+    return np.concatenate([np.broadcast_to(p, (rows,) + p.shape[1:]) for p in parts], axis=1)
 
 
 def _chain_data(k: KOp, gv: GateValues, tape, with_grad) -> np.ndarray:
@@ -496,6 +670,7 @@ def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off
     ops_words = []
     remap_words = []
     enc = []
+    feat = 0
     n_ops = len(seq)
     off_gslot = H_POS + n
     # gslot table size known after the loop; ops after it; remap records last
@@ -561,7 +736,18 @@ def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off
             w[3] = tpm
             rm |= 0
             w[6] = (rpm & 255) << 16
-        w[0] |= (P << 4) | ((256 if k.batched else 0))
+        w[0] |= (P << 4) | (F_BATCHED if k.batched else 0)
+        if k.kind == "mat1":
+            if k.xperm and P == 0:
+                w[0] |= F_XPERM
+            elif tm == 0 and rm == 0:
+                w[0] |= F_NOCTRL | (F_REAL if k.real else 0)
+        elif k.kind == "diag" and k.sign and not k.bits and P == 0:
+            w[0] |= F_SIGN
+        if k.kind in ("mat2", "swap"):
+            feat |= FEAT_MAT2
+        elif k.kind == "matk":
+            feat |= FEAT_MAT2 | FEAT_MATK
         w[1] = off
         w[2] = bstride
         w[4], w[5] = tm, tv
@@ -581,6 +767,7 @@ def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off
         if w[0] == OP_REMAP:
             w[7] += off_remap
     header[H_NOPS] = n_ops
+    header[H_FLAGS] = feat
     header[H_OFF_OPS] = off_ops
     header[H_NGP] = len(gslots)
     header[H_OFF_GSLOT] = off_gslot
@@ -605,7 +792,12 @@ class Compiled:
             off += op.n_params()
         self.n_slots = off
         kops = lower(tape, n, bsz, self.param_offsets)
-        if opts.mode == "fused" and opts.merge_1q:
+        if opts.mode == "fused" and opts.fusion_width >= 2:
+            kops = fuse_blocks(kops, tape, n, opts.fusion_width)
+            for k in kops:
+                if k.extract == "block":
+                    k.n_for_block = n
+        elif opts.mode == "fused" and opts.merge_1q:
             kops = merge_single_wire(kops, tape)
         self.kops = kops
         # pool layout: kop data (fwd, with derivative matrices when trainable)
@@ -629,6 +821,7 @@ class Compiled:
         self.pool_len = max(pos, 1)
         self.steps: List[tuple] = []
         self.perms: Dict[int, np.ndarray] = {}
+        self._fill = None
         if opts.mode == "fused" and n >= R_BITS:
             m = choose_stage_bits(n, opts)
             slot_of = {i: list(k.grad_slots) for i, k in enumerate(kops)}
@@ -662,37 +855,133 @@ class Compiled:
         """The matrix pool (complex64) for the current parameter values."""
         gv = GateValues(tape, with_grad)
         host = np.zeros(self.pool_len, dtype=np.complex64)
-        for i, k in enumerate(self.kops):
-            d = kop_data(k, gv, tape, with_grad and k.P > 0)       # [rows, 1+P, dd]
-            perm = self.perms.get(i)
-            if perm is not None:
-                dd = int(round(math.sqrt(d.shape[-1])))
-                d = d.reshape(d.shape[0], d.shape[1], dd, dd)[:, :, perm][:, :, :, perm]
-                d = d.reshape(d.shape[0], d.shape[1], dd * dd)
-            rows = self.bsz if k.batched else 1
-            if not (with_grad and k.P > 0) and k.P > 0:
-                # derivative slots unused in a forward-only pass: leave zeros
-                d = d[:, :1]
-            d = np.broadcast_to(d, (rows,) + d.shape[1:])
-            off, _ = self.data_off[i]
-            flat = d.reshape(rows, -1)
-            per = self.data_len[i] // rows
-            blk = host[off:off + per * rows].reshape(rows, per)
-            blk[:, :flat.shape[1]] = flat
-            if with_adj and i in self.adj_off:
-                aoff, _ = self.adj_off[i]
-                if k.kind == "diag":
-                    a = np.conj(d[:, 0])
-                else:
-                    dd = k.dim()
-                    u = d[:, 0].reshape(rows, dd, dd)
+        if self._fill is None:
+            self._fill = self._build_fill(tape)
+        singles, chains, others = self._fill
+        for key, (tidx, poff, aoff) in singles:
+            gkey, extract, didx, perm, P = key
+            rows = gv.grow[tidx]
+            U = gv.UG[gkey][rows]
+            if extract == "diag":
+                a, b = didx
+                data = np.stack([U[:, a, a], U[:, b, b]], axis=-1)                     # [G, 2]
+                adj = np.conj(data)
+                if with_grad and P:
+                    D = gv.DG[gkey][rows]
+                    data = np.concatenate([data, np.stack([D[:, :, a, a], D[:, :, b, b]], -1).reshape(len(tidx), -1)], 1)
+            else:
+                M = _block(U, extract)
+                if perm is not None:
+                    M = M[:, perm][:, :, perm]
+                dd = M.shape[-1]
+                data = M.reshape(len(tidx), dd * dd)
+                adj = None
+                if aoff is not None:
+                    Mu = _block(U, extract)
+                    adj = np.conj(np.swapaxes(Mu, -1, -2)).reshape(len(tidx), dd * dd)
+                if with_grad and P:
+                    D = _block(gv.DG[gkey][rows], extract)
                     if perm is not None:
-                        # adjoint data is used by per-gate kernels: un-permuted
-                        inv = np.argsort(perm)
-                        u = u[:, inv][:, :, inv]
-                    a = np.conj(np.swapaxes(u, -1, -2)).reshape(rows, -1)
-                host[aoff:aoff + a.size] = a.reshape(-1)
+                        D = D[:, :, perm][:, :, :, perm]
+                    data = np.concatenate([data, D.reshape(len(tidx), -1)], 1)
+            host[poff[:, None] + np.arange(data.shape[1])] = data
+            if with_adj and aoff is not None:
+                host[aoff[:, None] + np.arange(adj.shape[1])] = adj
+        for key, (tidx, poff, aoff) in chains:
+            gkeys, Ps = key
+            G_, L = tidx.shape
+            mats = [gv.UG[gkeys[m]][gv.grow[tidx[:, m]]] for m in range(L)]
+            eye = np.broadcast_to(np.eye(2, dtype=np.complex128), (G_, 2, 2))
+            pre = [eye]
+            for u in mats:
+                pre.append(np.matmul(u, pre[-1]))
+            parts = [pre[-1].reshape(G_, 4)]
+            if with_grad and sum(Ps):
+                suf = [eye] * (L + 1)
+                for m in range(L - 1, -1, -1):
+                    suf[m] = np.matmul(suf[m + 1], mats[m])
+                for m in range(L):
+                    if Ps[m]:
+                        D = gv.DG[gkeys[m]][gv.grow[tidx[:, m]]]
+                        for q in range(Ps[m]):
+                            parts.append(np.matmul(np.matmul(suf[m + 1], D[:, q]), pre[m]).reshape(G_, 4))
+            data = np.concatenate(parts, axis=1)
+            host[poff[:, None] + np.arange(data.shape[1])] = data
+            if with_adj and aoff is not None:
+                adj = np.conj(np.swapaxes(pre[-1], -1, -2)).reshape(G_, 4)
+                host[aoff[:, None] + np.arange(4)] = adj
+        for i in others:
+            self._fill_one(i, tape, gv, host, with_grad, with_adj)
         return host
+
+    def _build_fill(self, tape):
+        """Group kernel ops by how their pool data is formed, once per plan."""
+        singles: Dict[tuple, list] = {}
+        chains: Dict[tuple, list] = {}
+        others: List[int] = []
+        gk = [None] * len(tape)
+        for i, op in enumerate(tape):
+            gk[i] = ("__mat__", i) if op.is_matrix_param else (op.name, len(op.wires), op.n_params() > 0)
+        for i, k in enumerate(self.kops):
+            aoff = self.adj_off.get(i, (None, 0))[0]
+            if k.batched or k.kind == "swap" or k.extract == "block" or any(tape[t].is_matrix_param for t in k.src):
+                others.append(i)
+                continue
+            if k.extract == "chain":
+                key = (tuple(gk[t] for t in k.src), tuple(len(_member_slots(tape[t])) for t in k.src))
+                chains.setdefault(key, []).append((list(k.src), self.data_off[i][0], aoff))
+                continue
+            perm = self.perms.get(i)
+            key = (gk[k.src[0]], k.extract, tuple(k.diag_idx), None if perm is None else tuple(perm), k.P)
+            singles.setdefault(key, []).append((k.src[0], self.data_off[i][0], aoff))
+
+        def pack(d):
+            out = []
+            for key, lst in d.items():
+                tidx = np.array([x[0] for x in lst], dtype=np.int64)
+                poff = np.array([x[1] for x in lst], dtype=np.int64)
+                ao = [x[2] for x in lst]
+                aoff = None if any(a is None for a in ao) else np.array(ao, dtype=np.int64)
+                out.append((key, (tidx, poff, aoff)))
+            return out
+
+        singles_l = [((g, e, di, None if pm is None else np.array(pm), P), v)
+                     for (g, e, di, pm, P), v in pack(singles)]
+        return singles_l, pack(chains), others
+
+    def _fill_one(self, i, tape, gv, host, with_grad, with_adj):
+        k = self.kops[i]
+        d = kop_data(k, gv, tape, with_grad and k.P > 0)       # [rows, 1+P, dd]
+        perm = self.perms.get(i)
+        if perm is not None:
+            dd = int(round(math.sqrt(d.shape[-1])))
+            d = d.reshape(d.shape[0], d.shape[1], dd, dd)[:, :, perm][:, :, :, perm]
+            d = d.reshape(d.shape[0], d.shape[1], dd * dd)
+        rows = self.bsz if k.batched else 1
+        if not (with_grad and k.P > 0):
+            d = d[:, :1]
+        d = np.broadcast_to(d, (rows,) + d.shape[1:])
+        off, _ = self.data_off[i]
+        flat = d.reshape(rows, -1)
+        per = self.data_len[i] // rows
+        blk = host[off:off + per * rows].reshape(rows, per)
+        blk[:, :flat.shape[1]] = flat
+        if with_adj and i in self.adj_off:
+            aoff, _ = self.adj_off[i]
+            if k.kind == "diag":
+                a = np.conj(d[:, 0])
+            else:
+                dd = k.dim()
+                u = d[:, 0].reshape(rows, dd, dd)
+                if perm is not None:
+                    inv = np.argsort(perm)
+                    u = u[:, inv][:, :, inv]
+                a = np.conj(np.swapaxes(u, -1, -2)).reshape(rows, -1)
+            host[aoff:aoff + a.size] = a.reshape(-1)
+
+
+def _member_slots(op: TapeOp):
+    return range(op.n_params()) if op.wants_grad() else range(0)
 
 
 _cache_lock = threading.Lock()
