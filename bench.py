@@ -133,6 +133,56 @@ def kernel_roofline(n, bsz, iters=50):
     return ms, nbytes
 
 
+def time_batch(n_circ, steps, tq):
+    """256 circuits, n ~ U{8..22}, depth ~ U{4..40}: level-synchronous batch
+    launches captured in a HIP graph vs one engine call per circuit."""
+    from torchquantum_amd import batch as B
+    import torchquantum_amd.functional as tqf
+    circs = B.random_circuits(n_circ=n_circ, n_range=(8, 22), depth_range=(4, 40), seed=0)
+    upd = float(sum(len(ops) * (1 << n) for n, ops in circs))
+    t0 = time.perf_counter()
+    cb = B.CircuitBatch(circs, device="cuda")
+    t_build = time.perf_counter() - t0
+    cb.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cb.launch()
+    torch.cuda.synchronize()
+    t_eager = (time.perf_counter() - t0) / steps
+    cb.capture()
+    cb.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = cb.run()
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / steps
+    # one engine call per circuit (fused plans, same kernels, Python loop over circuits)
+    def per_circuit():
+        res = []
+        for n, ops in circs:
+            q = tq.QuantumDevice(n_wires=n, bsz=1, device="cuda")
+            for op in ops:
+                p = op["params"]
+                tqf.func_name_dict[op["name"]](q, op["wires"], params=None if p is None else
+                                               torch.tensor(p, dtype=torch.float32).reshape(1, -1))
+            res.append(tq.MeasureAll(tq.PauliZ)(q)[0, 0])
+        return torch.stack(res)
+    ref = per_circuit()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = per_circuit()
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    return {"workload": f"{n_circ} circuits, n ~ U{{8..22}}, depth ~ U{{4..40}} layers of n random gates "
+                        "from {H,RX,RY,RZ,U3,CNOT,CZ,SWAP}, <Z_0> each, bsz 1",
+            "levels": len(cb.levels), "launches_per_batch": cb.n_launches,
+            "ms_graph_replay": t_graph * 1e3, "ms_eager_batch": t_eager * 1e3, "ms_per_circuit_loop": t_loop * 1e3,
+            "amp_updates_per_s_graph": upd / t_graph, "speedup_vs_per_circuit_loop": t_loop / t_graph,
+            "build_s": t_build, "z0_max_abs_diff_vs_per_circuit": float((out - ref).abs().max())}
+
+
 def cpu_baseline(ops_sample, n, bsz, n_gates_total_per_step):
     from oracle import refpath
     threads = os.cpu_count() or 1
@@ -154,6 +204,7 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=2)
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--cpu-layers", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -200,6 +251,11 @@ def main():
             "grad_max_abs_diff_fused_vs_pergate": float((g_f - g_p).abs().max()),
             "loss": loss_f,
         }
+
+    if not args.skip_batch:
+        log("[bench] uneven batch (256 circuits)")
+        extra["batch"] = time_batch(256, 5, tq)
+        log(f"[bench] batch graph {extra['batch']['ms_graph_replay']:.2f} ms")
 
     cpu = None
     if not args.skip_cpu:

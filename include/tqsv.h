@@ -109,6 +109,37 @@ int tqsv_expval_pauli(tqsv_stream_t s, const void* state, int64_t bsz, int n, in
 int tqsv_pauli_apply(tqsv_stream_t s, void* out, const void* in, int64_t bsz, int n, int nterms,
                      const uint64_t* xmask, const uint64_t* zmask, const void* coef, int accumulate);
 
+/* Fold matrix gradients (ops flagged F_MG in a stage program: 8 floats
+ * sum(lam conj(psi)^T) per op) into parameter gradients:
+ * grad[row, slot_p] += Re sum_ab D_p[a][b] conj(Mg[a][b]); items: MgItem array
+ * (layout: tqsv_mg_item_bytes()).  Part of the adjoint sweep above. */
+int tqsv_mg_item_bytes(void);
+int tqsv_mg_to_grad(tqsv_stream_t s, float* grad, int64_t grad_stride, int64_t bsz, const void* items_dev,
+                    int n_items, const void* mats);
+
+/* ---- many circuits (bsz 1 each, different widths) in one launch: the work
+ *      list holds (circuit, stage program, group range) items (BatchItem,
+ *      tqsv_batch_item_bytes()); all items' programs have m <= max_m <= 12.
+ *      Replaces a Python loop of per-circuit gate_wrapper calls. ---------- */
+int tqsv_batch_item_bytes(void);
+int tqsv_batch_forward(tqsv_stream_t s, void* states, const void* items_dev, int n_items, const int* progs_dev,
+                       const void* mats, int max_m, int grid);
+/* per-circuit Z-string expectation (ZCirc array, tqsv_batch_zcirc_bytes()). */
+int tqsv_batch_zcirc_bytes(void);
+int tqsv_batch_zexpval(tqsv_stream_t s, const void* states, const void* circ_dev, int n_circ,
+                       const int64_t* chunk_circ_dev, int64_t n_chunks, float* part, float* out);
+
+/* ---- learned plan cost model (no reference counterpart: it serves the
+ *      fusion planner).  Row-major fp32.
+ *      C = alpha op(A) op(B) + beta C (+ bias[N]) (ReLU if relu). ---------- */
+int tqsv_gemm(tqsv_stream_t s, int M, int N, int K, float alpha, const float* A, int lda, int ta, const float* B,
+              int ldb, int tb, float beta, float* C, int ldc, const float* bias, int relu);
+int tqsv_relu_backward(tqsv_stream_t s, float* dY, const float* Y, int64_t n);
+int tqsv_colsum(tqsv_stream_t s, const float* X, int M, int N, float* out);
+int tqsv_mse_grad(tqsv_stream_t s, const float* pred, const float* y, int M, float* grad, float* loss);
+int tqsv_adam_step(tqsv_stream_t s, float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                   float b2, float eps, float c1, float c2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,7 +78,7 @@ def stage_apply(words, st, mats, n, init_zero=False, adjoint=False, lam=None, gr
                     assert int(h[E.H_FLAGS]) & E.FEAT_MATK, "MATK op in a stage without the MATK feature"
                 tok = (tid & w[4]) == w[5]
                 rm, rv = w[6] & 255, (w[6] >> 8) & 255
-                P = (w[0] >> 4) & 15 if w[7] >= 0 else 0
+                P = ((8 if (w[0] & E.F_MG) else (w[0] >> 4) & 15)) if w[7] >= 0 else 0
                 if not adjoint:
                     a = _fwd(typ, a, M, w, tok, rm, rv, tid)
                 else:
@@ -177,7 +177,25 @@ def _fwd(typ, a, M, w, tok, rm, rv, tid):
 
 def _bwd(typ, p, l, M, w, tok, rm, rv, tid, P):
     p, l = p.copy(), l.copy()
-    g = np.zeros(4)
+    g = np.zeros(8)
+    if typ == E.OP_MAT1 and (w[0] & E.F_MG):
+        K = w[3]
+        U = np.array(M[:4]).reshape(2, 2)
+        Ud = U.conj().T
+        mg = np.zeros((2, 2), dtype=complex)
+        for j, j1 in _pairs(K):
+            ok = tok & ((j & rm) == rv)
+            x = np.stack([p[:, j], p[:, j1]], -1)
+            lm = np.stack([l[:, j], l[:, j1]], -1)
+            y = x @ Ud.T
+            mg += np.einsum("ta,tb->ab", lm[ok], np.conj(y[ok]))
+            ln = lm @ Ud.T
+            p[:, j] = np.where(ok, y[:, 0], x[:, 0])
+            p[:, j1] = np.where(ok, y[:, 1], x[:, 1])
+            l[:, j] = np.where(ok, ln[:, 0], lm[:, 0])
+            l[:, j1] = np.where(ok, ln[:, 1], lm[:, 1])
+        g[:8] = np.stack([mg.reshape(-1).real, mg.reshape(-1).imag], -1).reshape(-1)
+        return p, l, g
     if typ == E.OP_MAT1:
         K = w[3]
         U = np.array(M[:4]).reshape(2, 2)
@@ -312,7 +330,7 @@ def run_compiled(c: E.Compiled, pool: np.ndarray, state_in, init_zero=False):
 def adjoint_compiled(c: E.Compiled, pool: np.ndarray, psi, lam):
     """Reverse sweep; mirrors engine.adjoint.  Returns grads [bsz, n_slots]."""
     n, bsz = c.n, c.bsz
-    grad = np.zeros((bsz, max(c.n_slots, 1)))
+    grad = np.zeros((bsz, c.grad_width))
     for kind, payload in reversed(c.steps):
         if kind == "stage":
             base, stg = payload
@@ -331,7 +349,15 @@ def adjoint_compiled(c: E.Compiled, pool: np.ndarray, psi, lam):
                 off, bs = c.adj_off[payload]
                 solo_apply(k, psi, pool[off:], n, bs)
                 solo_apply(k, lam, pool[off:], n, bs)
-    return grad
+    # Mg -> parameter gradients (semantics of tqsv_mg_to_grad)
+    for it in c.mg_items:
+        for row in range(bsz):
+            mg = grad[row, it["mg_slot"]:it["mg_slot"] + 8]
+            Mg = mg[0::2] + 1j * mg[1::2]
+            D = pool[it["d_off"] + row * it["bstride"]:]
+            for q in range(it["P"]):
+                grad[row, it["slot"][q]] += np.real(np.sum(D[4 * q:4 * q + 4] * np.conj(Mg)))
+    return grad[:, :max(c.n_slots, 1)]
 
 
 def _solo_adjoint_grad(k, psi, lam, M, bs, n, grad):

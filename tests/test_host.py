@@ -358,3 +358,47 @@ def test_batch_work_lists_cover_every_group():
                 w = h[int(h[E.H_OFF_OPS]) + 8 * q:][:8]
                 if (w[0] & 15) != E.OP_REMAP:
                     assert 0 <= w[1] < cb.mats.numel()
+
+
+def test_mg_mode_gradients_emulated():
+    """Chains with >= 2 trainable parameters use the matrix-gradient path
+    (F_MG): emulated gradients equal central FD of the oracle."""
+    rng = np.random.default_rng(23)
+    n, bsz = 6, 2
+    ops = []
+    for _ in range(3):
+        for nm in ("rx", "ry", "rz"):
+            for w in range(n):
+                rows = 2 if rng.random() < 0.3 else 1
+                ops.append(dict(name=nm, wires=[w], params=rng.normal(size=(rows, 1)), inverse=False))
+        for w in range(n):
+            ops.append(dict(name="cnot", wires=[w, (w + 1) % n], params=None, inverse=False))
+    ops.append(dict(name="cu3", wires=[0, 3], params=rng.normal(size=(1, 3)), inverse=True))
+    tape = _tape_from_ops(ops, grad=True)
+    c = E.Compiled(tape, n, bsz, E.EngineOptions(stage_bits=5), "cpu")
+    assert len(c.mg_items) > 0
+    pool = c.pool_host(tape, True, True).astype(np.complex128)
+    cvec = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+    psi = EM.run_compiled(c, pool, None, init_zero=True)
+    grad = EM.adjoint_compiled(c, pool, psi, cvec.copy())
+
+    def L(flat):
+        k = 0
+        o2 = []
+        for op in ops:
+            p = op["params"]
+            if p is not None:
+                p = flat[k:k + p.size].reshape(p.shape)
+                k += p.size
+            o2.append(dict(op, params=p))
+        return float(np.real(np.sum(np.conj(cvec) * O.run_ops(oracle_ops(o2, bsz), n, bsz))))
+
+    flat = np.concatenate([op["params"].reshape(-1) for op in ops if op["params"] is not None])
+    fd = O.central_fd(L, flat.copy(), h=1e-5)
+    got = []
+    for ti, op in enumerate(tape):
+        if op.params is None:
+            continue
+        g = grad[:, c.param_offsets[ti]:c.param_offsets[ti] + op.n_params()]
+        got.append(g.sum(0) if op.params.shape[0] == 1 else g.reshape(-1))
+    assert np.all(np.abs(np.concatenate(got) - fd) < 1e-4 * (1 + np.abs(fd)))

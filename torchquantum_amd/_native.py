@@ -50,6 +50,8 @@ SIGNATURES = {
     "tqsv_batch_item_bytes": (c_int, []),
     "tqsv_batch_zcirc_bytes": (c_int, []),
     "tqsv_batch_zexpval": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int64, c_float_p, c_float_p]),
+    "tqsv_mg_item_bytes": (c_int, []),
+    "tqsv_mg_to_grad": (c_int, [c_void_p, c_float_p, c_int64, c_int64, c_void_p, c_int, c_void_p]),
     "tqsv_gemm": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
                           c_float, c_void_p, c_int, c_void_p, c_int]),
     "tqsv_relu_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),

@@ -41,7 +41,8 @@ MAX_P_PER_OP = 4
 # stage header layout (must match tqsv_stage.hip)
 H_NOPS, H_M, H_T, H_NNL, H_OFF_OPS, H_NGP, H_OFF_GSLOT, H_FLAGS, H_POS = range(9)
 OP_MAT1, OP_MAT2, OP_DIAG, OP_MATK, OP_REMAP = 1, 2, 3, 4, 5
-F_BATCHED, F_NOCTRL, F_REAL, F_XPERM, F_SIGN = 1 << 8, 1 << 9, 1 << 10, 1 << 11, 1 << 12
+F_BATCHED, F_NOCTRL, F_REAL, F_XPERM, F_SIGN, F_MG = 1 << 8, 1 << 9, 1 << 10, 1 << 11, 1 << 12, 1 << 13
+MG_ITEM = np.dtype([("mg_slot", "<i4"), ("P", "<i4"), ("slot", "<i4", (4,)), ("d_off", "<i8"), ("bstride", "<i8")])
 FEAT_MAT2, FEAT_MATK = 1, 2
 
 
@@ -622,7 +623,8 @@ def reg_targets(k: KOp) -> Tuple[int, ...]:
 
 
 def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off: Dict[int, tuple],
-                 slot_of_kop: Dict[int, List[int]], stage_base: int) -> Stage:
+                 slot_of_kop: Dict[int, List[int]], stage_base: int,
+                 mg_slot_of: Optional[Dict[int, List[int]]] = None) -> Stage:
     """Program words of one stage (see tqsv_stage.hip for the format)."""
     m = len(L)
     T = m - R_BITS
@@ -737,11 +739,14 @@ def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off
             rm |= 0
             w[6] = (rpm & 255) << 16
         w[0] |= (P << 4) | (F_BATCHED if k.batched else 0)
+        mg = k.kind == "mat1" and P >= 2 and not k.xperm
         if k.kind == "mat1":
             if k.xperm and P == 0:
                 w[0] |= F_XPERM
             elif tm == 0 and rm == 0:
                 w[0] |= F_NOCTRL | (F_REAL if k.real else 0)
+            if mg:
+                w[0] |= F_MG
         elif k.kind == "diag" and k.sign and not k.bits and P == 0:
             w[0] |= F_SIGN
         if k.kind in ("mat2", "swap"):
@@ -754,7 +759,7 @@ def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off
         w[6] |= (rm & 255) | ((rv & 255) << 8)
         if P:
             w[7] = len(gslots)
-            gslots.extend(slot_of_kop[i])
+            gslots.extend(mg_slot_of[i] if mg else slot_of_kop[i])
         else:
             w[7] = -1
         ops_words.append(w)
@@ -822,9 +827,24 @@ class Compiled:
         self.steps: List[tuple] = []
         self.perms: Dict[int, np.ndarray] = {}
         self._fill = None
+        self.mg_items = np.zeros(0, dtype=MG_ITEM)
+        self.grad_width = max(self.n_slots, 1)
         if opts.mode == "fused" and n >= R_BITS:
             m = choose_stage_bits(n, opts)
             slot_of = {i: list(k.grad_slots) for i, k in enumerate(kops)}
+            # ops with >= 2 trainable parameters accumulate a 2x2 matrix gradient
+            # (8 floats past the parameter columns) that one small kernel folds
+            # into their parameter slots after the sweep
+            mg_slot_of = {}
+            items = []
+            for i, k in enumerate(kops):
+                if k.kind == "mat1" and k.P >= 2 and not k.xperm:
+                    base = self.n_slots + 8 * len(items)
+                    mg_slot_of[i] = list(range(base, base + 8))
+                    sl = list(k.grad_slots) + [0] * (4 - k.P)
+                    items.append((base, k.P, sl, self.data_off[i][0] + 4, self.data_off[i][1]))
+            self.mg_items = np.array(items, dtype=MG_ITEM) if items else np.zeros(0, dtype=MG_ITEM)
+            self.grad_width = max(self.n_slots + 8 * len(items), 1)
             words = []
             base = 0
             for kind, payload in partition(kops, n, m, opts.low_bits):
@@ -832,7 +852,7 @@ class Compiled:
                     self.steps.append(("solo", payload))
                     continue
                 inc, L = payload
-                st = encode_stage(inc, L, kops, n, self.data_off, slot_of, base)
+                st = encode_stage(inc, L, kops, n, self.data_off, slot_of, base, mg_slot_of)
                 for (ki, perm) in st.enc:
                     if ki >= 0 and perm is not None:
                         self.perms[ki] = perm
@@ -1083,7 +1103,7 @@ def adjoint(c: Compiled, psi: torch.Tensor, lam: torch.Tensor, pool: torch.Tenso
     """Reverse sweep in place on psi (forward output) and lam (dL/dpsi).
     Returns grads [bsz, n_slots] float32."""
     n, bsz = c.n, c.bsz
-    grad = torch.zeros((bsz, max(c.n_slots, 1)), dtype=torch.float32, device=c.device)
+    grad = torch.zeros((bsz, c.grad_width), dtype=torch.float32, device=c.device)
     stream = N.stream_of(psi)
     gstride = grad.shape[1]
     for kind, payload in reversed(c.steps):
@@ -1123,4 +1143,9 @@ def adjoint(c: Compiled, psi: torch.Tensor, lam: torch.Tensor, pool: torch.Tenso
                 off, bs = c.adj_off[i]
             run_solo_forward(psi, k, c, pool, off, bs, stream)
             run_solo_forward(lam, k, c, pool, off, bs, stream)
-    return grad
+    if len(c.mg_items):
+        if getattr(c, "_mg_dev", None) is None or c._mg_dev.device != psi.device:
+            c._mg_dev = torch.from_numpy(c.mg_items.view(np.uint8).copy()).to(psi.device)
+        N.call("tqsv_mg_to_grad", stream, N.fptr(grad), gstride, bsz, c._mg_dev.data_ptr(), len(c.mg_items),
+               pool.data_ptr())
+    return grad[:, :max(c.n_slots, 1)]
