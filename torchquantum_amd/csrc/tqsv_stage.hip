@@ -543,11 +543,15 @@ __device__ __forceinline__ uint32_t group_offset(const int* __restrict__ nlpos, 
   return o;
 }
 
+// The program and matrix pool are separate __restrict__ const parameters so
+// the compiler may fetch their (wave-uniform) words with scalar loads: the
+// kernel's stores provably cannot clobber them.
 template <int FEAT>
 __global__ __launch_bounds__(512) void k_stage_fwd(float2* __restrict__ out, const float2* __restrict__ in,
+                                                   const int* __restrict__ prog, const float2* __restrict__ mats,
                                                    StageArgs A) {
   extern __shared__ float2 lds[];
-  const int* __restrict__ h = A.prog;
+  const int* __restrict__ h = prog;
   const int n_ops = h[H_NOPS], T = h[H_T], nnl = h[H_NNL];
   const int* __restrict__ tpos = h + H_POS;
   const int* __restrict__ rpos = tpos + T;
@@ -588,7 +592,7 @@ __global__ __launch_bounds__(512) void k_stage_fwd(float2* __restrict__ out, con
         remap(a, lds, rec, rec + T + R, T, tid);
         continue;
       }
-      const float2* M = A.mats + w[1] + ((w[0] & F_BATCHED) ? (int64_t)row * w[2] : 0);
+      const float2* M = mats + w[1] + ((w[0] & F_BATCHED) ? (int64_t)row * w[2] : 0);
       fwd_op<FEAT>(a, w, M, tid);
     }
     float2* dst = out + base;
@@ -600,10 +604,11 @@ __global__ __launch_bounds__(512) void k_stage_fwd(float2* __restrict__ out, con
 
 template <int FEAT>
 __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, float2* __restrict__ lam,
-                                                   StageArgs A) {
+                                                   const int* __restrict__ prog, const float2* __restrict__ mats,
+                                                   float* __restrict__ grad, StageArgs A) {
   extern __shared__ float2 lds[];
   __shared__ float gacc[MAX_GP];
-  const int* __restrict__ h = A.prog;
+  const int* __restrict__ h = prog;
   const int n_ops = h[H_NOPS], T = h[H_T], nnl = h[H_NNL], n_gp = h[H_NGP];
   const int* __restrict__ tpos = h + H_POS;
   const int* __restrict__ rpos = tpos + T;
@@ -637,7 +642,7 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
         __syncthreads();
         for (int t = tid; t < n_gp; t += blockDim.x) {
           const float v = gacc[t];
-          if (v != 0.f) atomicAdd(A.grad + cur_row * A.grad_stride + gslot[t], v);
+          if (v != 0.f) atomicAdd(grad + cur_row * A.grad_stride + gslot[t], v);
           gacc[t] = 0.f;
         }
         __syncthreads();
@@ -664,16 +669,36 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
         remap(l, lds, rec + T + R, rec, T, tid);
         continue;
       }
-      const float2* M = A.mats + w[1] + ((w[0] & F_BATCHED) ? (int64_t)row * w[2] : 0);
+      const float2* M = mats + w[1] + ((w[0] & F_BATCHED) ? (int64_t)row * w[2] : 0);
       float gq[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       bwd_op<FEAT>(p, l, w, M, tid, gq);
       if (w[7] >= 0) {
         const int P = (w[0] & F_MG) ? 8 : ((w[0] >> 4) & 15);
+        if (wwidth == 64) {
+          // halving exchange: every lane ends with one value's wave total
+          if (P > 4) {
+            const float c = wave_reduce_t<8>(gq, lane);
+            if ((lane & 7) == 0) atomicAdd(&gacc[w[7] + (lane >> 3)], c);
+          } else if (P > 2) {
+            const float v4[4] = {gq[0], gq[1], gq[2], gq[3]};
+            const float c = wave_reduce_t<4>(v4, lane);
+            if ((lane & 15) == 0 && (lane >> 4) < P) atomicAdd(&gacc[w[7] + (lane >> 4)], c);
+          } else if (P == 2) {
+            const float v2[2] = {gq[0], gq[1]};
+            const float c = wave_reduce_t<2>(v2, lane);
+            if ((lane & 31) == 0) atomicAdd(&gacc[w[7] + (lane >> 5)], c);
+          } else {
+            const float v1[1] = {gq[0]};
+            const float c = wave_reduce_t<1>(v1, lane);
+            if (lane == 0) atomicAdd(&gacc[w[7]], c);
+          }
+        } else {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          if (q < P) {
-            const float s = wave_sum(gq[q], wwidth);
-            if (lane == 0) atomicAdd(&gacc[w[7] + q], s);
+          for (int q = 0; q < 8; ++q) {
+            if (q < P) {
+              const float s = wave_sum(gq[q], wwidth);
+              if (lane == 0) atomicAdd(&gacc[w[7] + q], s);
+            }
           }
         }
       }
@@ -693,7 +718,7 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
     __syncthreads();
     for (int t = tid; t < n_gp; t += blockDim.x) {
       const float v = gacc[t];
-      if (v != 0.f) atomicAdd(A.grad + cur_row * A.grad_stride + gslot[t], v);
+      if (v != 0.f) atomicAdd(grad + cur_row * A.grad_stride + gslot[t], v);
     }
   }
 }
@@ -906,11 +931,14 @@ TQSV_API int tqsv_stage_forward(hipStream_t s, void* state_out, const void* stat
   StageArgs A{prog_dev, (const float2*)mats, n, init_zero, n_groups, gpw, nullptr, 0};
   const int feat = prog_host[H_FLAGS] & 3;
   if (feat == 0)
-    k_stage_fwd<0><<<grid, 1 << T, shm, s>>>((float2*)state_out, (const float2*)state_in, A);
+    k_stage_fwd<0><<<grid, 1 << T, shm, s>>>((float2*)state_out, (const float2*)state_in, prog_dev,
+                                                      (const float2*)mats, A);
   else if (feat == FEAT_MAT2)
-    k_stage_fwd<FEAT_MAT2><<<grid, 1 << T, shm, s>>>((float2*)state_out, (const float2*)state_in, A);
+    k_stage_fwd<FEAT_MAT2><<<grid, 1 << T, shm, s>>>((float2*)state_out, (const float2*)state_in, prog_dev,
+                                                      (const float2*)mats, A);
   else
-    k_stage_fwd<FEAT_MAT2 | FEAT_MATK><<<grid, 1 << T, shm, s>>>((float2*)state_out, (const float2*)state_in, A);
+    k_stage_fwd<FEAT_MAT2 | FEAT_MATK><<<grid, 1 << T, shm, s>>>((float2*)state_out, (const float2*)state_in, prog_dev,
+                                                      (const float2*)mats, A);
   return check_launch("tqsv_stage_forward");
 }
 
@@ -933,11 +961,14 @@ TQSV_API int tqsv_stage_adjoint(hipStream_t s, void* psi, void* lam, int64_t bsz
   StageArgs A{prog_dev, (const float2*)mats, n, 0, n_groups, gpw, grad, grad_stride};
   const int feat = prog_host[H_FLAGS] & 3;
   if (feat == 0)
-    k_stage_adj<0><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, A);
+    k_stage_adj<0><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, prog_dev, (const float2*)mats,
+                                                      grad, A);
   else if (feat == FEAT_MAT2)
-    k_stage_adj<FEAT_MAT2><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, A);
+    k_stage_adj<FEAT_MAT2><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, prog_dev, (const float2*)mats,
+                                                      grad, A);
   else
-    k_stage_adj<FEAT_MAT2 | FEAT_MATK><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, A);
+    k_stage_adj<FEAT_MAT2 | FEAT_MATK><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, prog_dev, (const float2*)mats,
+                                                      grad, A);
   return check_launch("tqsv_stage_adjoint");
 }
 

@@ -60,6 +60,8 @@ class EngineOptions:
     fusion_width: k; >= 2 pre-multiplies runs of gates acting within k wires
                  into one dense 2^k x 2^k block (k <= 4; trainable blocks k <= 2).
     grid_cap:    workgroups per stage launch (0 = auto).
+    adjoint_mg:  adjoint sweep accumulates 2x2 matrix gradients for ops with
+                 >= 2 trainable parameters (else one inner product per parameter).
     """
     mode: str = "fused"
     stage_bits: int = 0
@@ -67,9 +69,11 @@ class EngineOptions:
     merge_1q: bool = True
     fusion_width: int = 1
     grid_cap: int = 0
+    adjoint_mg: bool = True
 
     def key(self):
-        return (self.mode, self.stage_bits, self.low_bits, self.merge_1q, self.fusion_width, self.grid_cap)
+        return (self.mode, self.stage_bits, self.low_bits, self.merge_1q, self.fusion_width, self.grid_cap,
+                self.adjoint_mg)
 
 
 DEFAULT_OPTIONS = EngineOptions()
@@ -739,7 +743,7 @@ def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off
             rm |= 0
             w[6] = (rpm & 255) << 16
         w[0] |= (P << 4) | (F_BATCHED if k.batched else 0)
-        mg = k.kind == "mat1" and P >= 2 and not k.xperm
+        mg = mg_slot_of is not None and i in mg_slot_of
         if k.kind == "mat1":
             if k.xperm and P == 0:
                 w[0] |= F_XPERM
@@ -838,7 +842,7 @@ class Compiled:
             mg_slot_of = {}
             items = []
             for i, k in enumerate(kops):
-                if k.kind == "mat1" and k.P >= 2 and not k.xperm:
+                if opts.adjoint_mg and k.kind == "mat1" and k.P >= 2 and not k.xperm:
                     base = self.n_slots + 8 * len(items)
                     mg_slot_of[i] = list(range(base, base + 8))
                     sl = list(k.grad_slots) + [0] * (4 - k.P)
