@@ -118,28 +118,72 @@ __global__ __launch_bounds__(256) void k_1q_t0(float4* __restrict__ st, uint64_t
   }
 }
 
-// target bit > 0 and bit 0 free: two consecutive items share 16-byte accesses
-template <bool BATCHED>
+// target bit > 0 and bit 0 free: two consecutive items share 16-byte accesses.
+// Each thread keeps two such item pairs (grid-stride unroll) in flight; NT
+// marks the accesses non-temporal for states larger than the Infinity Cache.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <bool NT>
+__device__ __forceinline__ float4 ld4(const float4* p) {
+  if constexpr (NT) {
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+  } else {
+    return *p;
+  }
+}
+template <bool NT>
+__device__ __forceinline__ void st4(float4* p, float4 v) {
+  if constexpr (NT) {
+    v4f t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, reinterpret_cast<v4f*>(p));
+  } else {
+    *p = v;
+  }
+}
+
+__device__ __forceinline__ void rot2(const float2 (&m)[4], const float4 u, const float4 w, float4& uo, float4& wo) {
+  const float2 a0 = make_float2(u.x, u.y), a0n = make_float2(u.z, u.w);
+  const float2 a1 = make_float2(w.x, w.y), a1n = make_float2(w.z, w.w);
+  const float2 b0 = cfma(m[1], a1, cmul(m[0], a0));
+  const float2 b1 = cfma(m[3], a1, cmul(m[2], a0));
+  const float2 b0n = cfma(m[1], a1n, cmul(m[0], a0n));
+  const float2 b1n = cfma(m[3], a1n, cmul(m[2], a0n));
+  uo = make_float4(b0.x, b0.y, b0n.x, b0n.y);
+  wo = make_float4(b1.x, b1.y, b1n.x, b1n.y);
+}
+
+template <bool BATCHED, bool NT>
 __global__ __launch_bounds__(256) void k_1q_v2(float4* __restrict__ st, uint64_t n_pairs, InsertSpec ins,
                                               uint64_t cval, uint64_t tmask, const float2* __restrict__ mat,
                                               int64_t bstride, int lg_items_row) {
-  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < n_pairs;
-       q += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t it = 2 * q;
-    const uint64_t i0 = insert_zeros(it, ins) | cval;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < n_pairs; q += 2 * stride) {
+    const uint64_t qb = q + stride;
+    const bool two = qb < n_pairs;
+    const uint64_t i0 = insert_zeros(2 * q, ins) | cval;
     const uint64_t i1 = i0 | tmask;
+    const uint64_t j0 = two ? (insert_zeros(2 * qb, ins) | cval) : i0;
+    const uint64_t j1 = j0 | tmask;
+    const float4 u = ld4<NT>(st + (i0 >> 1));
+    const float4 w = ld4<NT>(st + (i1 >> 1));
+    float4 u2 = u, w2 = w;
+    if (two) {
+      u2 = ld4<NT>(st + (j0 >> 1));
+      w2 = ld4<NT>(st + (j1 >> 1));
+    }
     float2 m[4];
-    load_m2<BATCHED>(mat, bstride, it >> lg_items_row, m);
-    const float4 u = st[i0 >> 1];
-    const float4 w = st[i1 >> 1];
-    const float2 a0 = make_float2(u.x, u.y), a0n = make_float2(u.z, u.w);
-    const float2 a1 = make_float2(w.x, w.y), a1n = make_float2(w.z, w.w);
-    const float2 b0 = cfma(m[1], a1, cmul(m[0], a0));
-    const float2 b1 = cfma(m[3], a1, cmul(m[2], a0));
-    const float2 b0n = cfma(m[1], a1n, cmul(m[0], a0n));
-    const float2 b1n = cfma(m[3], a1n, cmul(m[2], a0n));
-    st[i0 >> 1] = make_float4(b0.x, b0.y, b0n.x, b0n.y);
-    st[i1 >> 1] = make_float4(b1.x, b1.y, b1n.x, b1n.y);
+    load_m2<BATCHED>(mat, bstride, (2 * q) >> lg_items_row, m);
+    float4 uo, wo;
+    rot2(m, u, w, uo, wo);
+    st4<NT>(st + (i0 >> 1), uo);
+    st4<NT>(st + (i1 >> 1), wo);
+    if (two) {
+      if (BATCHED) load_m2<BATCHED>(mat, bstride, (2 * qb) >> lg_items_row, m);
+      rot2(m, u2, w2, uo, wo);
+      st4<NT>(st + (j0 >> 1), uo);
+      st4<NT>(st + (j1 >> 1), wo);
+    }
   }
 }
 
@@ -371,8 +415,16 @@ TQSV_API int tqsv_apply_1q(hipStream_t s, void* state, int64_t bsz, int n, int t
     else k_1q_t0<false><<<grid_for(n_items, 256), 256, 0, s>>>((float4*)state, n_items, ins, cval, m, 0, lg_items_row);
   } else if (!(cmask & 1ull)) {
     const uint64_t n_pairs = n_items / 2;
-    if (b) k_1q_v2<true><<<grid_for(n_pairs, 256), 256, 0, s>>>((float4*)state, n_pairs, ins, cval, tmask, m, mat_bstride, lg_items_row);
-    else k_1q_v2<false><<<grid_for(n_pairs, 256), 256, 0, s>>>((float4*)state, n_pairs, ins, cval, tmask, m, 0, lg_items_row);
+    // states beyond the 256 MB Infinity Cache stream with non-temporal hints
+    const bool nt = ((uint64_t)bsz << n) * sizeof(float2) > (256ull << 20);
+    const unsigned g = grid_for((n_pairs + 1) / 2, 256);
+    if (b) {
+      if (nt) k_1q_v2<true, true><<<g, 256, 0, s>>>((float4*)state, n_pairs, ins, cval, tmask, m, mat_bstride, lg_items_row);
+      else k_1q_v2<true, false><<<g, 256, 0, s>>>((float4*)state, n_pairs, ins, cval, tmask, m, mat_bstride, lg_items_row);
+    } else {
+      if (nt) k_1q_v2<false, true><<<g, 256, 0, s>>>((float4*)state, n_pairs, ins, cval, tmask, m, 0, lg_items_row);
+      else k_1q_v2<false, false><<<g, 256, 0, s>>>((float4*)state, n_pairs, ins, cval, tmask, m, 0, lg_items_row);
+    }
   } else {
     if (b) k_1q_gen<true><<<grid_for(n_items, 256), 256, 0, s>>>((float2*)state, n_items, ins, cval, tmask, m, mat_bstride, lg_items_row);
     else k_1q_gen<false><<<grid_for(n_items, 256), 256, 0, s>>>((float2*)state, n_items, ins, cval, tmask, m, 0, lg_items_row);
