@@ -118,14 +118,14 @@ def kernel_roofline(n, bsz, iters=50):
     m = torch.tensor([[np.cos(th / 2), -np.sin(th / 2)], [np.sin(th / 2), np.cos(th / 2)]],
                      dtype=torch.complex64, device="cuda")
     stream = N.stream_of(st)
-    bit = n // 2
-    for _ in range(5):
-        N.call("tqsv_apply_1q", stream, st.data_ptr(), bsz, n, bit, 0, 0, m.data_ptr(), 0)
+    for i in range(5):
+        N.call("tqsv_apply_1q", stream, st.data_ptr(), bsz, n, 1 + i % (n - 1), 0, 0, m.data_ptr(), 0)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     e0.record()
+    # target bits 1..n-1 (bit 0 pairs are adjacent and take the k_1q_t0 kernel)
     for i in range(iters):
-        N.call("tqsv_apply_1q", stream, st.data_ptr(), bsz, n, (bit + i) % n, 0, 0, m.data_ptr(), 0)
+        N.call("tqsv_apply_1q", stream, st.data_ptr(), bsz, n, 1 + i % (n - 1), 0, 0, m.data_ptr(), 0)
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / iters
@@ -183,6 +183,81 @@ def time_batch(n_circ, steps, tq):
             "build_s": t_build, "z0_max_abs_diff_vs_per_circuit": float((out - ref).abs().max())}
 
 
+def copy_calibration(nbytes=1 << 31):
+    """A plain device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE."""
+    a = torch.empty(nbytes // 8, dtype=torch.complex64, device="cuda").fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    return nbytes, e0.elapsed_time(e1)
+
+
+def load_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (two rocprofv3 --pmc runs, scaled by the copy calibration)."""
+    path = os.path.join(ROOT, "profiles", "k_1q_v2_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def config5(args, tq):
+    """30 qubits, bsz 4 (32 GiB): 20 x [RY all, CZ even/odd] + a Toffoli or CSWAP
+    on seeded wires after every 5th layer; loss = sum_i <Z_i Z_i+1> (open chain)."""
+    import torchquantum_amd.functional as tqf
+    from circuits import config5_ops, chain_zz
+    n, bsz = args.c5_qubits, args.c5_bsz
+    ops = config5_ops(seed=0, n=n)
+    obs = chain_zz(n)
+    params = [torch.tensor(op["params"], dtype=torch.float32, device="cuda", requires_grad=True)
+              if op["params"] is not None else None for op in ops]
+    upd = len(ops) * bsz * float(1 << n)
+
+    def build(mode):
+        q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
+        q.engine_options = tq.EngineOptions(mode=mode)
+        for op, p in zip(ops, params):
+            tqf.func_name_dict[op["name"]](q, op["wires"], params=p)
+        return q
+
+    res = {"workload": f"{n} qubits, bsz {bsz}, 20 x [RY all, CZ even+odd] + Toffoli/CSWAP every 5 layers",
+           "gates": len(ops)}
+    with torch.no_grad():
+        for mode in ("fused", "pergate"):
+            q = build(mode)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ev = tq.expval_joint_analytical_batch(q, obs[:4])
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            res[f"{mode}_forward_s"] = dt
+            res[f"{mode}_amp_updates_per_s"] = upd / dt
+            res[f"{mode}_zz_first4"] = ev[0].tolist()
+            if mode == "fused":
+                sv = q.get_states_1d()
+                res["norm_max_dev"] = float((torch.linalg.vector_norm(sv, dim=1) - 1).abs().max())
+                sample = sv[:, :: (1 << 12)].clone()
+            else:
+                res["fused_vs_pergate_amp_max_abs"] = float((q.get_states_1d()[:, :: (1 << 12)] - sample).abs().max())
+            del q
+            torch.cuda.empty_cache()
+    q = build("fused")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = tq.expval_pauli_sum(q, obs, [1.0] * len(obs)).mean()
+    loss.backward()
+    torch.cuda.synchronize()
+    res["fused_gradient_s"] = time.perf_counter() - t0
+    res["n_params"] = sum(p.numel() for p in params if p is not None)
+    res["loss"] = float(loss)
+    return res
+
+
 def cpu_baseline(ops_sample, n, bsz, n_gates_total_per_step):
     from oracle import refpath
     threads = os.cpu_count() or 1
@@ -205,6 +280,11 @@ def main():
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--only-roofline", action="store_true",
+                    help="run only the dominant kernel's timing loops (for rocprofv3 summaries)")
+    ap.add_argument("--config5", action="store_true", help="run the 30-qubit workload only")
+    ap.add_argument("--c5-qubits", type=int, default=30)
+    ap.add_argument("--c5-bsz", type=int, default=4)
     ap.add_argument("--cpu-layers", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -216,6 +296,18 @@ def main():
         sys.exit(2)
     import torchquantum_amd as tq
     from circuits import config2_ops
+
+    if args.only_roofline:
+        cb, cms = copy_calibration()
+        ms, nbytes = kernel_roofline(20, 16)
+        ms_big, nbytes_big = kernel_roofline(26, 16, iters=20)
+        print(json.dumps({"copy_bytes": cb, "copy_ms": cms, "k_1q_v2_20q_bsz16_ms": ms,
+                          "k_1q_v2_20q_bsz16_GBs": nbytes / ms / 1e6, "k_1q_v2_26q_bsz16_ms": ms_big,
+                          "k_1q_v2_26q_bsz16_GBs": nbytes_big / ms_big / 1e6}), flush=True)
+        return
+    if args.config5:
+        print(json.dumps({"config5": config5(args, tq)}), flush=True)
+        return
 
     n, bsz = 20, 16
     ops = config2_ops(seed=0, n=n, layers=20)
@@ -279,7 +371,8 @@ def main():
         "fused": {"amp_updates_per_s": upd_per_step / dt_f, "ms_per_step": dt_f * 1e3,
                   "speedup_vs_pergate": dt_pg / dt_f, "expval_max_abs_diff_vs_pergate": parity},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (load_traffic() or {}).get("k_1q_v2_20q_bsz16_bytes_per_launch"),
                      "kernel": "k_1q_v2 (RY, 20 qubits bsz 16: 256 MiB/launch)", "avg_launch_ms": ms,
                      "at_26q_bsz16": {"achieved": achieved_big, "frac": achieved_big / HBM_PEAK_GBS,
                                       "avg_launch_ms": ms_big}},

@@ -121,3 +121,33 @@ def config3_observables(n):
         obs.append("".join(s))
         coef.append(0.5)
     return obs, coef
+
+
+def config5_ops(seed=0, n=30, layers=20):
+    """20 x [RY(U[0, 2pi)) on every wire, CZ even then odd pairs]; after every
+    5th layer one Toffoli or CSWAP on three seeded distinct wires."""
+    g = torch.Generator().manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    ops = []
+    for layer in range(layers):
+        for w in range(n):
+            ops.append(dict(name="ry", wires=[w], params=(torch.rand(1, 1, generator=g) * 2 * np.pi).numpy()
+                            .astype(np.float64), inverse=False))
+        for start in (0, 1):
+            for i in range(start, n - 1, 2):
+                ops.append(dict(name="cz", wires=[i, i + 1], params=None, inverse=False))
+        if (layer + 1) % 5 == 0:
+            w = [int(x) for x in rng.choice(n, 3, replace=False)]
+            ops.append(dict(name="toffoli" if rng.random() < 0.5 else "cswap", wires=w, params=None,
+                            inverse=False))
+    return ops
+
+
+def chain_zz(n):
+    """sum_i Z_i Z_{i+1}, i = 0..n-2 (open chain)."""
+    out = []
+    for i in range(n - 1):
+        s = ["I"] * n
+        s[i] = s[i + 1] = "Z"
+        out.append("".join(s))
+    return out

@@ -188,3 +188,42 @@ def test_fused_matches_pergate_large():
     a = _run(n, bsz, ops, "fused")
     b = _run(n, bsz, ops, "pergate")
     assert np.abs(a - b).max() < 1e-5
+
+
+@gpu
+def test_config5_family_parity():
+    """The 30-qubit workload's circuit family (RY + CZ brickwork, Toffoli/CSWAP
+    every 5 layers) at 14 qubits, bsz 2: amplitudes and chain <ZZ> vs the oracle
+    (1e-5) for fused and per-gate plans; 12 parameter gradients vs FD."""
+    import torchquantum_amd as tq
+    from circuits import config5_ops, chain_zz
+    n, bsz = 14, 2
+    ops = config5_ops(seed=3, n=n)
+    obs = chain_zz(n)
+    st = O.run_ops(oracle_ops(ops, 1), n, 1)
+    ref_zz = np.array([O.expval_pauli(st, o)[0] for o in obs])
+    for mode in ("fused", "pergate"):
+        q = _qdev(n, bsz, mode)
+        ts = apply_ops_tq(q, ops, DEV, requires_grad=True)
+        assert np.abs(q.get_states_1d().detach().cpu().numpy() - st).max() < 1e-5
+        loss = tq.expval_pauli_sum(q, obs, [1.0] * len(obs))
+        assert np.abs(loss.detach().cpu().numpy() - ref_zz.sum()).max() < 1e-4
+        loss.sum().backward()
+        got = np.concatenate([t.grad.detach().cpu().numpy().reshape(-1) for t in ts]) / bsz
+        flat = np.concatenate([op["params"].reshape(-1) for op in ops if op["params"] is not None])
+
+        def f(v):
+            k = 0
+            o2 = []
+            for op in ops:
+                p = op["params"]
+                if p is not None:
+                    p = v[k:k + p.size].reshape(p.shape)
+                    k += p.size
+                o2.append(dict(op, params=p))
+            s = O.run_ops(oracle_ops(o2, 1), n, 1)
+            return float(sum(O.expval_pauli(s, o)[0] for o in obs))
+
+        idx = np.random.default_rng(0).choice(flat.size, 12, replace=False)
+        fd = O.central_fd(f, flat.copy(), h=1e-4, idx=idx)[idx]
+        assert np.all(np.abs(got[idx] - fd) <= 2e-4 + 1e-3 * np.abs(fd)), (mode, np.abs(got[idx] - fd).max())
