@@ -229,6 +229,7 @@ def config5(args, tq):
            "gates": len(ops)}
     with torch.no_grad():
         for mode in ("fused", "pergate"):
+            log(f"[config5] {mode} forward")
             q = build(mode)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -246,6 +247,7 @@ def config5(args, tq):
                 res["fused_vs_pergate_amp_max_abs"] = float((q.get_states_1d()[:, :: (1 << 12)] - sample).abs().max())
             del q
             torch.cuda.empty_cache()
+    log("[config5] fused gradient")
     q = build("fused")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
