@@ -109,8 +109,10 @@ def time_gradient(n, bsz, layers, mode, steps, warmup, tq):
 
 
 def kernel_roofline(n, bsz, iters=50):
-    """Average duration of the 1-qubit per-gate kernel (RY on a mid wire) with
-    HIP events on the launch stream; algorithmic bytes = 16 B x bsz x 2^n."""
+    """Average duration of the dense 1-qubit gate (RY, target cycling over every
+    bit 0..n-1, as RY on all wires does: k_1q_t0 for bit 0, k_1q_tile for bits
+    1..8, k_1q_pair above) with HIP events on the launch stream; algorithmic
+    bytes = 16 B x bsz x 2^n per launch."""
     from torchquantum_amd import _native as N
     st = torch.zeros((bsz, 1 << n), dtype=torch.complex64, device="cuda")
     st[:, 0] = 1
@@ -119,13 +121,12 @@ def kernel_roofline(n, bsz, iters=50):
                      dtype=torch.complex64, device="cuda")
     stream = N.stream_of(st)
     for i in range(5):
-        N.call("tqsv_apply_1q", stream, st.data_ptr(), bsz, n, 1 + i % (n - 1), 0, 0, m.data_ptr(), 0)
+        N.call("tqsv_apply_1q", stream, st.data_ptr(), bsz, n, i % n, 0, 0, m.data_ptr(), 0)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     e0.record()
-    # target bits 1..n-1 (bit 0 pairs are adjacent and take the k_1q_t0 kernel)
     for i in range(iters):
-        N.call("tqsv_apply_1q", stream, st.data_ptr(), bsz, n, 1 + i % (n - 1), 0, 0, m.data_ptr(), 0)
+        N.call("tqsv_apply_1q", stream, st.data_ptr(), bsz, n, i % n, 0, 0, m.data_ptr(), 0)
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / iters
@@ -199,7 +200,7 @@ def copy_calibration(nbytes=1 << 31):
 def load_traffic():
     """HBM bytes per launch of the dominant kernel from the committed counter
     summary (two rocprofv3 --pmc runs, scaled by the copy calibration)."""
-    path = os.path.join(ROOT, "profiles", "k_1q_v2_traffic.json")
+    path = os.path.join(ROOT, "profiles", "k_1q_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -301,11 +302,11 @@ def main():
 
     if args.only_roofline:
         cb, cms = copy_calibration()
-        ms, nbytes = kernel_roofline(20, 16)
-        ms_big, nbytes_big = kernel_roofline(26, 16, iters=20)
-        print(json.dumps({"copy_bytes": cb, "copy_ms": cms, "k_1q_v2_20q_bsz16_ms": ms,
-                          "k_1q_v2_20q_bsz16_GBs": nbytes / ms / 1e6, "k_1q_v2_26q_bsz16_ms": ms_big,
-                          "k_1q_v2_26q_bsz16_GBs": nbytes_big / ms_big / 1e6}), flush=True)
+        ms, nbytes = kernel_roofline(20, 16, iters=60)
+        ms_big, nbytes_big = kernel_roofline(26, 16, iters=26)
+        print(json.dumps({"copy_bytes": cb, "copy_ms": cms, "k_1q_20q_bsz16_ms": ms,
+                          "k_1q_20q_bsz16_GBs": nbytes / ms / 1e6, "k_1q_26q_bsz16_ms": ms_big,
+                          "k_1q_26q_bsz16_GBs": nbytes_big / ms_big / 1e6}), flush=True)
         return
     if args.config5:
         print(json.dumps({"config5": config5(args, tq)}), flush=True)
@@ -324,9 +325,9 @@ def main():
     parity = float((ev_pg - ev_f).abs().max())
 
     log("[bench] kernel roofline")
-    ms, nbytes = kernel_roofline(n, bsz)
+    ms, nbytes = kernel_roofline(n, bsz, iters=60)
     achieved = nbytes / (ms * 1e-3) / 1e9
-    ms_big, nbytes_big = kernel_roofline(26, 16, iters=20)
+    ms_big, nbytes_big = kernel_roofline(26, 16, iters=26)
     achieved_big = nbytes_big / (ms_big * 1e-3) / 1e9
 
     extra = {}
@@ -374,8 +375,9 @@ def main():
                   "speedup_vs_pergate": dt_pg / dt_f, "expval_max_abs_diff_vs_pergate": parity},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (load_traffic() or {}).get("k_1q_v2_20q_bsz16_bytes_per_launch"),
-                     "kernel": "k_1q_v2 (RY, 20 qubits bsz 16: 256 MiB/launch)", "avg_launch_ms": ms,
+                     "traffic": (load_traffic() or {}).get("k_1q_20q_bsz16_bytes_per_launch"),
+                     "kernel": "dense 1-qubit gate k_1q_t0/k_1q_tile/k_1q_pair (RY over every target bit, "
+                               "20 qubits bsz 16: 256 MiB/launch)", "avg_launch_ms": ms,
                      "at_26q_bsz16": {"achieved": achieved_big, "frac": achieved_big / HBM_PEAK_GBS,
                                       "avg_launch_ms": ms_big}},
         "cpu_baseline": cpu,

@@ -63,6 +63,17 @@ int tqsv_apply_swap(tqsv_stream_t s, void* state, int64_t bsz, int n, int bit_a,
  * qubitunitary / qubitunitaryfast (functional/qubit_unitary.py:19-69), QFT, RCCX ... */
 int tqsv_apply_kq(tqsv_stream_t s, void* state, int64_t bsz, int n, int k, const int* bits, uint64_t cmask,
                   uint64_t cval, const void* mat, int64_t mat_bstride);
+/* A run of the calls above from one host array of gate records
+ * (tqsv_gate_words() int64 words each: kind 1 = 1q, 2 = 2q, 3 = swap,
+ * 4 = diagonal, 5 = k-qubit; k; bits[10]; cmask; cval; matrix offset in
+ * complex elements from mats; matrix row stride).  Replaces the reference's
+ * Python loop of gate_wrapper calls over a circuit's gates
+ * (torchquantum/functional/gate_wrapper.py:324-446, called once per gate from
+ * QuantumModule.forward).  Stops at the first invalid gate (its index is in
+ * the message); gates before it have been enqueued. */
+int tqsv_gate_words(void);
+int tqsv_run_gates(tqsv_stream_t s, void* state, int64_t bsz, int n, const int64_t* gates, int n_gates,
+                   const void* mats);
 
 /* ---- fused stages: replace the static-mode block application
  *      (torchquantum/graph/graphs.py:384-532, 836-860) and runs of per-gate

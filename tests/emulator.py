@@ -304,12 +304,23 @@ def solo_apply(k: E.KOp, st, M, n, bstride, adjoint_of=False):
             s[idx] = s[idx] @ U.T
 
 
+def decode_gate_record(w):
+    """A tqsv_run_gates record (16 int64 words) -> (KOp, matrix offset, row stride),
+    read the way the C executor reads it."""
+    kinds = {1: "mat1", 2: "mat2", 3: "swap", 4: "diag", 5: "matk"}
+    kb = int(w[1])
+    cm, cv = int(w[12]), int(w[13])
+    cbits = tuple(b for b in range(64) if (cm >> b) & 1)
+    k = E.KOp(kinds[int(w[0])], tuple(int(b) for b in w[2:2 + kb]), cbits, tuple((cv >> b) & 1 for b in cbits))
+    return k, int(w[14]), int(w[15])
+
+
 def run_compiled(c: E.Compiled, pool: np.ndarray, state_in, init_zero=False):
     """Forward of a compiled plan; mirrors engine.forward."""
     n, bsz = c.n, c.bsz
     st = np.zeros((bsz, 1 << n), dtype=np.complex128) if state_in is None else np.array(state_in, np.complex128)
     first = True
-    for kind, payload in c.steps:
+    for kind, payload in c.fwd_steps:
         if kind == "stage":
             base, stg = payload
             stage_apply(c.prog_host[base:], st, pool, n, init_zero=init_zero and first)
@@ -317,9 +328,9 @@ def run_compiled(c: E.Compiled, pool: np.ndarray, state_in, init_zero=False):
             if init_zero and first:
                 st[:] = 0
                 st[:, 0] = 1
-            k = c.kops[payload]
-            off, bs = c.data_off[payload]
-            solo_apply(k, st, pool[off:], n, bs)
+            for w in payload:
+                k, off, bs = decode_gate_record(w)
+                solo_apply(k, st, pool[off:], n, bs)
         first = False
     if init_zero and first:
         st[:] = 0

@@ -18,6 +18,7 @@ _lib = None
 
 c_int_p = POINTER(c_int)
 c_u64_p = POINTER(c_uint64)
+c_i64_p = POINTER(c_int64)
 c_float_p = POINTER(c_float)
 
 # name -> (restype, argtypes); every entry point of include/tqsv.h
@@ -31,6 +32,8 @@ SIGNATURES = {
     "tqsv_apply_diag": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_uint64, c_uint64, c_uint64, c_void_p, c_int64]),
     "tqsv_apply_swap": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_int, c_uint64, c_uint64]),
     "tqsv_apply_kq": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_int_p, c_uint64, c_uint64, c_void_p, c_int64]),
+    "tqsv_gate_words": (c_int, []),
+    "tqsv_run_gates": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_i64_p, c_int, c_void_p]),
     "tqsv_stage_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int_p, c_void_p, c_void_p, c_int, c_int]),
     "tqsv_stage_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int_p, c_void_p, c_void_p,
                                    c_float_p, c_int64, c_int]),

@@ -70,10 +70,12 @@ static bool build_insert(InsertSpec& s, uint64_t mask, int n) {
 // ---------------------------------------------------------------------------
 // basis-state initialisation
 // ---------------------------------------------------------------------------
-__global__ void k_init_basis(float4* __restrict__ st, uint64_t n_pairs, int n, uint64_t index) {
+__global__ __launch_bounds__(256) void k_init_basis(float4* __restrict__ st, uint64_t n_pairs, int n,
+                                                   uint64_t index, int reps) {
   const uint64_t row_mask = (1ull << n) - 1ull;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_pairs;
-       i += (uint64_t)gridDim.x * blockDim.x) {
+  uint64_t i = tile_first(reps);
+  for (int r = 0; r < reps; ++r, i += 256) {
+    if (i >= n_pairs) return;
     const uint64_t a = 2 * i;  // amplitudes a, a+1 (same row: rows have even length)
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if ((a & row_mask) == (index & ~1ull)) {
@@ -101,44 +103,21 @@ __device__ __forceinline__ void load_m2(const float2* __restrict__ mat, int64_t 
 }
 
 // target bit 0: the pair is adjacent -> one 16-byte access per pair
-template <bool BATCHED>
+template <bool BATCHED, bool NT>
 __global__ __launch_bounds__(256) void k_1q_t0(float4* __restrict__ st, uint64_t n_items, InsertSpec ins,
                                               uint64_t cval, const float2* __restrict__ mat, int64_t bstride,
-                                              int lg_items_row) {
-  for (uint64_t it = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; it < n_items;
-       it += (uint64_t)gridDim.x * blockDim.x) {
+                                              int lg_items_row, int reps) {
+  uint64_t it = tile_first(reps);
+  for (int r = 0; r < reps; ++r, it += 256) {
+    if (it >= n_items) return;
     const uint64_t i0 = insert_zeros(it, ins) | cval;
     float2 m[4];
     load_m2<BATCHED>(mat, bstride, it >> lg_items_row, m);
-    float4 v = st[i0 >> 1];
+    const float4 v = ld4<NT>(st + (i0 >> 1));
     const float2 a0 = make_float2(v.x, v.y), a1 = make_float2(v.z, v.w);
     const float2 b0 = cfma(m[1], a1, cmul(m[0], a0));
     const float2 b1 = cfma(m[3], a1, cmul(m[2], a0));
-    st[i0 >> 1] = make_float4(b0.x, b0.y, b1.x, b1.y);
-  }
-}
-
-// target bit > 0 and bit 0 free: two consecutive items share 16-byte accesses.
-// Each thread keeps two such item pairs (grid-stride unroll) in flight; NT
-// marks the accesses non-temporal for states larger than the Infinity Cache.
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-template <bool NT>
-__device__ __forceinline__ float4 ld4(const float4* p) {
-  if constexpr (NT) {
-    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
-    return make_float4(v.x, v.y, v.z, v.w);
-  } else {
-    return *p;
-  }
-}
-template <bool NT>
-__device__ __forceinline__ void st4(float4* p, float4 v) {
-  if constexpr (NT) {
-    v4f t = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(t, reinterpret_cast<v4f*>(p));
-  } else {
-    *p = v;
+    st4<NT>(st + (i0 >> 1), make_float4(b0.x, b0.y, b1.x, b1.y));
   }
 }
 
@@ -153,37 +132,64 @@ __device__ __forceinline__ void rot2(const float2 (&m)[4], const float4 u, const
   wo = make_float4(b1.x, b1.y, b1n.x, b1n.y);
 }
 
-template <bool BATCHED, bool NT>
-__global__ __launch_bounds__(256) void k_1q_v2(float4* __restrict__ st, uint64_t n_pairs, InsertSpec ins,
-                                              uint64_t cval, uint64_t tmask, const float2* __restrict__ mat,
-                                              int64_t bstride, int lg_items_row) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < n_pairs; q += 2 * stride) {
-    const uint64_t qb = q + stride;
-    const bool two = qb < n_pairs;
-    const uint64_t i0 = insert_zeros(2 * q, ins) | cval;
-    const uint64_t i1 = i0 | tmask;
-    const uint64_t j0 = two ? (insert_zeros(2 * qb, ins) | cval) : i0;
-    const uint64_t j1 = j0 | tmask;
-    const float4 u = ld4<NT>(st + (i0 >> 1));
-    const float4 w = ld4<NT>(st + (i1 >> 1));
-    float4 u2 = u, w2 = w;
-    if (two) {
-      u2 = ld4<NT>(st + (j0 >> 1));
-      w2 = ld4<NT>(st + (j1 >> 1));
+// Target bit 1..8, bit 0 free, controls only on bits >= 9.  Items are float4
+// (two adjacent amplitudes) with the control bits removed; a workgroup's 256
+// items are one contiguous 4 KiB tile holding both halves of its pairs (the
+// target is float4 bit tb4 = target - 1 <= 7).  Each lane reads and writes
+// one float4 and trades it with the lane holding the partner: a wave shuffle
+// for tb4 <= 5, LDS for tb4 = 6, 7.  Requires n_items % 256 == 0.
+template <bool BATCHED, bool NT, bool LDSX>
+__global__ __launch_bounds__(256) void k_1q_tile(float4* __restrict__ st, uint64_t n_items, InsertSpec ins,
+                                                uint64_t cval4, int tb4, const float2* __restrict__ mat,
+                                                int64_t bstride, int lg_items_row, int reps) {
+  __shared__ float4 xch[LDSX ? 256 : 1];
+  const int S = 1 << tb4;
+  const bool hi = (threadIdx.x >> tb4) & 1;
+  uint64_t it = tile_first(reps);
+  for (int r = 0; r < reps; ++r, it += 256) {
+    if (it >= n_items) return;  // whole workgroups leave together (n_items % 256 == 0)
+    const uint64_t f = insert_zeros(it, ins) | cval4;
+    const float4 v = ld4<NT>(st + f);
+    float4 p;
+    if constexpr (LDSX) {
+      xch[threadIdx.x] = v;
+      __syncthreads();
+      p = xch[threadIdx.x ^ S];
+      __syncthreads();
+    } else {
+      p.x = __shfl_xor(v.x, S);
+      p.y = __shfl_xor(v.y, S);
+      p.z = __shfl_xor(v.z, S);
+      p.w = __shfl_xor(v.w, S);
     }
     float2 m[4];
-    load_m2<BATCHED>(mat, bstride, (2 * q) >> lg_items_row, m);
+    load_m2<BATCHED>(mat, bstride, it >> lg_items_row, m);
+    // lane with target 0: m00 own + m01 partner; target 1: m11 own + m10 partner
+    const float2 co = hi ? m[3] : m[0], cp = hi ? m[2] : m[1];
+    const float2 o = cfma(cp, make_float2(p.x, p.y), cmul(co, make_float2(v.x, v.y)));
+    const float2 on = cfma(cp, make_float2(p.z, p.w), cmul(co, make_float2(v.z, v.w)));
+    st4<NT>(st + f, make_float4(o.x, o.y, on.x, on.y));
+  }
+}
+
+// Target bit >= 1 and bit 0 free (any controls off bit 0): items are float4
+// pairs (f0, f0 | S) with the target and control bits removed (ins4 and cval4
+// in float4 bit positions, S = 1 << (target - 1)).
+template <bool BATCHED, bool NT>
+__global__ __launch_bounds__(256) void k_1q_pair(float4* __restrict__ st, uint64_t n_pairs, InsertSpec ins4,
+                                                uint64_t cval4, uint64_t S, const float2* __restrict__ mat,
+                                                int64_t bstride, int lg_pairs_row, int reps) {
+  uint64_t q = tile_first(reps);
+  for (int r = 0; r < reps; ++r, q += 256) {
+    if (q >= n_pairs) return;
+    const uint64_t f0 = insert_zeros(q, ins4) | cval4, f1 = f0 | S;
+    const float4 u = ld4<NT>(st + f0), w = ld4<NT>(st + f1);
+    float2 m[4];
+    load_m2<BATCHED>(mat, bstride, q >> lg_pairs_row, m);
     float4 uo, wo;
     rot2(m, u, w, uo, wo);
-    st4<NT>(st + (i0 >> 1), uo);
-    st4<NT>(st + (i1 >> 1), wo);
-    if (two) {
-      if (BATCHED) load_m2<BATCHED>(mat, bstride, (2 * qb) >> lg_items_row, m);
-      rot2(m, u2, w2, uo, wo);
-      st4<NT>(st + (j0 >> 1), uo);
-      st4<NT>(st + (j1 >> 1), wo);
-    }
+    st4<NT>(st + f0, uo);
+    st4<NT>(st + f1, wo);
   }
 }
 
@@ -191,9 +197,10 @@ __global__ __launch_bounds__(256) void k_1q_v2(float4* __restrict__ st, uint64_t
 template <bool BATCHED>
 __global__ __launch_bounds__(256) void k_1q_gen(float2* __restrict__ st, uint64_t n_items, InsertSpec ins,
                                                uint64_t cval, uint64_t tmask, const float2* __restrict__ mat,
-                                               int64_t bstride, int lg_items_row) {
-  for (uint64_t it = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; it < n_items;
-       it += (uint64_t)gridDim.x * blockDim.x) {
+                                               int64_t bstride, int lg_items_row, int reps) {
+  uint64_t it = tile_first(reps);
+  for (int r = 0; r < reps; ++r, it += 256) {
+    if (it >= n_items) return;
     const uint64_t i0 = insert_zeros(it, ins) | cval;
     const uint64_t i1 = i0 | tmask;
     float2 m[4];
@@ -207,7 +214,6 @@ __global__ __launch_bounds__(256) void k_1q_gen(float2* __restrict__ st, uint64_
 // ---------------------------------------------------------------------------
 // 2-qubit (optionally controlled) dense gate; matrix index = 2*b(hi) + b(lo)
 // ---------------------------------------------------------------------------
-template <bool BATCHED>
 __device__ __forceinline__ void mv4(const float2* __restrict__ p, const float2 (&a)[4], float2 (&b)[4]) {
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -218,82 +224,93 @@ __device__ __forceinline__ void mv4(const float2* __restrict__ p, const float2 (
   }
 }
 
-template <bool BATCHED>
-__global__ __launch_bounds__(256) void k_2q_v2(float4* __restrict__ st, uint64_t n_pairs, InsertSpec ins,
-                                              uint64_t cval, uint64_t mhi, uint64_t mlo,
-                                              const float2* __restrict__ mat, int64_t bstride, int lg_items_row) {
-  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < n_pairs;
-       q += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t it = 2 * q;
-    const uint64_t base = insert_zeros(it, ins) | cval;
-    const uint64_t idx[4] = {base, base | mlo, base | mhi, base | mhi | mlo};
-    const float2* p = BATCHED ? mat + (it >> lg_items_row) * bstride : mat;
+// bit 0 free: items are float4 quadruples (ins4/cval4/mhi4/mlo4 in float4 bit
+// positions), two adjacent amplitudes per access
+template <bool BATCHED, bool NT>
+__global__ __launch_bounds__(256) void k_2q_v2(float4* __restrict__ st, uint64_t n_quads, InsertSpec ins4,
+                                              uint64_t cval4, uint64_t mhi4, uint64_t mlo4,
+                                              const float2* __restrict__ mat, int64_t bstride, int lg_quads_row,
+                                              int reps) {
+  uint64_t q = tile_first(reps);
+  for (int r = 0; r < reps; ++r, q += 256) {
+    if (q >= n_quads) return;
+    const uint64_t base = insert_zeros(q, ins4) | cval4;
+    const uint64_t idx[4] = {base, base | mlo4, base | mhi4, base | mhi4 | mlo4};
+    const float2* p = BATCHED ? mat + (q >> lg_quads_row) * bstride : mat;
     float4 v[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) v[r] = st[idx[r] >> 1];
+    for (int k = 0; k < 4; ++k) v[k] = ld4<NT>(st + idx[k]);
     float2 a[4], b[4], an[4], bn[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      a[r] = make_float2(v[r].x, v[r].y);
-      an[r] = make_float2(v[r].z, v[r].w);
+    for (int k = 0; k < 4; ++k) {
+      a[k] = make_float2(v[k].x, v[k].y);
+      an[k] = make_float2(v[k].z, v[k].w);
     }
-    mv4<BATCHED>(p, a, b);
-    mv4<BATCHED>(p, an, bn);
+    mv4(p, a, b);
+    mv4(p, an, bn);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) st[idx[r] >> 1] = make_float4(b[r].x, b[r].y, bn[r].x, bn[r].y);
+    for (int k = 0; k < 4; ++k) st4<NT>(st + idx[k], make_float4(b[k].x, b[k].y, bn[k].x, bn[k].y));
   }
 }
 
 template <bool BATCHED>
 __global__ __launch_bounds__(256) void k_2q_gen(float2* __restrict__ st, uint64_t n_items, InsertSpec ins,
                                                uint64_t cval, uint64_t mhi, uint64_t mlo,
-                                               const float2* __restrict__ mat, int64_t bstride, int lg_items_row) {
-  for (uint64_t it = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; it < n_items;
-       it += (uint64_t)gridDim.x * blockDim.x) {
+                                               const float2* __restrict__ mat, int64_t bstride, int lg_items_row,
+                                               int reps) {
+  uint64_t it = tile_first(reps);
+  for (int r = 0; r < reps; ++r, it += 256) {
+    if (it >= n_items) return;
     const uint64_t base = insert_zeros(it, ins) | cval;
     const uint64_t idx[4] = {base, base | mlo, base | mhi, base | mhi | mlo};
     const float2* p = BATCHED ? mat + (it >> lg_items_row) * bstride : mat;
     float2 a[4], b[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) a[r] = st[idx[r]];
-    mv4<BATCHED>(p, a, b);
+    for (int k = 0; k < 4; ++k) a[k] = st[idx[k]];
+    mv4(p, a, b);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) st[idx[r]] = b[r];
+    for (int k = 0; k < 4; ++k) st[idx[k]] = b[k];
   }
 }
 
 // ---------------------------------------------------------------------------
 // diagonal gate: amplitudes whose control bits match get phase[parity(j & pmask)]
 // ---------------------------------------------------------------------------
+// generic (bit 0 is a control): one amplitude per item
 template <bool BATCHED>
 __global__ __launch_bounds__(256) void k_diag(float2* __restrict__ st, uint64_t n_items, InsertSpec ins,
                                              uint64_t cval, uint64_t pmask, const float2* __restrict__ ph,
-                                             int64_t bstride, int lg_items_row) {
-  for (uint64_t it = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; it < n_items;
-       it += (uint64_t)gridDim.x * blockDim.x) {
+                                             int64_t bstride, int lg_items_row, int reps) {
+  uint64_t it = tile_first(reps);
+  for (int r = 0; r < reps; ++r, it += 256) {
+    if (it >= n_items) return;
     const uint64_t i = insert_zeros(it, ins) | cval;
     const float2* p = BATCHED ? ph + (it >> lg_items_row) * bstride : ph;
     const int par = __popcll(i & pmask) & 1;
-    const float2 f = par ? p[1] : p[0];
-    st[i] = cmul(f, st[i]);
+    st[i] = cmul(par ? p[1] : p[0], st[i]);
   }
 }
 
-// no controls: two adjacent amplitudes per thread, 16-byte accesses
-template <bool BATCHED>
-__global__ __launch_bounds__(256) void k_diag_v2(float4* __restrict__ st, uint64_t n_pairs, uint64_t pmask,
-                                                const float2* __restrict__ ph, int64_t bstride, int n) {
-  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < n_pairs;
-       q += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t i = 2 * q;
-    const float2* p = BATCHED ? ph + (i >> n) * bstride : ph;
+// bit 0 free of controls: items are float4 (two adjacent amplitudes, each
+// with its own parity) with the control bits removed (ins4/cval4 in float4
+// bit positions; no controls = every float4 of the state)
+template <bool BATCHED, bool NT>
+__global__ __launch_bounds__(256) void k_diag4(float4* __restrict__ st, uint64_t n_items, InsertSpec ins4,
+                                              uint64_t cval4, uint64_t pmask, const float2* __restrict__ ph,
+                                              int64_t bstride, int lg_items_row, int reps) {
+  uint64_t it = tile_first(reps);
+  for (int r = 0; r < reps; ++r, it += 256) {
+    if (it >= n_items) return;
+    const uint64_t f = insert_zeros(it, ins4) | cval4;
+    const float2* p = BATCHED ? ph + (it >> lg_items_row) * bstride : ph;
     const float2 f0 = p[0], f1 = p[1];
-    float4 v = st[q];
+    const uint64_t i = 2 * f;
     const int par0 = __popcll(i & pmask) & 1;
     const int par1 = __popcll((i | 1ull) & pmask) & 1;
+    const float4 v = ld4<NT>(st + f);
     const float2 a = cmul(par0 ? f1 : f0, make_float2(v.x, v.y));
     const float2 b = cmul(par1 ? f1 : f0, make_float2(v.z, v.w));
-    st[q] = make_float4(a.x, a.y, b.x, b.y);
+    st4<NT>(st + f, make_float4(a.x, a.y, b.x, b.y));
   }
 }
 
@@ -301,14 +318,30 @@ __global__ __launch_bounds__(256) void k_diag_v2(float4* __restrict__ st, uint64
 // swap of bits a, b (optionally controlled): exchanges |..0..1..> <-> |..1..0..>
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_swap(float2* __restrict__ st, uint64_t n_items, InsertSpec ins,
-                                             uint64_t cval, uint64_t ma, uint64_t mb) {
-  for (uint64_t it = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; it < n_items;
-       it += (uint64_t)gridDim.x * blockDim.x) {
+                                             uint64_t cval, uint64_t ma, uint64_t mb, int reps) {
+  uint64_t it = tile_first(reps);
+  for (int r = 0; r < reps; ++r, it += 256) {
+    if (it >= n_items) return;
     const uint64_t base = insert_zeros(it, ins) | cval;
     const uint64_t i = base | mb, j = base | ma;
     const float2 x = st[i], y = st[j];
     st[i] = y;
     st[j] = x;
+  }
+}
+
+// bit 0 free: float4 items (positions in float4 bits)
+template <bool NT>
+__global__ __launch_bounds__(256) void k_swap4(float4* __restrict__ st, uint64_t n_items, InsertSpec ins4,
+                                              uint64_t cval4, uint64_t ma4, uint64_t mb4, int reps) {
+  uint64_t it = tile_first(reps);
+  for (int r = 0; r < reps; ++r, it += 256) {
+    if (it >= n_items) return;
+    const uint64_t base = insert_zeros(it, ins4) | cval4;
+    const uint64_t i = base | mb4, j = base | ma4;
+    const float4 x = ld4<NT>(st + i), y = ld4<NT>(st + j);
+    st4<NT>(st + i, y);
+    st4<NT>(st + j, x);
   }
 }
 
@@ -388,9 +421,34 @@ TQSV_API int tqsv_init_basis(hipStream_t s, void* state, int64_t bsz, int n, int
     k_init_basis_n0<<<grid_for(bsz, 256), 256, 0, s>>>((float2*)state, (uint64_t)bsz);
   } else {
     const uint64_t pairs = ((uint64_t)bsz << n) / 2;
-    k_init_basis<<<grid_for(pairs, 256), 256, 0, s>>>((float4*)state, pairs, n, (uint64_t)index);
+    const Tiles t = tiles_for(pairs);
+    k_init_basis<<<t.grid, 256, 0, s>>>((float4*)state, pairs, n, (uint64_t)index, t.reps);
   }
   return check_launch("tqsv_init_basis");
+}
+
+// (BATCHED, NT) instantiation of a 256-thread tiled kernel
+#define LAUNCH_BN(kern, b, nt, t, s, ...)                                  \
+  do {                                                                     \
+    if (b) {                                                               \
+      if (nt) kern<true, true><<<(t).grid, 256, 0, s>>>(__VA_ARGS__);      \
+      else kern<true, false><<<(t).grid, 256, 0, s>>>(__VA_ARGS__);        \
+    } else {                                                               \
+      if (nt) kern<false, true><<<(t).grid, 256, 0, s>>>(__VA_ARGS__);     \
+      else kern<false, false><<<(t).grid, 256, 0, s>>>(__VA_ARGS__);       \
+    }                                                                      \
+  } while (0)
+
+template <bool LDSX>
+static void launch_1q_tile(bool b, bool nt, Tiles t, hipStream_t s, float4* st, uint64_t n_items,
+                           const InsertSpec& ins, uint64_t cval4, int tb4, const float2* m, int64_t bs, int lg) {
+  if (b) {
+    if (nt) k_1q_tile<true, true, LDSX><<<t.grid, 256, 0, s>>>(st, n_items, ins, cval4, tb4, m, bs, lg, t.reps);
+    else k_1q_tile<true, false, LDSX><<<t.grid, 256, 0, s>>>(st, n_items, ins, cval4, tb4, m, bs, lg, t.reps);
+  } else {
+    if (nt) k_1q_tile<false, true, LDSX><<<t.grid, 256, 0, s>>>(st, n_items, ins, cval4, tb4, m, bs, lg, t.reps);
+    else k_1q_tile<false, false, LDSX><<<t.grid, 256, 0, s>>>(st, n_items, ins, cval4, tb4, m, bs, lg, t.reps);
+  }
 }
 
 TQSV_API int tqsv_apply_1q(hipStream_t s, void* state, int64_t bsz, int n, int tbit, uint64_t cmask,
@@ -404,30 +462,43 @@ TQSV_API int tqsv_apply_1q(hipStream_t s, void* state, int64_t bsz, int n, int t
     set_error("control mask overlaps target / out of range / value outside mask");
     return E_ARG;
   }
-  InsertSpec ins;
-  if (!build_insert(ins, cmask | tmask, n)) { set_error("too many control bits (max 11)"); return E_LIMIT; }
-  const int lg_items_row = n - 1 - __builtin_popcountll(cmask);
-  const uint64_t n_items = (uint64_t)bsz << lg_items_row;
+  const int c = __builtin_popcountll(cmask);
   const bool b = mat_bstride != 0;
+  const bool nt = stream_nt(bsz, n);
   const float2* m = (const float2*)mat;
-  if (tbit == 0) {
-    if (b) k_1q_t0<true><<<grid_for(n_items, 256), 256, 0, s>>>((float4*)state, n_items, ins, cval, m, mat_bstride, lg_items_row);
-    else k_1q_t0<false><<<grid_for(n_items, 256), 256, 0, s>>>((float4*)state, n_items, ins, cval, m, 0, lg_items_row);
-  } else if (!(cmask & 1ull)) {
-    const uint64_t n_pairs = n_items / 2;
-    // states beyond the 256 MB Infinity Cache stream with non-temporal hints
-    const bool nt = ((uint64_t)bsz << n) * sizeof(float2) > (256ull << 20);
-    const unsigned g = grid_for((n_pairs + 1) / 2, 256);
-    if (b) {
-      if (nt) k_1q_v2<true, true><<<g, 256, 0, s>>>((float4*)state, n_pairs, ins, cval, tmask, m, mat_bstride, lg_items_row);
-      else k_1q_v2<true, false><<<g, 256, 0, s>>>((float4*)state, n_pairs, ins, cval, tmask, m, mat_bstride, lg_items_row);
+  InsertSpec ins;
+  if (tbit == 0 || (cmask & 1ull)) {
+    if (!build_insert(ins, cmask | tmask, n)) { set_error("too many control bits (max 11)"); return E_LIMIT; }
+    const int lg = n - 1 - c;
+    const uint64_t n_items = (uint64_t)bsz << lg;
+    const Tiles t = tiles_for(n_items);
+    if (tbit == 0) {
+      LAUNCH_BN(k_1q_t0, b, nt, t, s, (float4*)state, n_items, ins, cval, m, mat_bstride, lg, t.reps);
+    } else if (b) {
+      k_1q_gen<true><<<t.grid, 256, 0, s>>>((float2*)state, n_items, ins, cval, tmask, m, mat_bstride, lg, t.reps);
     } else {
-      if (nt) k_1q_v2<false, true><<<g, 256, 0, s>>>((float4*)state, n_pairs, ins, cval, tmask, m, 0, lg_items_row);
-      else k_1q_v2<false, false><<<g, 256, 0, s>>>((float4*)state, n_pairs, ins, cval, tmask, m, 0, lg_items_row);
+      k_1q_gen<false><<<t.grid, 256, 0, s>>>((float2*)state, n_items, ins, cval, tmask, m, 0, lg, t.reps);
     }
+    return check_launch("tqsv_apply_1q");
+  }
+  // bit 0 free: float4 positions are amplitude positions - 1
+  const int tb4 = tbit - 1;
+  const uint64_t c4 = cmask >> 1, cval4 = cval >> 1;
+  const int lg_items = n - 1 - c;  // float4 per row with the control bits removed
+  if (tb4 <= 7 && (c4 & 0xFFull) == 0 && lg_items >= 8) {
+    if (!build_insert(ins, c4, n - 1)) { set_error("too many control bits (max 11)"); return E_LIMIT; }
+    const uint64_t n_items = (uint64_t)bsz << lg_items;
+    const Tiles t = tiles_for(n_items);
+    if (tb4 >= 6)
+      launch_1q_tile<true>(b, nt, t, s, (float4*)state, n_items, ins, cval4, tb4, m, mat_bstride, lg_items);
+    else
+      launch_1q_tile<false>(b, nt, t, s, (float4*)state, n_items, ins, cval4, tb4, m, mat_bstride, lg_items);
   } else {
-    if (b) k_1q_gen<true><<<grid_for(n_items, 256), 256, 0, s>>>((float2*)state, n_items, ins, cval, tmask, m, mat_bstride, lg_items_row);
-    else k_1q_gen<false><<<grid_for(n_items, 256), 256, 0, s>>>((float2*)state, n_items, ins, cval, tmask, m, 0, lg_items_row);
+    if (!build_insert(ins, c4 | (1ull << tb4), n - 1)) { set_error("too many control bits (max 11)"); return E_LIMIT; }
+    const int lg = n - 2 - c;
+    const uint64_t n_pairs = (uint64_t)bsz << lg;
+    const Tiles t = tiles_for(n_pairs);
+    LAUNCH_BN(k_1q_pair, b, nt, t, s, (float4*)state, n_pairs, ins, cval4, 1ull << tb4, m, mat_bstride, lg, t.reps);
   }
   return check_launch("tqsv_apply_1q");
 }
@@ -444,19 +515,25 @@ TQSV_API int tqsv_apply_2q(hipStream_t s, void* state, int64_t bsz, int n, int b
     set_error("control mask overlaps targets / out of range / value outside mask");
     return E_ARG;
   }
-  InsertSpec ins;
-  if (!build_insert(ins, cmask | mhi | mlo, n)) { set_error("too many control bits (max 10)"); return E_LIMIT; }
-  const int lg_items_row = n - 2 - __builtin_popcountll(cmask);
-  const uint64_t n_items = (uint64_t)bsz << lg_items_row;
+  const int c = __builtin_popcountll(cmask);
+  const uint64_t all = cmask | mhi | mlo;
   const bool b = mat_bstride != 0;
   const float2* m = (const float2*)mat;
-  if (!((cmask | mhi | mlo) & 1ull)) {
-    const uint64_t n_pairs = n_items / 2;
-    if (b) k_2q_v2<true><<<grid_for(n_pairs, 256), 256, 0, s>>>((float4*)state, n_pairs, ins, cval, mhi, mlo, m, mat_bstride, lg_items_row);
-    else k_2q_v2<false><<<grid_for(n_pairs, 256), 256, 0, s>>>((float4*)state, n_pairs, ins, cval, mhi, mlo, m, 0, lg_items_row);
+  InsertSpec ins;
+  if (!(all & 1ull)) {
+    if (!build_insert(ins, all >> 1, n - 1)) { set_error("too many control bits (max 10)"); return E_LIMIT; }
+    const int lg = n - 3 - c;
+    const uint64_t n_quads = (uint64_t)bsz << lg;
+    const Tiles t = tiles_for(n_quads);
+    LAUNCH_BN(k_2q_v2, b, stream_nt(bsz, n), t, s, (float4*)state, n_quads, ins, cval >> 1, mhi >> 1, mlo >> 1, m,
+              mat_bstride, lg, t.reps);
   } else {
-    if (b) k_2q_gen<true><<<grid_for(n_items, 256), 256, 0, s>>>((float2*)state, n_items, ins, cval, mhi, mlo, m, mat_bstride, lg_items_row);
-    else k_2q_gen<false><<<grid_for(n_items, 256), 256, 0, s>>>((float2*)state, n_items, ins, cval, mhi, mlo, m, 0, lg_items_row);
+    if (!build_insert(ins, all, n)) { set_error("too many control bits (max 10)"); return E_LIMIT; }
+    const int lg = n - 2 - c;
+    const uint64_t n_items = (uint64_t)bsz << lg;
+    const Tiles t = tiles_for(n_items);
+    if (b) k_2q_gen<true><<<t.grid, 256, 0, s>>>((float2*)state, n_items, ins, cval, mhi, mlo, m, mat_bstride, lg, t.reps);
+    else k_2q_gen<false><<<t.grid, 256, 0, s>>>((float2*)state, n_items, ins, cval, mhi, mlo, m, 0, lg, t.reps);
   }
   return check_launch("tqsv_apply_2q");
 }
@@ -470,20 +547,25 @@ TQSV_API int tqsv_apply_diag(hipStream_t s, void* state, int64_t bsz, int n, uin
     set_error("diag masks out of range or overlapping");
     return E_ARG;
   }
+  const int c = __builtin_popcountll(cmask);
   const bool b = ph_bstride != 0;
   const float2* p = (const float2*)phases;
-  if (cmask == 0 && n >= 1) {
-    const uint64_t n_pairs = ((uint64_t)bsz << n) / 2;
-    if (b) k_diag_v2<true><<<grid_for(n_pairs, 256), 256, 0, s>>>((float4*)state, n_pairs, pmask, p, ph_bstride, n);
-    else k_diag_v2<false><<<grid_for(n_pairs, 256), 256, 0, s>>>((float4*)state, n_pairs, pmask, p, 0, n);
+  InsertSpec ins;
+  if (n >= 1 && !(cmask & 1ull)) {
+    if (!build_insert(ins, cmask >> 1, n - 1)) { set_error("too many control bits (max 12)"); return E_LIMIT; }
+    const int lg = n - 1 - c;
+    const uint64_t n_items = (uint64_t)bsz << lg;
+    const Tiles t = tiles_for(n_items);
+    LAUNCH_BN(k_diag4, b, stream_nt(bsz, n), t, s, (float4*)state, n_items, ins, cval >> 1, pmask, p, ph_bstride,
+              lg, t.reps);
     return check_launch("tqsv_apply_diag");
   }
-  InsertSpec ins;
   if (!build_insert(ins, cmask, n)) { set_error("too many control bits (max 12)"); return E_LIMIT; }
-  const int lg_items_row = n - __builtin_popcountll(cmask);
-  const uint64_t n_items = (uint64_t)bsz << lg_items_row;
-  if (b) k_diag<true><<<grid_for(n_items, 256), 256, 0, s>>>((float2*)state, n_items, ins, cval, pmask, p, ph_bstride, lg_items_row);
-  else k_diag<false><<<grid_for(n_items, 256), 256, 0, s>>>((float2*)state, n_items, ins, cval, pmask, p, 0, lg_items_row);
+  const int lg = n - c;
+  const uint64_t n_items = (uint64_t)bsz << lg;
+  const Tiles t = tiles_for(n_items);
+  if (b) k_diag<true><<<t.grid, 256, 0, s>>>((float2*)state, n_items, ins, cval, pmask, p, ph_bstride, lg, t.reps);
+  else k_diag<false><<<t.grid, 256, 0, s>>>((float2*)state, n_items, ins, cval, pmask, p, 0, lg, t.reps);
   return check_launch("tqsv_apply_diag");
 }
 
@@ -498,10 +580,23 @@ TQSV_API int tqsv_apply_swap(hipStream_t s, void* state, int64_t bsz, int n, int
     set_error("control mask overlaps swap bits / out of range");
     return E_ARG;
   }
+  const int c = __builtin_popcountll(cmask);
+  const uint64_t all = cmask | ma | mb;
   InsertSpec ins;
-  if (!build_insert(ins, cmask | ma | mb, n)) { set_error("too many control bits (max 10)"); return E_LIMIT; }
-  const uint64_t n_items = (uint64_t)bsz << (n - 2 - __builtin_popcountll(cmask));
-  k_swap<<<grid_for(n_items, 256), 256, 0, s>>>((float2*)state, n_items, ins, cval, ma, mb);
+  if (!(all & 1ull)) {
+    if (!build_insert(ins, all >> 1, n - 1)) { set_error("too many control bits (max 10)"); return E_LIMIT; }
+    const uint64_t n_items = (uint64_t)bsz << (n - 3 - c);
+    const Tiles t = tiles_for(n_items);
+    if (stream_nt(bsz, n))
+      k_swap4<true><<<t.grid, 256, 0, s>>>((float4*)state, n_items, ins, cval >> 1, ma >> 1, mb >> 1, t.reps);
+    else
+      k_swap4<false><<<t.grid, 256, 0, s>>>((float4*)state, n_items, ins, cval >> 1, ma >> 1, mb >> 1, t.reps);
+    return check_launch("tqsv_apply_swap");
+  }
+  if (!build_insert(ins, all, n)) { set_error("too many control bits (max 10)"); return E_LIMIT; }
+  const uint64_t n_items = (uint64_t)bsz << (n - 2 - c);
+  const Tiles t = tiles_for(n_items);
+  k_swap<<<t.grid, 256, 0, s>>>((float2*)state, n_items, ins, cval, ma, mb, t.reps);
   return check_launch("tqsv_apply_swap");
 }
 
@@ -542,4 +637,59 @@ TQSV_API int tqsv_apply_kq(hipStream_t s, void* state, int64_t bsz, int n, int k
     k_kq<false><<<grid, threads, shm, s>>>((float2*)state, n_groups, ins, cval, ks, (const float2*)mat, 0,
                                            lg_groups_row);
   return check_launch("tqsv_apply_kq");
+}
+
+// ---------------------------------------------------------------------------
+// Per-gate plan executor: a run of gates, one launch each, issued from native
+// code so the host loop costs a few microseconds per gate.  Record layout
+// (TQSV_GATE_WORDS int64 words per gate):
+//   [0] kind: 1 dense 1q, 2 dense 2q, 3 swap, 4 diagonal, 5 dense k-qubit
+//   [1] k, the number of entries used in [2..11]
+//   [2..11] bits: target(s) (bits[0] = matrix MSB) / swap bits / parity bits
+//   [12] cmask  [13] cval
+//   [14] offset of the gate's matrix (phases) in `mats`, in complex64 elements
+//   [15] matrix stride between batch rows (complex elements, 0 = shared)
+// ---------------------------------------------------------------------------
+static const int kGateWords = 16;
+
+TQSV_API int tqsv_gate_words(void) { return kGateWords; }
+
+TQSV_API int tqsv_run_gates(hipStream_t s, void* state, int64_t bsz, int n, const int64_t* gates, int n_gates,
+                            const void* mats) {
+  if (n_gates < 0 || (n_gates > 0 && !gates)) { set_error("tqsv_run_gates: bad gate list"); return E_ARG; }
+  const char* base = (const char*)mats;
+  for (int g = 0; g < n_gates; ++g) {
+    const int64_t* w = gates + (int64_t)g * kGateWords;
+    const int k = (int)w[1];
+    if (k < 0 || k > 10) { set_error("tqsv_run_gates: gate %d has k=%d outside [0, 10]", g, k); return E_ARG; }
+    int bits[10];
+    uint64_t pmask = 0;
+    for (int q = 0; q < k; ++q) {
+      bits[q] = (int)w[2 + q];
+      if (bits[q] < 0 || bits[q] >= 64) { set_error("tqsv_run_gates: gate %d bit out of range", g); return E_ARG; }
+      pmask |= 1ull << bits[q];
+    }
+    const uint64_t cmask = (uint64_t)w[12], cval = (uint64_t)w[13];
+    const void* m = base ? base + 8 * w[14] : nullptr;
+    const int64_t bs = w[15];
+    const int kind = (int)w[0];
+    int need = kind == 1 ? 1 : (kind == 2 || kind == 3) ? 2 : -1;
+    if (need > 0 && k != need) { set_error("tqsv_run_gates: gate %d kind %d needs k=%d", g, kind, need); return E_ARG; }
+    int rc;
+    switch (kind) {
+      case 1: rc = tqsv_apply_1q(s, state, bsz, n, bits[0], cmask, cval, m, bs); break;
+      case 2: rc = tqsv_apply_2q(s, state, bsz, n, bits[0], bits[1], cmask, cval, m, bs); break;
+      case 3: rc = tqsv_apply_swap(s, state, bsz, n, bits[0], bits[1], cmask, cval); break;
+      case 4: rc = tqsv_apply_diag(s, state, bsz, n, cmask, cval, pmask, m, bs); break;
+      case 5: rc = tqsv_apply_kq(s, state, bsz, n, k, bits, cmask, cval, m, bs); break;
+      default: set_error("tqsv_run_gates: gate %d has unknown kind %d", g, kind); return E_ARG;
+    }
+    if (rc) {
+      char msg[sizeof(g_err)];
+      snprintf(msg, sizeof(msg), "%s", g_err);
+      set_error("tqsv_run_gates: gate %d: %s", g, msg);
+      return rc;
+    }
+  }
+  return OK;
 }

@@ -869,6 +869,26 @@ class Compiled:
             self.steps = [("solo", i) for i in range(len(kops))]
             self.prog_host = None
             self.prog_dev = None
+        self.fwd_steps = self._forward_schedule()
+
+    def _forward_schedule(self) -> List[tuple]:
+        """The forward steps with each run of per-gate launches packed into one
+        array of gate records for tqsv_run_gates: ("stage", payload) |
+        ("gates", int64 [count, GATE_WORDS])."""
+        out: List[tuple] = []
+        run: List[np.ndarray] = []
+        for kind, payload in self.steps:
+            if kind == "solo":
+                off, bs = self.data_off[payload]
+                run.append(gate_record(self.kops[payload], off, bs))
+                continue
+            if run:
+                out.append(("gates", np.stack(run)))
+                run = []
+            out.append((kind, payload))
+        if run:
+            out.append(("gates", np.stack(run)))
+        return out
 
     # ---------------------------------------------------------------- values
     def pool(self, tape: Sequence[TapeOp], with_grad: bool, with_adj: bool) -> torch.Tensor:
@@ -1041,6 +1061,26 @@ def compile_tape(tape: Sequence[TapeOp], n: int, bsz: int, opts: EngineOptions, 
 # ----------------------------------------------------------------------------
 # execution
 # ----------------------------------------------------------------------------
+GATE_WORDS = 16
+GATE_KIND = {"mat1": 1, "mat2": 2, "swap": 3, "diag": 4, "matk": 5}
+
+
+def gate_record(k: KOp, off: int, bstride: int) -> np.ndarray:
+    """One tqsv_run_gates record: kind, k, bits[10], cmask, cval, matrix
+    offset (complex elements into the pool), matrix row stride."""
+    if len(k.bits) > 10:
+        raise ValueError(f"dense gate on {len(k.bits)} wires: the per-gate kernels take at most 10")
+    w = np.zeros(GATE_WORDS, dtype=np.int64)
+    w[0] = GATE_KIND[k.kind]
+    w[1] = len(k.bits)
+    w[2:2 + len(k.bits)] = k.bits
+    w[12] = k.cmask
+    w[13] = k.cval
+    w[14] = off
+    w[15] = bstride
+    return w
+
+
 def _pool_ptr(pool: torch.Tensor, off: int) -> int:
     return pool.data_ptr() + 8 * off
 
@@ -1076,7 +1116,7 @@ def forward(c: Compiled, state_in: torch.Tensor, pool: torch.Tensor, init_zero: 
     if init_zero and (not c.steps or c.steps[0][0] != "stage"):
         N.call("tqsv_init_basis", stream, out.data_ptr(), bsz, n, 0)
         fresh = True
-    for kind, payload in c.steps:
+    for kind, payload in c.fwd_steps:
         if kind == "stage":
             base, stg = payload
             src = out if fresh else cur_in
@@ -1092,9 +1132,8 @@ def forward(c: Compiled, state_in: torch.Tensor, pool: torch.Tensor, init_zero: 
                 else:
                     out.copy_(cur_in.reshape(bsz, -1))
                 fresh = True
-            k = c.kops[payload]
-            off, bs = c.data_off[payload]
-            run_solo_forward(out, k, c, pool, off, bs, stream)
+            N.call("tqsv_run_gates", stream, out.data_ptr(), bsz, n, payload.ctypes.data_as(N.c_i64_p),
+                   len(payload), pool.data_ptr())
     if not fresh:
         if init_zero:
             N.call("tqsv_init_basis", stream, out.data_ptr(), bsz, n, 0)
