@@ -120,6 +120,16 @@ int tqsv_expval_pauli(tqsv_stream_t s, const void* state, int64_t bsz, int n, in
 int tqsv_pauli_apply(tqsv_stream_t s, void* out, const void* in, int64_t bsz, int n, int nterms,
                      const uint64_t* xmask, const uint64_t* zmask, const void* coef, int accumulate);
 
+/* ---- shot sampling: replaces measure()'s host-side random.choices over
+ *      |psi|^2 (torchquantum/measurement/measurements.py:37-74), also behind
+ *      expval_joint_sampling (:168-229).  out[row * n_shots + k]: in-row index
+ *      of shot k (wire 0 = most significant bit); uniforms come from a
+ *      counter-based generator keyed by (seed, row, shot).  n <= 32,
+ *      bsz * n_shots <= 2^26.  workspace: tqsv_sample_workspace_bytes(). */
+int64_t tqsv_sample_workspace_bytes(int64_t bsz, int n);
+int tqsv_sample(tqsv_stream_t s, const void* state, int64_t bsz, int n, int64_t n_shots, uint64_t seed,
+                int64_t* out, void* workspace, int64_t ws_bytes);
+
 /* Fold matrix gradients (ops flagged F_MG in a stage program: 8 floats
  * sum(lam conj(psi)^T) per op) into parameter gradients:
  * grad[row, slot_p] += Re sum_ab D_p[a][b] conj(Mg[a][b]); items: MgItem array

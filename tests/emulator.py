@@ -351,7 +351,7 @@ def adjoint_compiled(c: E.Compiled, pool: np.ndarray, psi, lam):
         if k.P > 0:
             off, bs = c.data_off[payload]
             # U^dag on psi, derivative inner products, U^dag on lam -- the per-gate adjoint kernels
-            _solo_adjoint_grad(k, psi, lam, pool[off:], bs, n, grad)
+            _solo_adjoint_grad(k, psi, lam, pool[off:], bs, n, grad, c.scatter_col.get(payload, k.grad_slots[0]))
         else:
             if k.kind == "swap":
                 solo_apply(k, psi, pool, n, 0)
@@ -368,10 +368,15 @@ def adjoint_compiled(c: E.Compiled, pool: np.ndarray, psi, lam):
             D = pool[it["d_off"] + row * it["bstride"]:]
             for q in range(it["P"]):
                 grad[row, it["slot"][q]] += np.real(np.sum(D[4 * q:4 * q + 4] * np.conj(Mg)))
+    # scatter columns of per-gate ops with non-consecutive slots
+    for i, col, sl in c.scatter:
+        for q, slot in enumerate(sl):
+            grad[:, slot] += grad[:, col + q]
     return grad[:, :max(c.n_slots, 1)]
 
 
-def _solo_adjoint_grad(k, psi, lam, M, bs, n, grad):
+def _solo_adjoint_grad(k, psi, lam, M, bs, n, grad, g0):
+    """The per-gate adjoint step; parameter q's gradient goes to column g0 + q."""
     bsz = psi.shape[0]
     d = 2 if k.kind == "diag" else k.dim() ** 2
     for row in range(bsz):
@@ -385,7 +390,7 @@ def _solo_adjoint_grad(k, psi, lam, M, bs, n, grad):
         for q, D in enumerate(Ds):
             v = y.copy()
             _apply_plain(k, v, D, n, zero_inactive=True)
-            grad[row, k.grad_slots[q]] += np.real(np.sum(np.conj(ls) * v))
+            grad[row, g0 + q] += np.real(np.sum(np.conj(ls) * v))
         psi[row] = y[0]
         lam[row] = _apply_adj(k, ls, U, n)[0]
 

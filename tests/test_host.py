@@ -174,6 +174,50 @@ def test_plan_adjoint_emulated(mode, seed):
     assert np.all(np.abs(got - fd) < 1e-4 * (1 + np.abs(fd)))
 
 
+@pytest.mark.parametrize("width", [2, 3])
+def test_unstaged_fused_blocks_adjoint_emulated(width):
+    """Fewer wires than register bits: a fused plan runs every op (trainable
+    2-wire blocks included) as per-gate launches, whose adjoint writes P
+    consecutive columns; blocks with non-consecutive parameter slots go
+    through extra columns that are added into their slots.  Gradients vs FD."""
+    rng = np.random.default_rng(31 + width)
+    n, bsz = 3, 2
+    ops = random_ops(n, 30, bsz, rng, names=["rx", "ry", "rz", "u3", "cnot", "cu3", "crx", "rzz", "h", "cz"])
+    tape = _tape_from_ops(ops, grad=True)
+    c = E.Compiled(tape, n, bsz, E.EngineOptions(mode="fused", fusion_width=width), "cpu")
+    assert all(kind == "solo" for kind, _ in c.steps)
+    assert c.scatter, "expected a fused block with non-consecutive parameter slots"
+    pool = c.pool_host(tape, True, True).astype(np.complex128)
+    init = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+    cvec = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+    psi = EM.run_compiled(c, pool, init)
+    assert np.abs(psi - O.run_ops(oracle_ops(ops, bsz), n, bsz, state=init)).max() < 1e-6
+    grad = EM.adjoint_compiled(c, pool, psi, cvec.copy())
+
+    def L(flat):
+        k = 0
+        o2 = []
+        for op in ops:
+            p = op["params"]
+            if p is not None:
+                p = flat[k:k + p.size].reshape(p.shape)
+                k += p.size
+            o2.append(dict(op, params=p))
+        return float(np.real(np.sum(np.conj(cvec) * O.run_ops(oracle_ops(o2, bsz), n, bsz, state=init))))
+
+    flat = np.concatenate([op["params"].reshape(-1) for op in ops if op["params"] is not None])
+    fd = O.central_fd(L, flat.copy(), h=1e-5)
+    got = []
+    for ti, op in enumerate(tape):
+        if op.params is None:
+            continue
+        off = c.param_offsets[ti]
+        g = grad[:, off:off + op.n_params()]
+        got.append(g.sum(0) if op.params.shape[0] == 1 else g.reshape(-1))
+    got = np.concatenate(got)
+    assert np.all(np.abs(got - fd) < 1e-4 * (1 + np.abs(fd)))
+
+
 def test_partition_properties():
     """Stages hold at most m local bits, keep program order on every bit and
     cover every op exactly once."""

@@ -11,12 +11,13 @@ __version__ = "0.1.0"
 from .macro import C_DTYPE, F_DTYPE
 from . import gates
 from . import functional
-from .module import QuantumModule, QuantumModuleList, QuantumModuleDict
+from .module import QuantumModule, QuantumModuleList, QuantumModuleDict, static_support
 from .device import QuantumDevice, set_engine_options, get_engine_options
 from .operator import *  # noqa: F401,F403
 from .operator import op_name_dict, fixed_ops, parameterized_ops, Operator, Operation, Observable
 from .measurement import (expval, MeasureAll, expval_joint_analytical, expval_joint_analytical_batch,
-                          expval_pauli_sum, MeasureMultiPauliSum)
+                          expval_pauli_sum, MeasureMultiPauliSum, gen_bitstrings, measure, sample_indices,
+                          find_observable_groups, expval_joint_sampling, expval_joint_sampling_grouping)
 from .layer import (Op1QAllLayer, Op2QAllLayer, EntanglePairwise, LayerTemplate0, RXYZCXLayer0, RandomLayer)
 from .engine import EngineOptions
 from . import device as _device_mod

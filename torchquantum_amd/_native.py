@@ -55,6 +55,8 @@ SIGNATURES = {
     "tqsv_batch_zexpval": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int64, c_float_p, c_float_p]),
     "tqsv_mg_item_bytes": (c_int, []),
     "tqsv_mg_to_grad": (c_int, [c_void_p, c_float_p, c_int64, c_int64, c_void_p, c_int, c_void_p]),
+    "tqsv_sample_workspace_bytes": (c_int64, [c_int64, c_int]),
+    "tqsv_sample": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int64, c_uint64, c_void_p, c_void_p, c_int64]),
     "tqsv_gemm": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
                           c_float, c_void_p, c_int, c_void_p, c_int]),
     "tqsv_relu_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),

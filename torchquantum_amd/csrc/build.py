@@ -24,6 +24,7 @@ SOURCES = [
     "tqsv_adjoint.hip",
     "tqsv_batch.hip",
     "tqsv_mlp.hip",
+    "tqsv_sample.hip",
 ]
 
 ARCH = os.environ.get("TQSV_ARCH", "gfx950")

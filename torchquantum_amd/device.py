@@ -77,7 +77,8 @@ class QuantumDevice(nn.Module):
         s[:, 0] = 1.0
         return s
 
-    def _flush(self):
+    def _flush(self, options: Optional[E.EngineOptions] = None):
+        opts = options if options is not None else self._options()
         if not self._tape:
             if self._sv is None:
                 self._sv = self._zero_state()
@@ -89,17 +90,17 @@ class QuantumDevice(nn.Module):
         dev = self._sv_device()
         if self._sv is None:
             dummy = torch.empty(0, dtype=C_DTYPE, device=dev)
-            self._sv = run_tape(tape, dummy, self.n_wires, self.bsz, self._options(), init_zero=True)
+            self._sv = run_tape(tape, dummy, self.n_wires, self.bsz, opts, init_zero=True)
         else:
             st = self._sv
             if st.dtype != C_DTYPE:
                 st = st.to(C_DTYPE)
-            self._sv = run_tape(tape, st.reshape(self.bsz, -1).contiguous(), self.n_wires, self.bsz,
-                                self._options())
+            self._sv = run_tape(tape, st.reshape(self.bsz, -1).contiguous(), self.n_wires, self.bsz, opts)
 
-    def flush(self):
-        """Execute pending gates now (normally implicit on any state read)."""
-        self._flush()
+    def flush(self, options: Optional[E.EngineOptions] = None):
+        """Execute pending gates now (normally implicit on any state read),
+        with ``options`` instead of the device's options if given."""
+        self._flush(options)
 
     # ---------------------------------------------------------------- states
     @property

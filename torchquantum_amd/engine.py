@@ -869,6 +869,18 @@ class Compiled:
             self.steps = [("solo", i) for i in range(len(kops))]
             self.prog_host = None
             self.prog_dev = None
+        # Per-gate adjoint launches write P consecutive gradient columns.  A
+        # trainable op whose slots are not consecutive (a fused block of
+        # several gates) gets its own columns past the others; they are
+        # added into its slots after the sweep.
+        self.scatter: List[tuple] = []
+        for kind, payload in self.steps:
+            if kind == "solo":
+                sl = list(kops[payload].grad_slots)
+                if sl and sl != list(range(sl[0], sl[0] + len(sl))):
+                    self.scatter.append((payload, self.grad_width, sl))
+                    self.grad_width += len(sl)
+        self.scatter_col = {i: col for i, col, _ in self.scatter}
         self.fwd_steps = self._forward_schedule()
 
     def _forward_schedule(self) -> List[tuple]:
@@ -1162,21 +1174,19 @@ def adjoint(c: Compiled, psi: torch.Tensor, lam: torch.Tensor, pool: torch.Tenso
         if k.P > 0:
             off, bs = c.data_off[i]
             p = _pool_ptr(pool, off)
-            slots = k.grad_slots
-            if list(slots) != list(range(slots[0], slots[0] + len(slots))):
-                raise RuntimeError("per-gate adjoint needs contiguous parameter slots")
+            g0 = c.scatter_col.get(i, k.grad_slots[0])
             if k.kind == "mat1":
                 N.call("tqsv_adjoint_1q", stream, psi.data_ptr(), lam.data_ptr(), bsz, n, k.bits[0], k.cmask,
-                       k.cval, p, bs, k.P, N.fptr(grad), gstride, slots[0])
+                       k.cval, p, bs, k.P, N.fptr(grad), gstride, g0)
             elif k.kind == "mat2":
                 N.call("tqsv_adjoint_2q", stream, psi.data_ptr(), lam.data_ptr(), bsz, n, k.bits[0], k.bits[1],
-                       k.cmask, k.cval, p, bs, k.P, N.fptr(grad), gstride, slots[0])
+                       k.cmask, k.cval, p, bs, k.P, N.fptr(grad), gstride, g0)
             elif k.kind == "diag":
                 pm = 0
                 for b in k.bits:
                     pm |= 1 << b
                 N.call("tqsv_adjoint_diag", stream, psi.data_ptr(), lam.data_ptr(), bsz, n, k.cmask, k.cval, pm,
-                       p, bs, k.P, N.fptr(grad), gstride, slots[0])
+                       p, bs, k.P, N.fptr(grad), gstride, g0)
             else:
                 raise NotImplementedError(f"adjoint of trainable '{k.kind}' op")
         else:
@@ -1191,4 +1201,7 @@ def adjoint(c: Compiled, psi: torch.Tensor, lam: torch.Tensor, pool: torch.Tenso
             c._mg_dev = torch.from_numpy(c.mg_items.view(np.uint8).copy()).to(psi.device)
         N.call("tqsv_mg_to_grad", stream, N.fptr(grad), gstride, bsz, c._mg_dev.data_ptr(), len(c.mg_items),
                pool.data_ptr())
+    for i, col, sl in c.scatter:
+        idx = torch.as_tensor(sl, dtype=torch.long, device=grad.device)
+        grad.index_add_(1, idx, grad[:, col:col + len(sl)].clone())
     return grad[:, :max(c.n_slots, 1)]

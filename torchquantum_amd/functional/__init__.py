@@ -64,10 +64,9 @@ def gate_wrapper(name, q_device, wires, params=None, n_wires=None, static=False,
             "inverse": inverse,
             "trainable": params.requires_grad if params is not None else False,
         })
-    if static:
-        parent_graph.add_func(name=name, wires=wires, parent_graph=parent_graph, params=params,
-                              n_wires=n_wires, inverse=inverse)
-        return
+    # static mode (gate_wrapper.py:393-403 hands the gate to a QuantumGraph):
+    # here every gate goes on the device tape; the graph-top module's
+    # static_support flush compiles the tape with its wires_per_block.
     n = q_device.n_wires
     if len(set(wires)) != len(wires):
         raise ValueError(f"gate '{name}' applied to repeated wires {wires}")

@@ -6,7 +6,7 @@ import numpy as np
 import torch
 
 from . import operator as op
-from .module import QuantumModule, QuantumModuleList
+from .module import QuantumModule, QuantumModuleList, static_support
 
 __all__ = ["Op1QAllLayer", "Op2QAllLayer", "EntanglePairwise", "LayerTemplate0", "RXYZCXLayer0",
            "RandomLayer"]
@@ -23,6 +23,7 @@ class Op1QAllLayer(QuantumModule):
         for _ in range(n_wires):
             self.ops_all.append(op(has_params=has_params, trainable=trainable))
 
+    @static_support
     def forward(self, q_device):
         for k in range(self.n_wires):
             self.ops_all[k](q_device, wires=k)
@@ -44,6 +45,7 @@ class Op2QAllLayer(QuantumModule):
         for _ in range(n_ops):
             self.ops_all.append(op(has_params=has_params, trainable=trainable))
 
+    @static_support
     def forward(self, q_device):
         for k in range(len(self.ops_all)):
             wires = [k, (k + self.jump) % self.n_wires]
@@ -64,6 +66,7 @@ class EntanglePairwise(QuantumModule):
         for _ in range(n_wires - 1):
             self.ops_all.append(op(has_params=has_params, trainable=trainable))
 
+    @static_support
     def forward(self, q_device):
         k = 0
         for start in (0, 1):
@@ -89,6 +92,7 @@ class LayerTemplate0(QuantumModule):
     def build_layers(self):
         raise NotImplementedError
 
+    @static_support
     def forward(self, q_device):
         for layer in self.layers_all:
             layer(q_device)
@@ -141,6 +145,7 @@ class RandomLayer(QuantumModule):
             self.op_list.append(o)
             self.op_wires.append(w)
 
+    @static_support
     def forward(self, q_device):
         for o, w in zip(self.op_list, self.op_wires):
             o(q_device, wires=w)

@@ -2,12 +2,47 @@
 reference's static-mode attributes, op-history replay and ``get_unitary``."""
 from __future__ import annotations
 
+import functools
 from typing import Iterable
 
 import torch
 import torch.nn as nn
 
-__all__ = ["QuantumModule", "QuantumModuleList", "QuantumModuleDict"]
+__all__ = ["QuantumModule", "QuantumModuleList", "QuantumModuleDict", "static_support"]
+
+# widest dense block the fused planner builds
+MAX_STATIC_BLOCK = 4
+
+
+def _find_device(module, args, kwargs):
+    from .device import QuantumDevice
+    for a in list(args) + list(kwargs.values()):
+        if isinstance(a, QuantumDevice):
+            return a
+    q = getattr(module, "q_device", None)
+    return q if isinstance(q, QuantumDevice) else None
+
+
+def static_support(f):
+    """Decorator for ``QuantumModule.forward`` (graph/graphs.py:60-81).
+
+    In the reference, static mode records the gates of the forward into a
+    QuantumGraph, packs them into blocks of ``wires_per_block`` wires and
+    applies one unitary per block when the graph-top module's forward ends.
+    Here the gates are already recorded on the device tape; when a graph-top
+    module in static mode finishes its forward, the tape is compiled by the
+    fused planner with fusion width ``wires_per_block`` (blocks wider than
+    MAX_STATIC_BLOCK wires are built as MAX_STATIC_BLOCK-wire blocks, which
+    gives the same state) and run."""
+
+    @functools.wraps(f)
+    def forward_static(self, *args, **kwargs):
+        res = f(self, *args, **kwargs)
+        if getattr(self, "static_mode", False) and getattr(self, "is_graph_top", False):
+            self.static_forward(_find_device(self, args, kwargs))
+        return res
+
+    return forward_static
 
 
 class QuantumModule(nn.Module):
@@ -57,16 +92,40 @@ class QuantumModule(nn.Module):
         self.wires_per_block = wires_per_block
 
     def static_on(self, is_graph_top=True, wires_per_block=3):
-        """Static mode (modules.py:230-261).  The engine always compiles the
-        recorded gates into fused stages, so static mode only records the
-        block width, used as the fused-stage hint."""
-        self.wires_per_block = wires_per_block
-        self.static_mode = False
+        """Static mode (modules.py:230-261): this module and its quantum
+        sub-modules record into one graph; the top module's forward (under
+        ``static_support``) ends by running the recorded gates fused into
+        blocks of ``wires_per_block`` wires."""
+        if int(wires_per_block) < 1:
+            raise ValueError(f"wires_per_block must be >= 1 (got {wires_per_block})")
+        self.wires_per_block = int(wires_per_block)
+        self.static_mode = True
         self.is_graph_top = is_graph_top
+        for m in self.children():
+            if isinstance(m, QuantumModule) and not _is_device(m):
+                m.static_on(is_graph_top=False, wires_per_block=wires_per_block)
 
     def static_off(self):
+        """modules.py:263-277"""
         self.static_mode = False
         self.graph = None
+        for m in self.children():
+            if isinstance(m, QuantumModule) and not _is_device(m):
+                m.static_off()
+
+    def static_options(self):
+        """Engine options of a static-mode flush."""
+        from .engine import EngineOptions
+        width = max(1, min(int(self.wires_per_block or 1), MAX_STATIC_BLOCK))
+        return EngineOptions(mode="fused", fusion_width=width)
+
+    def static_forward(self, q_device):
+        """Run the gates recorded on ``q_device`` as fused blocks of
+        ``wires_per_block`` wires (modules.py:296-316)."""
+        if q_device is None:
+            raise RuntimeError("static_forward: no QuantumDevice among the forward arguments")
+        self.q_device = q_device
+        q_device.flush(self.static_options())
 
     def forward(self, q_device, *args, **kwargs):
         if self.Operator_list is not None:
@@ -86,6 +145,11 @@ class QuantumModule(nn.Module):
         else:
             self.forward(qdev, x)
         return qdev.get_states_1d().T
+
+
+def _is_device(m):
+    from .device import QuantumDevice
+    return isinstance(m, QuantumDevice)
 
 
 class QuantumModuleList(nn.ModuleList, QuantumModule):
