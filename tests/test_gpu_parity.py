@@ -227,3 +227,41 @@ def test_config5_family_parity():
         idx = np.random.default_rng(0).choice(flat.size, 12, replace=False)
         fd = O.central_fd(f, flat.copy(), h=1e-4, idx=idx)[idx]
         assert np.all(np.abs(got[idx] - fd) <= 2e-4 + 1e-3 * np.abs(fd)), (mode, np.abs(got[idx] - fd).max())
+
+
+@gpu
+def test_parameters_updated_between_flushes():
+    """A GPU parameter tensor changed in place (optimizer-style), through a view
+    or by .data assignment between runs is read afresh every time (the host
+    copies of parameter values are reused only while a tensor is unchanged)."""
+    import torchquantum_amd as tq
+    import torchquantum_amd.functional as tqf
+    n = 3
+    base = torch.tensor([0.3, 1.1], device=DEV)
+    w = torch.tensor([[0.7]], device=DEV)
+
+    def run():
+        q = tq.QuantumDevice(n_wires=n, bsz=1, device=DEV)
+        tqf.ry(q, wires=0, params=base[0:1])          # 1-D view: normalised to [1, 1]
+        tqf.rx(q, wires=1, params=base[1:2])
+        tqf.ry(q, wires=2, params=w)
+        tqf.cnot(q, wires=[0, 2])
+        return q.get_states_1d().cpu().numpy()
+
+    def ref():
+        b = base.cpu().numpy().astype(np.float64)
+        ops = [dict(name="ry", wires=[0], params=b[0:1], inverse=False),
+               dict(name="rx", wires=[1], params=b[1:2], inverse=False),
+               dict(name="ry", wires=[2], params=w.cpu().numpy().reshape(-1).astype(np.float64), inverse=False),
+               dict(name="cnot", wires=[0, 2], params=None, inverse=False)]
+        return O.run_ops(ops, n, 1)
+
+    for step in range(4):
+        assert np.abs(run() - ref()).max() < 1e-5, step
+        with torch.no_grad():
+            if step == 0:
+                base.add_(0.25)
+            elif step == 1:
+                w.mul_(-2.0)
+            else:
+                w.data = torch.tensor([[2.9]], device=DEV)

@@ -446,3 +446,26 @@ def test_mg_mode_gradients_emulated():
         g = grad[:, c.param_offsets[ti]:c.param_offsets[ti] + op.n_params()]
         got.append(g.sum(0) if op.params.shape[0] == 1 else g.reshape(-1))
     assert np.all(np.abs(np.concatenate(got) - fd) < 1e-4 * (1 + np.abs(fd)))
+
+
+def test_host_mirror_invalidation():
+    """Host copies of parameter tensors are reused only while the tensor (or
+    the base of a view) is unchanged: in-place writes and .data assignment
+    invalidate them; views of one base are kept apart."""
+    m = E._HostMirror()
+    p = torch.tensor([[0.1, 0.2, 0.3]])
+    m.put(p, np.array([[0.1, 0.2, 0.3]]))
+    assert m.get(p) is not None
+    with torch.no_grad():
+        p.add_(1.0)                          # version bump
+    assert m.get(p) is None
+    m.put(p, p.numpy().copy())
+    p.data = torch.zeros(1, 3)               # new storage
+    assert m.get(p) is None
+    base = torch.arange(6, dtype=torch.float32)
+    v1, v2 = base[:3].unsqueeze(0), base[3:].unsqueeze(0)
+    m.put(v1, v1.numpy().copy())
+    assert m.get(v2) is None                 # another view of the same base
+    assert m.get(base[:3].unsqueeze(0)) is not None   # a fresh view object of the same elements
+    base.mul_(2.0)
+    assert m.get(v1) is None

@@ -160,6 +160,107 @@ __global__ __launch_bounds__(AJ_THREADS) void k_adj_diag(float2* __restrict__ ps
   block_accum(g, P, grad + row * gstride + gslot);
 }
 
+// Bit 0 free of targets and controls: items are float4 (two adjacent
+// amplitudes per access), and each workgroup owns a contiguous tile of
+// 256 * reps items of one row (grid = (tiles per row, bsz)): the streaming
+// shape of the forward per-gate kernels, with one gradient flush per tile.
+template <bool BATCHED, bool NT>
+__global__ __launch_bounds__(AJ_THREADS) void k_adj_1q4(float4* __restrict__ psi, float4* __restrict__ lam, int n,
+                                                       uint64_t items_row, InsertSpec ins4, uint64_t cval4,
+                                                       uint64_t S, const float2* __restrict__ mat, int64_t bstride,
+                                                       int P, float* __restrict__ grad, int64_t gstride, int gslot,
+                                                       int reps) {
+  const uint64_t row = blockIdx.y;
+  const float2* M = BATCHED ? mat + row * bstride : mat;
+  const float2 u0 = M[0], u1 = M[1], u2 = M[2], u3 = M[3];
+  float g[4] = {0.f, 0.f, 0.f, 0.f};
+  const uint64_t rbase = row << (n - 1);
+  uint64_t q = (uint64_t)blockIdx.x * AJ_THREADS * (uint64_t)reps + threadIdx.x;
+  for (int r = 0; r < reps; ++r, q += AJ_THREADS) {
+    if (q >= items_row) break;
+    const uint64_t f0 = rbase + (insert_zeros(q, ins4) | cval4), f1 = f0 | S;
+    const float4 xa = ld4<NT>(psi + f0), xb = ld4<NT>(psi + f1);
+    const float4 la = ld4<NT>(lam + f0), lb = ld4<NT>(lam + f1);
+    const float2 x0[2] = {make_float2(xa.x, xa.y), make_float2(xa.z, xa.w)};
+    const float2 x1[2] = {make_float2(xb.x, xb.y), make_float2(xb.z, xb.w)};
+    const float2 l0[2] = {make_float2(la.x, la.y), make_float2(la.z, la.w)};
+    const float2 l1[2] = {make_float2(lb.x, lb.y), make_float2(lb.z, lb.w)};
+    float2 y0[2], y1[2], m0[2], m1[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      y0[h] = cfmac(u2, x1[h], cmulc(u0, x0[h]));
+      y1[h] = cfmac(u3, x1[h], cmulc(u1, x0[h]));
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k < P) {
+          const float2* D = M + 4 * (k + 1);
+          const float2 v0 = cfma(D[1], y1[h], cmul(D[0], y0[h]));
+          const float2 v1 = cfma(D[3], y1[h], cmul(D[2], y0[h]));
+          g[k] += redot(l0[h], v0) + redot(l1[h], v1);
+        }
+      }
+      m0[h] = cfmac(u2, l1[h], cmulc(u0, l0[h]));
+      m1[h] = cfmac(u3, l1[h], cmulc(u1, l0[h]));
+    }
+    st4<NT>(psi + f0, make_float4(y0[0].x, y0[0].y, y0[1].x, y0[1].y));
+    st4<NT>(psi + f1, make_float4(y1[0].x, y1[0].y, y1[1].x, y1[1].y));
+    st4<NT>(lam + f0, make_float4(m0[0].x, m0[0].y, m0[1].x, m0[1].y));
+    st4<NT>(lam + f1, make_float4(m1[0].x, m1[0].y, m1[1].x, m1[1].y));
+  }
+  block_accum(g, P, grad + row * gstride + gslot);
+}
+
+template <bool BATCHED, bool NT>
+__global__ __launch_bounds__(AJ_THREADS) void k_adj_diag4(float4* __restrict__ psi, float4* __restrict__ lam, int n,
+                                                         uint64_t items_row, InsertSpec ins4, uint64_t cval4,
+                                                         uint64_t pmask, const float2* __restrict__ ph,
+                                                         int64_t bstride, int P, float* __restrict__ grad,
+                                                         int64_t gstride, int gslot, int reps) {
+  const uint64_t row = blockIdx.y;
+  const float2* M = BATCHED ? ph + row * bstride : ph;
+  const float2 p0 = M[0], p1 = M[1];
+  float g[4] = {0.f, 0.f, 0.f, 0.f};
+  const uint64_t rbase = row << (n - 1);
+  uint64_t q = (uint64_t)blockIdx.x * AJ_THREADS * (uint64_t)reps + threadIdx.x;
+  for (int r = 0; r < reps; ++r, q += AJ_THREADS) {
+    if (q >= items_row) break;
+    const uint64_t j4 = insert_zeros(q, ins4) | cval4;  // in-row float4 index
+    const float4 xv = ld4<NT>(psi + rbase + j4), lv = ld4<NT>(lam + rbase + j4);
+    const float2 x[2] = {make_float2(xv.x, xv.y), make_float2(xv.z, xv.w)};
+    const float2 l[2] = {make_float2(lv.x, lv.y), make_float2(lv.z, lv.w)};
+    float2 y[2], m[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int par = __popcll((2 * j4 + h) & pmask) & 1;
+      const float2 f = par ? p1 : p0;
+      y[h] = cmulc(f, x[h]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k < P) {
+          const float2 df = par ? M[2 * (k + 1) + 1] : M[2 * (k + 1)];
+          g[k] += redot(l[h], cmul(df, y[h]));
+        }
+      }
+      m[h] = cmulc(f, l[h]);
+    }
+    st4<NT>(psi + rbase + j4, make_float4(y[0].x, y[0].y, y[1].x, y[1].y));
+    st4<NT>(lam + rbase + j4, make_float4(m[0].x, m[0].y, m[1].x, m[1].y));
+  }
+  block_accum(g, P, grad + row * gstride + gslot);
+}
+
+// tiles per row for the float4 adjoint kernels: 4 items per thread, more when
+// the grid would pass 2^22 workgroups per row
+static dim3 adj_tiles(uint64_t items_row, int64_t bsz, int& reps) {
+  reps = 4;
+  uint64_t t = (items_row + (uint64_t)AJ_THREADS * reps - 1) / ((uint64_t)AJ_THREADS * reps);
+  while (t > (1ull << 22)) {
+    reps *= 2;
+    t = (items_row + (uint64_t)AJ_THREADS * reps - 1) / ((uint64_t)AJ_THREADS * reps);
+  }
+  return dim3((unsigned)(t ? t : 1), (unsigned)bsz, 1);
+}
+
 static int build_insert_adj(InsertSpec& s, uint64_t mask, int n) {
   s.nb = 0;
   for (int p = 0; p < n; ++p)
@@ -199,8 +300,24 @@ TQSV_API int tqsv_adjoint_1q(hipStream_t s, void* psi, void* lam, int64_t bsz, i
   const uint64_t tmask = 1ull << tbit;
   if ((cmask & tmask) || (n < 64 && (cmask >> n)) || (cval & ~cmask)) { set_error("adjoint_1q: bad control mask"); return E_ARG; }
   InsertSpec ins;
+  const int c = __builtin_popcountll(cmask);
+  if (tbit >= 1 && !(cmask & 1ull)) {
+    if (!build_insert_adj(ins, (cmask | tmask) >> 1, n - 1)) { set_error("adjoint_1q: too many controls"); return E_LIMIT; }
+    const uint64_t items_row = 1ull << (n - 2 - c);
+    int reps;
+    const dim3 grid = adj_tiles(items_row, bsz, reps);
+    const bool nt = stream_nt(2 * bsz, n);
+    const uint64_t S = 1ull << (tbit - 1);
+#define ADJ1Q4(B, NTV)                                                                                        \
+  k_adj_1q4<B, NTV><<<grid, AJ_THREADS, 0, s>>>((float4*)psi, (float4*)lam, n, items_row, ins, cval >> 1, S, \
+                                                (const float2*)mat, mat_bstride, P, grad, grad_stride, gslot, reps)
+    if (mat_bstride) { if (nt) ADJ1Q4(true, true); else ADJ1Q4(true, false); }
+    else { if (nt) ADJ1Q4(false, true); else ADJ1Q4(false, false); }
+#undef ADJ1Q4
+    return check_launch("tqsv_adjoint_1q");
+  }
   if (!build_insert_adj(ins, cmask | tmask, n)) { set_error("adjoint_1q: too many controls"); return E_LIMIT; }
-  const uint64_t items_row = 1ull << (n - 1 - __builtin_popcountll(cmask));
+  const uint64_t items_row = 1ull << (n - 1 - c);
   const dim3 grid = adj_grid(items_row, bsz);
   if (mat_bstride)
     k_adj_1q<true><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, tmask,
@@ -244,8 +361,24 @@ TQSV_API int tqsv_adjoint_diag(hipStream_t s, void* psi, void* lam, int64_t bsz,
   if (rc) return rc;
   if ((n < 64 && ((cmask | pmask) >> n)) || (cval & ~cmask) || (cmask & pmask)) { set_error("adjoint_diag: bad masks"); return E_ARG; }
   InsertSpec ins;
+  const int c = __builtin_popcountll(cmask);
+  if (!(cmask & 1ull)) {
+    if (!build_insert_adj(ins, cmask >> 1, n - 1)) { set_error("adjoint_diag: too many controls"); return E_LIMIT; }
+    const uint64_t items_row = 1ull << (n - 1 - c);
+    int reps;
+    const dim3 grid = adj_tiles(items_row, bsz, reps);
+    const bool nt = stream_nt(2 * bsz, n);
+#define ADJD4(B, NTV)                                                                                           \
+  k_adj_diag4<B, NTV><<<grid, AJ_THREADS, 0, s>>>((float4*)psi, (float4*)lam, n, items_row, ins, cval >> 1,    \
+                                                  pmask, (const float2*)phases, ph_bstride, P, grad, grad_stride, \
+                                                  gslot, reps)
+    if (ph_bstride) { if (nt) ADJD4(true, true); else ADJD4(true, false); }
+    else { if (nt) ADJD4(false, true); else ADJD4(false, false); }
+#undef ADJD4
+    return check_launch("tqsv_adjoint_diag");
+  }
   if (!build_insert_adj(ins, cmask, n)) { set_error("adjoint_diag: too many controls"); return E_LIMIT; }
-  const uint64_t items_row = 1ull << (n - __builtin_popcountll(cmask));
+  const uint64_t items_row = 1ull << (n - c);
   const dim3 grid = adj_grid(items_row, bsz);
   if (ph_bstride)
     k_adj_diag<true><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, pmask,

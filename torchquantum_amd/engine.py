@@ -392,9 +392,60 @@ class GateValues:
                     self.gkey[i] = key
 
 
+class _HostMirror:
+    """Host copies of GPU parameter tensors, reused while a tensor is unchanged.
+
+    Gate matrices are evaluated on the host, so GPU-resident parameters are
+    copied to the host once per flush.  That copy is ordered after all work
+    already queued on the stream and would stall the host until the GPU
+    drains.  A copy is reused while torch's version counter and the data
+    pointer of the tensor (or of the base of a view) are unchanged: every
+    in-place write (optimizer steps, copy_, .data assignment) invalidates it.
+    Entries die with their tensors (weak keys)."""
+
+    def __init__(self):
+        self._d: Dict[int, tuple] = {}      # id(base) -> (weakref(base), stamp, {view: array})
+        # reentrant: a weakref callback (_drop) can run from a collection
+        # triggered while this thread holds the lock
+        self._lock = threading.RLock()
+
+    @staticmethod
+    def _key(p: torch.Tensor):
+        base = p._base if p._base is not None else p
+        view = (p.storage_offset(), tuple(p.shape), tuple(p.stride()), p.dtype)
+        return base, view, (base._version, base.data_ptr())
+
+    def _drop(self, key: int):
+        with self._lock:
+            self._d.pop(key, None)
+
+    def get(self, p: torch.Tensor) -> Optional[np.ndarray]:
+        base, view, stamp = self._key(p)
+        with self._lock:
+            e = self._d.get(id(base))
+            if e is None or e[0]() is not base or e[1] != stamp:
+                return None
+            return e[2].get(view)
+
+    def put(self, p: torch.Tensor, value: np.ndarray):
+        import weakref
+        base, view, stamp = self._key(p)
+        key = id(base)
+        with self._lock:
+            e = self._d.get(key)
+            if e is None or e[0]() is not base or e[1] != stamp:
+                e = (weakref.ref(base, lambda _r, k=key: self._drop(k)), stamp, {})
+                self._d[key] = e
+            e[2][view] = value
+
+
+_MIRROR = _HostMirror()
+
+
 def _host_params(tape: Sequence[TapeOp]) -> List[Optional[np.ndarray]]:
-    """Parameter values of every tape op as numpy (float64 or complex128);
-    tensors on a GPU are gathered with one concatenation and one copy."""
+    """Parameter values of every tape op as numpy (float64 or complex128).
+    GPU tensors without a valid host copy are gathered with one
+    concatenation and one device-to-host copy."""
     out: List[Optional[np.ndarray]] = [None] * len(tape)
     dev: Dict[tuple, List[int]] = {}
     for i, op in enumerate(tape):
@@ -403,6 +454,10 @@ def _host_params(tape: Sequence[TapeOp]) -> List[Optional[np.ndarray]]:
         p = op.params.detach()
         if p.device.type == "cpu":
             out[i] = p.numpy().astype(np.complex128 if p.is_complex() else np.float64)
+            continue
+        hit = _MIRROR.get(op.params)
+        if hit is not None:
+            out[i] = hit
         else:
             dev.setdefault((p.device, p.dtype), []).append(i)
     for (d, dt), idx in dev.items():
@@ -413,6 +468,8 @@ def _host_params(tape: Sequence[TapeOp]) -> List[Optional[np.ndarray]]:
             sh = tuple(tape[i].params.shape)
             sz = int(np.prod(sh))
             out[i] = flat[off:off + sz].reshape(sh)
+            out[i].flags.writeable = False
+            _MIRROR.put(tape[i].params, out[i])
             off += sz
     return out
 
@@ -904,8 +961,13 @@ class Compiled:
 
     # ---------------------------------------------------------------- values
     def pool(self, tape: Sequence[TapeOp], with_grad: bool, with_adj: bool) -> torch.Tensor:
+        """The matrix pool on the device: staged through pinned host memory and
+        copied asynchronously on the current stream (no wait for queued work)."""
         host = self.pool_host(tape, with_grad, with_adj)
-        return torch.from_numpy(host).to(self.device, non_blocking=False)
+        t = torch.from_numpy(host)
+        if torch.device(self.device).type != "cuda":
+            return t.to(self.device)
+        return t.pin_memory().to(self.device, non_blocking=True)
 
     def pool_host(self, tape: Sequence[TapeOp], with_grad: bool, with_adj: bool) -> np.ndarray:
         """The matrix pool (complex64) for the current parameter values."""

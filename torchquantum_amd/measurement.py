@@ -47,9 +47,10 @@ def expval(qdev, wires: Union[int, List[int]], observables):
         for rotation in obs.diagonalizing_gates():
             rotation(qdev, wires=wire)
     z = _z_all(qdev)
-    evs = [obs.eigvals.real.to(torch.float32).to(z.device) for obs in observables]
+    ev_host = [obs.eigvals.real.detach().to("cpu", torch.float32) for obs in observables]
+    evs = [e.to(z.device) for e in ev_host]
     norm = None
-    if any(float(e[0] + e[1]) != 0.0 for e in evs):
+    if any(float(e[0] + e[1]) != 0.0 for e in ev_host):
         # marginals sum to the squared norm, which need not be 1 (set_states, cu2, reset...)
         norm = expval_pauli_strings(qdev.get_states_1d(), qdev.n_wires, ["I" * qdev.n_wires])[:, 0]
     out = []
