@@ -31,6 +31,7 @@ import numpy as np
 import torch
 
 HBM_PEAK_GBS = 8000.0
+MFMA_FP32_PEAK_TFLOPS = 157.3
 
 
 def log(*a):
@@ -132,6 +133,44 @@ def kernel_roofline(n, bsz, iters=50):
     ms = e0.elapsed_time(e1) / iters
     nbytes = 16.0 * bsz * (1 << n)
     return ms, nbytes
+
+
+def dense_block_bench(n=26, bsz=16, iters=6):
+    """Dense k-qubit blocks (k = 5, 6) on an 8 GiB state: the matrix-core
+    kernel (one matrix for all rows) against the VALU kernel (the same matrix
+    given per row, which takes the LDS/VALU path).  Real FLOPs per amplitude:
+    8 * 2^k (a 2^(k+1) x 2^(k+1) real GEMM per group of 2^k amplitudes)."""
+    from torchquantum_amd import _native as N
+    st = torch.zeros((bsz, 1 << n), dtype=torch.complex64, device="cuda")
+    st[:, 0] = 1
+    stream = N.stream_of(st)
+    out = {}
+    rng = np.random.default_rng(0)
+    for k in (5, 6):
+        z = rng.normal(size=(2 ** k, 2 ** k)) + 1j * rng.normal(size=(2 ** k, 2 ** k))
+        u, _ = np.linalg.qr(z)
+        m1 = torch.tensor(u, dtype=torch.complex64, device="cuda")
+        mb = m1.unsqueeze(0).expand(bsz, -1, -1).contiguous()
+        bits = N.int_array([n - 1 - w for w in range(3, 3 + k)])      # wires 3..3+k-1
+        res = {}
+        for name, mat, bstride in (("mfma", m1, 0), ("valu", mb, (1 << k) ** 2)):
+            for _ in range(2):
+                N.call("tqsv_apply_kq", stream, st.data_ptr(), bsz, n, k, bits, 0, 0, mat.data_ptr(), bstride)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(iters):
+                N.call("tqsv_apply_kq", stream, st.data_ptr(), bsz, n, k, bits, 0, 0, mat.data_ptr(), bstride)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / iters
+            amps = float(bsz) * (1 << n)
+            res[name] = {"ms": ms, "tflops": 8 * (2 ** k) * amps / (ms * 1e-3) / 1e12,
+                         "gbs": 16 * amps / (ms * 1e-3) / 1e9}
+        res["mfma_frac_of_fp32_peak"] = res["mfma"]["tflops"] / MFMA_FP32_PEAK_TFLOPS
+        res["speedup_mfma_vs_valu"] = res["valu"]["ms"] / res["mfma"]["ms"]
+        out[f"k{k}"] = res
+    return {"workload": f"{n} qubits, bsz {bsz}, dense 2^k x 2^k block on wires 3..3+k-1", **out}
 
 
 def time_batch(n_circ, steps, tq):
@@ -275,8 +314,8 @@ def cpu_baseline(ops_sample, n, bsz, n_gates_total_per_step):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--adj-qubits", type=int, default=24)
     ap.add_argument("--adj-bsz", type=int, default=64)
     ap.add_argument("--adj-steps", type=int, default=2)
@@ -286,6 +325,7 @@ def main():
     ap.add_argument("--only-roofline", action="store_true",
                     help="run only the dominant kernel's timing loops (for rocprofv3 summaries)")
     ap.add_argument("--config5", action="store_true", help="run the 30-qubit workload only")
+    ap.add_argument("--only-dense", action="store_true", help="run only the dense k = 5, 6 block timings")
     ap.add_argument("--c5-qubits", type=int, default=30)
     ap.add_argument("--c5-bsz", type=int, default=4)
     ap.add_argument("--cpu-layers", type=int, default=2)
@@ -307,6 +347,9 @@ def main():
         print(json.dumps({"copy_bytes": cb, "copy_ms": cms, "k_1q_20q_bsz16_ms": ms,
                           "k_1q_20q_bsz16_GBs": nbytes / ms / 1e6, "k_1q_26q_bsz16_ms": ms_big,
                           "k_1q_26q_bsz16_GBs": nbytes_big / ms_big / 1e6}), flush=True)
+        return
+    if args.only_dense:
+        print(json.dumps({"dense_block": dense_block_bench()}), flush=True)
         return
     if args.config5:
         print(json.dumps({"config5": config5(args, tq)}), flush=True)
@@ -331,6 +374,8 @@ def main():
     achieved_big = nbytes_big / (ms_big * 1e-3) / 1e9
 
     extra = {}
+    log("[bench] dense k = 5, 6 blocks")
+    extra["dense_block"] = dense_block_bench()
     if not args.skip_adjoint:
         an, ab = args.adj_qubits, args.adj_bsz
         log(f"[bench] adjoint gradient fused n={an} bsz={ab}")

@@ -60,7 +60,9 @@ int tqsv_apply_diag(tqsv_stream_t s, void* state, int64_t bsz, int n, uint64_t c
 int tqsv_apply_swap(tqsv_stream_t s, void* state, int64_t bsz, int n, int bit_a, int bit_b, uint64_t cmask,
                     uint64_t cval);
 /* dense 2^k x 2^k (k <= 10) on host-listed bits (bits[0] = matrix MSB):
- * qubitunitary / qubitunitaryfast (functional/qubit_unitary.py:19-69), QFT, RCCX ... */
+ * qubitunitary / qubitunitaryfast (functional/qubit_unitary.py:19-69), QFT, RCCX,
+ * fused blocks (graph/graphs.py:836-860).  k = 5, 6 with one matrix for all
+ * rows (mat_bstride = 0) and >= 64 groups per row run on the matrix cores. */
 int tqsv_apply_kq(tqsv_stream_t s, void* state, int64_t bsz, int n, int k, const int* bits, uint64_t cmask,
                   uint64_t cval, const void* mat, int64_t mat_bstride);
 /* A run of the calls above from one host array of gate records

@@ -265,3 +265,72 @@ def test_parameters_updated_between_flushes():
                 w.mul_(-2.0)
             else:
                 w.data = torch.tensor([[2.9]], device=DEV)
+
+
+def _dense_apply_np(state, U, bits, cmask, cval, n):
+    """numpy complex128: U on bit positions ``bits`` (bits[0] = matrix MSB)
+    where (index & cmask) == cval, for every row of ``state``."""
+    k = len(bits)
+    tm = sum(1 << b for b in bits)
+    j = np.arange(1 << n)
+    base = j[((j & tm) == 0) & ((j & cmask) == cval)]
+    offs = np.array([sum(1 << bits[q] for q in range(k) if (r >> (k - 1 - q)) & 1) for r in range(1 << k)])
+    idx = base[:, None] + offs[None, :]
+    out = state.copy()
+    for b in range(state.shape[0]):
+        out[b][idx] = state[b][idx] @ U.T
+    return out
+
+
+@gpu
+@pytest.mark.parametrize("k", [5, 6])
+@pytest.mark.parametrize("nctrl", [0, 1, 2])
+def test_dense_kq_matrix_cores(k, nctrl):
+    """k = 5, 6 dense gates with one matrix for all rows and >= 64 groups per
+    row take the matrix-core kernel: unsorted target bits (low bits
+    included), 0-2 controls, through the C ABI; vs numpy complex128, 1e-5."""
+    from torchquantum_amd import _native as N
+    rng = np.random.default_rng(100 * k + nctrl)
+    n, bsz = 14, 2
+    for trial in range(3):
+        bits = [int(b) for b in rng.permutation(n)[:k]]
+        if trial == 0:
+            bits = [int(b) for b in rng.permutation(k)]          # the k lowest bits, shuffled
+        rest = [b for b in range(n) if b not in bits]
+        cb = [int(b) for b in rng.choice(rest, nctrl, replace=False)]
+        cmask = sum(1 << b for b in cb)
+        cval = sum(1 << b for b in cb if rng.random() < 0.5)
+        z = rng.normal(size=(2 ** k, 2 ** k)) + 1j * rng.normal(size=(2 ** k, 2 ** k))
+        U, _ = np.linalg.qr(z)
+        init = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+        init /= np.linalg.norm(init, axis=1, keepdims=True)
+        st = torch.tensor(init, dtype=torch.complex64, device=DEV)
+        m = torch.tensor(U, dtype=torch.complex64, device=DEV)
+        N.call("tqsv_apply_kq", N.stream_of(st), st.data_ptr(), bsz, n, k, N.int_array(bits), cmask, cval,
+               m.data_ptr(), 0)
+        ref = _dense_apply_np(init, U, bits, cmask, cval, n)
+        assert np.abs(st.cpu().numpy() - ref).max() < 1e-5, (k, nctrl, trial, bits)
+
+
+@gpu
+@pytest.mark.parametrize("width", [5, 6])
+def test_wide_fusion_blocks(width):
+    """Fusion width 5 / 6: the 5- and 6-wire blocks run on the matrix-core
+    kernel between stages (shared parameters) or the VALU kernel (per-row
+    parameters); states vs the oracle, 1e-5, and a gradient through the
+    trainable (<= 2-wire) blocks vs FD."""
+    import torchquantum_amd as tq
+    rng = np.random.default_rng(40 + width)
+    n, bsz = 12, 2
+    names = ["h", "rx", "ry", "rz", "u3", "cnot", "cz", "swap", "crx", "rzz", "toffoli", "s", "t", "sx", "cu3"]
+    for batched_frac in (0.0, 0.5):
+        ops = random_ops(n, 160, bsz, rng, names=names, batched_frac=batched_frac)
+        got = _run(n, bsz, ops, "fused", fusion_width=width)
+        ref = O.run_ops(oracle_ops(ops, bsz), n, bsz)
+        assert np.abs(got - ref).max() < 1e-5, (width, batched_frac)
+        from torchquantum_amd import engine as E
+        q = _qdev(n, bsz, "fused", fusion_width=width)
+        apply_ops_tq(q, ops, DEV)
+        c = E.Compiled(q._tape, n, bsz, q.engine_options, "cpu")       # the plan, built on the host only
+        wide = [k for k in c.kops if k.kind == "matk" and len(k.bits) > 4]
+        assert wide and (batched_frac > 0 or any(not k.batched for k in wide))

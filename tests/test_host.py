@@ -320,18 +320,23 @@ def test_refpath_matches_oracle():
     assert np.abs(got - ref).max() < 1e-5
 
 
-@pytest.mark.parametrize("width", [2, 3, 4])
-@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("width", [2, 3, 4, 5, 6])
+@pytest.mark.parametrize("seed", range(6))
 def test_block_fusion_emulated(width, seed):
     """Dense block fusion (fusion width k): same states as the oracle; blocks
-    never exceed k bits; trainable blocks keep gradients exact (k = 2)."""
+    never exceed k bits; blocks wider than the stage kernel's dense ops
+    (5, 6 wires) run as per-gate launches between stages."""
     rng = np.random.default_rng(300 + seed)
-    n, bsz = 6, 2
-    ops = random_ops(n, 50, bsz, rng)
+    n, bsz = (6 if width <= 4 else 8), 2
+    ops = random_ops(n, 50 if width <= 4 else 120, bsz, rng)
     tape = _tape_from_ops(ops)
     c = E.Compiled(tape, n, bsz, E.EngineOptions(fusion_width=width, stage_bits=5), "cpu")
     assert all(len(k.allbits) <= max(width, 5) for k in c.kops)
     assert len(c.kops) < len(tape)
+    if width > 4:
+        wide = [i for i, k in enumerate(c.kops) if k.kind == "matk" and len(k.bits) > 4]
+        assert wide, "expected a block wider than 4 wires"
+        assert all(("solo", i) in c.steps for i in wide)
     pool = c.pool_host(tape, False, True).astype(np.complex128)
     got = EM.run_compiled(c, pool, None, init_zero=True)
     assert np.abs(got - O.run_ops(oracle_ops(ops, bsz), n, bsz)).max() < 5e-6

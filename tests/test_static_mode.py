@@ -44,7 +44,7 @@ def test_static_on_propagates():
     assert all(o.static_mode for o in m.rd.op_list)
     assert m.static_options().mode == "fused" and m.static_options().fusion_width == 3
     m.static_on(wires_per_block=9)
-    assert m.static_options().fusion_width == 4        # widest block the planner builds
+    assert m.static_options().fusion_width == 6        # widest block the planner builds
     m.static_off()
     assert not m.static_mode and not m.rd.static_mode
     with pytest.raises(ValueError):

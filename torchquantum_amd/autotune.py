@@ -22,7 +22,8 @@ import torch
 from . import engine as E
 from .costmodel import MLPCostModel, plan_features
 
-DEFAULT_CANDIDATES = [E.EngineOptions(fusion_width=k, stage_bits=m) for k in (1, 2, 3, 4) for m in (10, 12, 13)]
+DEFAULT_CANDIDATES = ([E.EngineOptions(fusion_width=k, stage_bits=m) for k in (1, 2, 3, 4) for m in (10, 12, 13)]
+                      + [E.EngineOptions(fusion_width=k, stage_bits=12) for k in (5, 6)])
 
 
 def circuit_key(tape: Sequence[E.TapeOp], n: int, bsz: int) -> str:

@@ -388,6 +388,142 @@ __global__ __launch_bounds__(256) void k_kq(float2* __restrict__ st, uint64_t n_
   }
 }
 
+// ---------------------------------------------------------------------------
+// dense k-qubit gate on the matrix cores, k = 5, 6 (one matrix for all rows).
+// Out = U In over groups of d = 2^k amplitudes is the real GEMM
+//     [Or; Oi] = [[Ur, -Ui], [Ui, Ur]] [Ir; Ii]     (2d x 2d) x (2d x 64)
+// for a tile of 64 groups, on v_mfma_f32_16x16x4f32.  A workgroup (4 waves)
+// keeps its rows of the real matrix as MFMA A-fragments in registers for the
+// whole launch, stages each tile through LDS as [2d][64] floats (re rows,
+// then im rows), and streams tiles: 4 per workgroup.  The tile's amplitudes
+// are loaded and stored in address order (the tile spans the k target bits
+// and the 6 lowest free bits, so its offsets from the tile base are the same
+// for every tile and are computed once per thread).
+// ---------------------------------------------------------------------------
+struct MSpec {
+  int k;
+  int bits[6];   // target bits, bits[0] = matrix MSB
+  int sp[12];    // target bits and the 6 tile-column bits, ascending
+  int jpos[6];   // the 6 tile-column bits (lowest free bits), ascending
+};
+
+constexpr int MQ_RS = 80;      // LDS row stride in floats (64 columns + 16 pad: rows 2 apart hit other banks)
+constexpr int MQ_TILES = 4;    // tiles per workgroup
+
+typedef float mq_v4 __attribute__((ext_vector_type(4)));
+
+template <int K, bool PAIR>
+__global__ __launch_bounds__(256) void k_kq_mfma(float2* __restrict__ st, uint64_t n_tiles, InsertSpec ins,
+                                                uint64_t cval, MSpec ms, const float2* __restrict__ U) {
+  constexpr int D = 1 << K;        // amplitudes per group
+  constexpr int M2 = 2 * D;        // real rows / cols
+  constexpr int MT = D / 32;       // 16-row m-tiles per wave
+  constexpr int KS = M2 / 4;       // k-steps
+  constexpr int EPT = D / 4;       // tile elements per thread (64 * D / 256)
+  constexpr int NL = PAIR ? EPT / 2 : EPT;  // global accesses per thread (float4 when PAIR)
+  __shared__ float X[M2 * MQ_RS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // A fragments: W[16 * (wave * MT + mt) + lane % 16][4 * ks + lane / 16]
+  float wf[MT][KS];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int i = 16 * (wave * MT + mt) + (lane & 15);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int kk = 4 * ks + (lane >> 4);
+      const float2 u = U[(i % D) * D + (kk % D)];
+      const bool ri = i >= D, rk = kk >= D;
+      wf[mt][ks] = (ri == rk) ? u.x : (ri ? u.y : -u.y);
+    }
+  }
+  // this thread's tile elements (element index e: bit b at tile position
+  // sp[b]); PAIR: elements 2 p and 2 p + 1 are adjacent amplitudes (sp[0] = 0)
+  uint32_t rel[NL];
+  int lds[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int idx = PAIR ? 2 * (tid + 256 * (e >> 1)) + (e & 1) : tid + 256 * e;
+    uint32_t r = 0;
+#pragma unroll
+    for (int b = 0; b < K + 6; ++b) r |= (uint32_t)((idx >> b) & 1) << ms.sp[b];
+    int c = 0, j = 0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) c |= (int)((r >> ms.bits[q]) & 1u) << (K - 1 - q);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) j |= (int)((r >> ms.jpos[q]) & 1u) << q;
+    if (!PAIR || !(e & 1)) rel[PAIR ? e >> 1 : e] = r;
+    lds[e] = c * MQ_RS + j;
+  }
+  float2 v[EPT];
+  auto load_tile = [&](uint64_t tile) {
+    const uint64_t base = insert_zeros(tile << 6, ins) | cval;
+    if constexpr (PAIR) {
+#pragma unroll
+      for (int q = 0; q < NL; ++q) {
+        const float4 a = *reinterpret_cast<const float4*>(st + base + rel[q]);
+        v[2 * q] = make_float2(a.x, a.y);
+        v[2 * q + 1] = make_float2(a.z, a.w);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NL; ++q) v[q] = st[base + rel[q]];
+    }
+  };
+  uint64_t tile = (uint64_t)blockIdx.x * MQ_TILES;
+  if (tile >= n_tiles) return;
+  load_tile(tile);
+  for (int t = 0; t < MQ_TILES; ++t, ++tile) {
+    if (tile >= n_tiles) return;  // uniform per workgroup
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      X[lds[e]] = v[e].x;
+      X[D * MQ_RS + lds[e]] = v[e].y;
+    }
+    __syncthreads();
+    // the next tile's loads are in flight while the matrix cores run
+    if (t + 1 < MQ_TILES && tile + 1 < n_tiles) load_tile(tile + 1);
+    mq_v4 acc[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = (mq_v4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      float bf[4];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) bf[nt] = X[(4 * ks + (lane >> 4)) * MQ_RS + 16 * nt + (lane & 15)];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(wf[mt][ks], bf[nt], acc[mt][nt], 0, 0, 0);
+    }
+    __syncthreads();
+    // D[i][j]: i = 16 m-tile + 4 (lane / 16) + v, j = 16 n-tile + lane % 16
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          X[(16 * (wave * MT + mt) + 4 * (lane >> 4) + r) * MQ_RS + 16 * nt + (lane & 15)] = acc[mt][nt][r];
+    __syncthreads();
+    const uint64_t base = insert_zeros(tile << 6, ins) | cval;
+    if constexpr (PAIR) {
+#pragma unroll
+      for (int q = 0; q < NL; ++q) {
+        const int l0 = lds[2 * q], l1 = lds[2 * q + 1];
+        *reinterpret_cast<float4*>(st + base + rel[q]) =
+            make_float4(X[l0], X[D * MQ_RS + l0], X[l1], X[D * MQ_RS + l1]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NL; ++q) st[base + rel[q]] = make_float2(X[lds[q]], X[D * MQ_RS + lds[q]]);
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace tqsv
 
 using namespace tqsv;
@@ -623,6 +759,32 @@ TQSV_API int tqsv_apply_kq(hipStream_t s, void* state, int64_t bsz, int n, int k
   }
   InsertSpec ins;
   if (!build_insert(ins, cmask | tm, n)) { set_error("too many target+control bits (max 12)"); return E_LIMIT; }
+  const int cc = __builtin_popcountll(cmask);
+  if ((k == 5 || k == 6) && mat_bstride == 0 && n - k - cc >= 6) {
+    // matrix cores: tiles of 64 groups whose columns are the 6 lowest free bits
+    MSpec ms;
+    ms.k = k;
+    for (int q = 0; q < k; ++q) ms.bits[q] = bits[q];
+    int nj = 0;
+    for (int p = 0; p < n && nj < 6; ++p)
+      if (!(((cmask | tm) >> p) & 1ull)) ms.jpos[nj++] = p;
+    int ns = 0;
+    for (int p = 0; p < n; ++p)
+      if (((tm >> p) & 1ull) || (nj && p <= ms.jpos[5] && [&] { for (int q = 0; q < 6; ++q) if (ms.jpos[q] == p) return true; return false; }()))
+        ms.sp[ns++] = p;
+    const uint64_t n_tiles = (uint64_t)bsz << (n - k - cc - 6);
+    const unsigned grid = (unsigned)((n_tiles + MQ_TILES - 1) / MQ_TILES);
+    // bit 0 in the tile: element pairs are adjacent amplitudes (16-byte accesses)
+    const bool pair = ms.sp[0] == 0;
+    if (k == 5) {
+      if (pair) k_kq_mfma<5, true><<<grid, 256, 0, s>>>((float2*)state, n_tiles, ins, cval, ms, (const float2*)mat);
+      else k_kq_mfma<5, false><<<grid, 256, 0, s>>>((float2*)state, n_tiles, ins, cval, ms, (const float2*)mat);
+    } else {
+      if (pair) k_kq_mfma<6, true><<<grid, 256, 0, s>>>((float2*)state, n_tiles, ins, cval, ms, (const float2*)mat);
+      else k_kq_mfma<6, false><<<grid, 256, 0, s>>>((float2*)state, n_tiles, ins, cval, ms, (const float2*)mat);
+    }
+    return check_launch("tqsv_apply_kq");
+  }
   const int d = 1 << k;
   const int lg_groups_row = n - k - __builtin_popcountll(cmask);
   const uint64_t n_groups = (uint64_t)bsz << lg_groups_row;

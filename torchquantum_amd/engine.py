@@ -34,6 +34,7 @@ from . import gates as G
 from . import _native as N
 
 R_BITS = 4            # register bits per thread in the stage kernel
+MAX_FUSION_WIDTH = 6  # widest fused block: 5- and 6-wire blocks run on the matrix-core kernel between stages
 MAX_STAGE_BITS = 13   # LDS: 2^13 * 8 B = 64 KiB per workgroup
 MAX_GP_STAGE = 1024
 MAX_P_PER_OP = 4
@@ -58,7 +59,9 @@ class EngineOptions:
     low_bits:    lowest bit positions always kept local (coalescing).
     merge_1q:    multiply runs of single-wire gates into one 2x2 (fused mode).
     fusion_width: k; >= 2 pre-multiplies runs of gates acting within k wires
-                 into one dense 2^k x 2^k block (k <= 4; trainable blocks k <= 2).
+                 into one dense 2^k x 2^k block (k <= 6; trainable blocks k <= 2).
+                 Blocks of <= 4 wires run inside stages; 5- and 6-wire blocks
+                 run as their own launches on the matrix cores.
     grid_cap:    workgroups per stage launch (0 = auto).
     adjoint_mg:  adjoint sweep accumulates 2x2 matrix gradients for ops with
                  >= 2 trainable parameters (else one inner product per parameter).
@@ -248,42 +251,58 @@ def merge_single_wire(kops: List[KOp], tape: Sequence[TapeOp]) -> List[KOp]:
 def fuse_blocks(kops: List[KOp], tape: Sequence[TapeOp], n: int, width: int) -> List[KOp]:
     """Greedy gate fusion into dense blocks of <= ``width`` bits.
 
-    An op joins the block that last touched its bits when exactly one block
-    did (so no op in between shares a bit with it and it commutes forward)
-    and the union stays within ``width`` bits.  Multi-member blocks become
-    one dense op (mat1 / mat2 / matk) whose matrix is the ordered product of
-    its members embedded in the block's bits.  Blocks with trainable members
-    are kept to <= 2 bits and <= MAX_P_PER_OP parameters (the widths whose
-    kernels return parameter gradients)."""
-    width = max(1, min(width, R_BITS))
-    blocks: List[dict] = []
+    Walking the ops in order, an op joins the blocks that last touched its
+    bits.  Those blocks are merged with it into one block when each of them is
+    still open (no other block has touched any of its bits since) and the
+    union stays within ``width`` bits: the merged block then sits at the op's
+    position, which is legal because nothing after the merged blocks acts on
+    their bits.  Multi-member blocks become one dense op (mat1 / mat2 / matk)
+    whose matrix is the ordered product of its members embedded in the
+    block's bits.  Blocks with trainable members are kept to <= 2 bits and <=
+    MAX_P_PER_OP parameters (the widths whose kernels return parameter
+    gradients).  Blocks of 5 or 6 bits run on the matrix-core kernel."""
+    width = max(1, min(width, MAX_FUSION_WIDTH))
+    blocks: List[Optional[dict]] = []
     last: Dict[int, int] = {}
+
+    def place(B: dict):
+        # B follows every block last seen on its bits: those are closed (a
+        # later block acts on one of their bits, so they can no longer move)
+        for b in B["bits"]:
+            j = last.get(b)
+            if j is not None and blocks[j] is not None:
+                blocks[j]["closed"] = True
+        blocks.append(B)
+        for b in B["bits"]:
+            last[b] = len(blocks) - 1
+
     for i, k in enumerate(kops):
         bs = set(k.allbits)
         if len(k.src) != 1 or (k.kind == "matk" and len(k.bits) > R_BITS) or len(bs) > width:
-            blocks.append(dict(bits=bs, members=[i], solo=True, P=k.P))
-            for b in bs:
-                last[b] = len(blocks) - 1
+            place(dict(bits=bs, members=[i], solo=True, P=k.P, closed=False))
             continue
-        cands = {last[b] for b in bs if b in last}
-        if len(cands) == 1:
-            j = cands.pop()
-            B = blocks[j]
-            nb = B["bits"] | bs
-            P = B["P"] + k.P
+        cands = sorted({last[b] for b in bs if b in last})
+        cb = [blocks[j] for j in cands]
+        if cands and not any(B["solo"] or B["closed"] for B in cb):
+            nb = set(bs)
+            P = k.P
+            for B in cb:
+                nb |= B["bits"]
+                P += B["P"]
             lim = 2 if P > 0 else width
-            if not B["solo"] and len(nb) <= min(width, lim) and P <= MAX_P_PER_OP:
-                B["bits"] = nb
-                B["members"].append(i)
-                B["P"] = P
-                for b in bs:
-                    last[b] = j
+            if len(nb) <= min(width, lim) and P <= MAX_P_PER_OP:
+                # the open candidates move to this op's position: nothing after
+                # them acts on their bits
+                members = sorted([m for B in cb for m in B["members"]] + [i])
+                for j in cands:
+                    blocks[j] = None
+                place(dict(bits=nb, members=members, solo=False, P=P, closed=False))
                 continue
-        blocks.append(dict(bits=bs, members=[i], solo=False, P=k.P))
-        for b in bs:
-            last[b] = len(blocks) - 1
+        place(dict(bits=bs, members=[i], solo=False, P=k.P, closed=False))
     out = []
     for B in blocks:
+        if B is None:
+            continue
         if len(B["members"]) == 1:
             out.append(kops[B["members"][0]])
             continue
@@ -440,6 +459,55 @@ class _HostMirror:
 
 
 _MIRROR = _HostMirror()
+
+
+class _PinnedRing:
+    """Matrix-pool uploads through a fixed ring of pinned host buffers.
+
+    A flush copies its pool host -> device asynchronously, so the host can run
+    ahead of the GPU.  A slot is refilled only after the copy that last used
+    it has completed (its event), which bounds how far ahead the host runs;
+    buffers grow on demand and are never freed, so steady-state flushes make
+    no pinned allocations."""
+
+    SLOTS = 8
+
+    def __init__(self):
+        self.slots: List[Optional[tuple]] = [None] * self.SLOTS
+        self.i = 0
+        self.lock = threading.Lock()
+
+    def upload(self, host: np.ndarray, device: torch.device) -> torch.Tensor:
+        flat = np.ascontiguousarray(host, dtype=np.complex64).reshape(-1)
+        n = flat.size
+        with self.lock:
+            i = self.i
+            self.i = (i + 1) % self.SLOTS
+            buf, ev = self.slots[i] if self.slots[i] is not None else (None, None)
+            if ev is not None:
+                ev.synchronize()
+            if buf is None or buf.numel() < n:
+                cap = max(n, 2 * (buf.numel() if buf is not None else 0), 4096)
+                buf = torch.empty(cap, dtype=torch.complex64, pin_memory=True)
+            buf[:n].numpy()[:] = flat
+            out = torch.empty(n, dtype=torch.complex64, device=device)
+            out.copy_(buf[:n], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(torch.cuda.current_stream(device))
+            self.slots[i] = (buf, ev)
+        return out.reshape(host.shape)
+
+
+_RINGS: Dict[torch.device, _PinnedRing] = {}
+
+
+def _upload(host: np.ndarray, device: torch.device) -> torch.Tensor:
+    if device.index is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    ring = _RINGS.get(device)
+    if ring is None:
+        ring = _RINGS.setdefault(device, _PinnedRing())
+    return ring.upload(host, device)
 
 
 def _host_params(tape: Sequence[TapeOp]) -> List[Optional[np.ndarray]]:
@@ -964,10 +1032,9 @@ class Compiled:
         """The matrix pool on the device: staged through pinned host memory and
         copied asynchronously on the current stream (no wait for queued work)."""
         host = self.pool_host(tape, with_grad, with_adj)
-        t = torch.from_numpy(host)
         if torch.device(self.device).type != "cuda":
-            return t.to(self.device)
-        return t.pin_memory().to(self.device, non_blocking=True)
+            return torch.from_numpy(host).to(self.device)
+        return _upload(host, torch.device(self.device))
 
     def pool_host(self, tape: Sequence[TapeOp], with_grad: bool, with_adj: bool) -> np.ndarray:
         """The matrix pool (complex64) for the current parameter values."""

@@ -47,21 +47,23 @@ def expval(qdev, wires: Union[int, List[int]], observables):
         for rotation in obs.diagonalizing_gates():
             rotation(qdev, wires=wire)
     z = _z_all(qdev)
-    ev_host = [obs.eigvals.real.detach().to("cpu", torch.float32) for obs in observables]
-    evs = [e.to(z.device) for e in ev_host]
-    norm = None
-    if any(float(e[0] + e[1]) != 0.0 for e in ev_host):
+    # <O> = p0 * e0 + p1 * e1 with p0 - p1 = <Z_wire>, p0 + p1 = |psi|^2
+    ev = torch.stack([obs.eigvals.real.detach().to("cpu", torch.float32).reshape(-1)[:2] for obs in observables])
+    half_diff, half_sum = 0.5 * (ev[:, 0] - ev[:, 1]), 0.5 * (ev[:, 0] + ev[:, 1])
+    coef = torch.stack([half_diff, half_sum])
+    if z.is_cuda:
+        coef = coef.pin_memory()
+    coef = coef.to(z.device, non_blocking=True)
+    if list(wires) == list(range(qdev.n_wires)):
+        zc = z
+    else:
+        zc = z[:, torch.as_tensor(list(wires), dtype=torch.long).to(z.device)]
+    out = zc * coef[0]
+    if bool((half_sum != 0).any()):
         # marginals sum to the squared norm, which need not be 1 (set_states, cu2, reset...)
         norm = expval_pauli_strings(qdev.get_states_1d(), qdev.n_wires, ["I" * qdev.n_wires])[:, 0]
-    out = []
-    for wire, ev in zip(wires, evs):
-        # <O> = p0 * e0 + p1 * e1 with p0 - p1 = <Z_wire>, p0 + p1 = |psi|^2
-        zc = z[:, wire]
-        val = 0.5 * (ev[0] - ev[1]) * zc
-        if norm is not None:
-            val = val + 0.5 * (ev[0] + ev[1]) * norm
-        out.append(val)
-    return torch.stack(out, dim=-1)
+        out = out + norm[:, None] * coef[1]
+    return out
 
 
 class MeasureAll(QuantumModule):

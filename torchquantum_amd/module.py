@@ -11,7 +11,7 @@ import torch.nn as nn
 __all__ = ["QuantumModule", "QuantumModuleList", "QuantumModuleDict", "static_support"]
 
 # widest dense block the fused planner builds
-MAX_STATIC_BLOCK = 4
+MAX_STATIC_BLOCK = 6
 
 
 def _find_device(module, args, kwargs):
@@ -33,7 +33,8 @@ def static_support(f):
     module in static mode finishes its forward, the tape is compiled by the
     fused planner with fusion width ``wires_per_block`` (blocks wider than
     MAX_STATIC_BLOCK wires are built as MAX_STATIC_BLOCK-wire blocks, which
-    gives the same state) and run."""
+    gives the same state) and run.  Blocks of 5 and 6 wires run on the
+    matrix-core kernel."""
 
     @functools.wraps(f)
     def forward_static(self, *args, **kwargs):
