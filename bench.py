@@ -296,7 +296,7 @@ def config5(args, tq):
     torch.cuda.synchronize()
     res["fused_gradient_s"] = time.perf_counter() - t0
     res["n_params"] = sum(p.numel() for p in params if p is not None)
-    res["loss"] = float(loss)
+    res["loss"] = float(loss.detach())
     return res
 
 
@@ -326,6 +326,8 @@ def main():
                     help="run only the dominant kernel's timing loops (for rocprofv3 summaries)")
     ap.add_argument("--config5", action="store_true", help="run the 30-qubit workload only")
     ap.add_argument("--only-dense", action="store_true", help="run only the dense k = 5, 6 block timings")
+    ap.add_argument("--only-gradient", action="store_true",
+                    help="run only the fused adjoint gradient (--adj-qubits, --adj-bsz, --adj-steps)")
     ap.add_argument("--c5-qubits", type=int, default=30)
     ap.add_argument("--c5-bsz", type=int, default=4)
     ap.add_argument("--cpu-layers", type=int, default=2)
@@ -350,6 +352,11 @@ def main():
         return
     if args.only_dense:
         print(json.dumps({"dense_block": dense_block_bench()}), flush=True)
+        return
+    if args.only_gradient:
+        an, ab = args.adj_qubits, args.adj_bsz
+        dt, loss, _ = time_gradient(an, ab, 10, "fused", args.adj_steps, 1, tq)
+        print(json.dumps({"fused_gradient_s": dt, "workload": f"{an} qubits, bsz {ab}", "loss": loss}), flush=True)
         return
     if args.config5:
         print(json.dumps({"config5": config5(args, tq)}), flush=True)
