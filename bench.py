@@ -106,7 +106,7 @@ def time_gradient(n, bsz, layers, mode, steps, warmup, tq):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     grads = torch.cat([p.grad.reshape(-1) for p in params])
-    return dt, float(loss), grads
+    return dt, float(loss.detach()), grads
 
 
 def kernel_roofline(n, bsz, iters=50):
@@ -333,8 +333,8 @@ def main():
     ap.add_argument("--cpu-layers", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md but not built)"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding over several "
+                                   "GPUs is designed but not built)"}))
         sys.exit(2)
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no ROCm GPU visible"}))
@@ -404,6 +404,17 @@ def main():
         extra["batch"] = time_batch(256, 5, tq)
         log(f"[bench] batch graph {extra['batch']['ms_graph_replay']:.2f} ms")
 
+    # the reference's own CPU-runnable case: 10 qubits, bsz 32, 8 x [U3 all, CNOT ring], <Z_i>
+    from circuits import config1_ops
+    ops1 = config1_ops(seed=0)
+    upd1 = len(ops1) * 32 * (1 << 10)
+    log("[bench] 10-qubit bsz-32 circuit (160 gates)")
+    t1_pg, _ = time_forward(10, 32, ops1, "pergate", 20, 3, tq)
+    t1_f, _ = time_forward(10, 32, ops1, "fused", 20, 3, tq)
+    extra["small_circuit"] = {"workload": "10 qubits, bsz 32, 8 x [U3 all wires, CNOT ring], <Z_i> (160 gates)",
+                              "pergate_ms_per_circuit_batch": t1_pg * 1e3, "fused_ms_per_circuit_batch": t1_f * 1e3,
+                              "fused_amp_updates_per_s": upd1 / t1_f}
+
     cpu = None
     if not args.skip_cpu:
         sample = ops[: 39 * args.cpu_layers]
@@ -412,6 +423,10 @@ def main():
         cpu = {"value": rate, "unit": "amplitude updates/s", "cores": threads, "kind": "port",
                "sample": f"first {len(sample)} gates of the 780-gate circuit, 20 qubits bsz 16, torch CPU "
                          f"bmm path restated from gate_wrapper.py:96-140 ({cdt:.1f} s)"}
+        rate1, _, cdt1 = cpu_baseline(ops1, 10, 32, len(ops1))
+        cpu["small_circuit"] = {"ms_per_circuit_batch": cdt1 * 1e3, "amp_updates_per_s": rate1,
+                                "sample": "the whole 10-qubit bsz-32 circuit (160 gates), same CPU path"}
+        extra["small_circuit"]["fused_speedup_vs_cpu"] = cdt1 / t1_f
 
     line = {
         "metric": "amplitude updates/s & adjoint grads/s vs plain per-gate kernel, % HBM peak",

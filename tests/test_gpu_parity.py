@@ -334,3 +334,25 @@ def test_wide_fusion_blocks(width):
         c = E.Compiled(q._tape, n, bsz, q.engine_options, "cpu")       # the plan, built on the host only
         wide = [k for k in c.kops if k.kind == "matk" and len(k.bits) > 4]
         assert wide and (batched_frac > 0 or any(not k.batched for k in wide))
+
+
+@gpu
+@pytest.mark.parametrize("mode", ["pergate", "fused"])
+def test_tiny_and_wide_batches(mode):
+    """Edge shapes: no gates at all, 1 and 2 wires, bsz 1, and a wide batch
+    (bsz 1024) with per-row parameters; vs the oracle, 1e-5."""
+    import torchquantum_amd as tq
+    import torchquantum_amd.functional as tqf
+    q = _qdev(3, 2, mode)
+    st = q.get_states_1d().cpu().numpy()
+    assert np.abs(st - O.zero_state(3, 2)).max() == 0.0
+    rng = np.random.default_rng(5)
+    for n, bsz in ((1, 1), (1, 3), (2, 1), (2, 4)):
+        names = ["h", "rx", "ry", "rz", "u3", "s", "t", "sx"] + (["cnot", "cz", "swap", "crx", "rzz"] if n > 1 else [])
+        ops = random_ops(n, 12, bsz, rng, names=names)
+        got = _run(n, bsz, ops, mode)
+        assert np.abs(got - O.run_ops(oracle_ops(ops, bsz), n, bsz)).max() < 1e-5, (n, bsz)
+    n, bsz = 8, 1024
+    ops = random_ops(n, 30, bsz, rng, names=["rx", "ry", "u3", "cnot", "crx", "rzz"], batched_frac=0.7)
+    got = _run(n, bsz, ops, mode)
+    assert np.abs(got - O.run_ops(oracle_ops(ops, bsz), n, bsz)).max() < 1e-5

@@ -47,6 +47,34 @@ def test_abi_rejects_bad_arguments_without_gpu():
     assert b"out of range" in lib.tqsv_last_error()
     assert lib.tqsv_apply_2q(None, 16, 1, 4, 1, 1, 0, 0, 16, 0) == -1
     assert lib.tqsv_expval_workspace_bytes(1, 40, 1) == -1
+    assert lib.tqsv_sample_workspace_bytes(1, 33) == -1
+
+
+@pytest.mark.skipif(not _native.lib_available(), reason="libtqsv.so not built")
+def test_run_gates_rejects_bad_records_without_gpu():
+    """tqsv_run_gates validates each record before its launch and names the
+    failing gate; records come from engine.gate_record."""
+    import ctypes
+    lib = _native.lib()
+    assert lib.tqsv_gate_words() == E.GATE_WORDS
+
+    def run(recs, state=16):
+        arr = np.ascontiguousarray(np.stack(recs).astype(np.int64))
+        return lib.tqsv_run_gates(None, state, 1, 4, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                  len(recs), 16)
+
+    good = E.gate_record(E.KOp("mat1", (1,)), 0, 0)
+    bad_kind = good.copy()
+    bad_kind[0] = 9
+    assert run([bad_kind]) == -1 and b"gate 0" in lib.tqsv_last_error() and b"kind 9" in lib.tqsv_last_error()
+    bad_k = E.gate_record(E.KOp("mat2", (3, 1)), 0, 0)
+    bad_k[1] = 1
+    assert run([bad_k]) == -1 and b"needs k=2" in lib.tqsv_last_error()
+    out_of_range = E.gate_record(E.KOp("mat1", (6,)), 0, 0)     # 4 wires: bit 6 does not exist
+    assert run([out_of_range]) == -1 and b"gate 0" in lib.tqsv_last_error()
+    assert run([good], state=0) == -1 and b"null" in lib.tqsv_last_error()
+    with pytest.raises(ValueError):
+        E.gate_record(E.KOp("matk", tuple(range(11))), 0, 0)
 
 
 # ------------------------------------------------------------------ gate tables

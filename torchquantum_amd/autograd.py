@@ -48,6 +48,7 @@ class CircuitFunction(torch.autograd.Function):
         psi = out.clone()
         lam = gout.to(torch.complex64).reshape(out.shape).contiguous().clone()
         grad = E.adjoint(c, psi, lam, ctx.pool)
+        gsum = None            # batch-row sums for parameters shared by all rows, one reduction
         grads: List = []
         for j, ti in enumerate(payload.param_idx):
             if not ctx.needs_input_grad[2 + j]:
@@ -56,9 +57,12 @@ class CircuitFunction(torch.autograd.Function):
             op = payload.tape[ti]
             off = c.param_offsets[ti]
             P = op.n_params()
-            g = grad[:, off:off + P]
             if op.params.shape[0] == 1:
-                g = g.sum(0, keepdim=True)
+                if gsum is None:
+                    gsum = grad.sum(0, keepdim=True)
+                g = gsum[:, off:off + P]
+            else:
+                g = grad[:, off:off + P]
             grads.append(g.reshape(op.params.shape).to(op.params.dtype))
         gstate = lam if ctx.needs_input_grad[1] else None
         return (None, gstate, *grads)

@@ -6,7 +6,7 @@ column sums, MSE gradient, Adam).  torch only holds the buffers.
 Plan features are computed on the host from a compiled plan (no launch), so
 the planner can score every candidate option set cheaply and run only the
 predicted best; exhaustive timing (``autotune.Autotuner``) is the fallback
-and the judge of the prediction error.
+and the reference for the prediction error.
 """
 from __future__ import annotations
 

@@ -170,7 +170,8 @@ __device__ __forceinline__ void b_mat1(float2 (&p)[NS], float2 (&l)[NS], const f
 // psi <- U^dag psi ; Mg += lam conj(psi_prev)^T ; lam <- U^dag lam.  The
 // parameter gradients follow as Re sum_ab D_p[a][b] conj(Mg[a][b]) in a tiny
 // pass afterwards (16 FMA per pair instead of 20 per parameter).
-template <int K>
+// NC: the op has no controls (every pair is updated: no selects)
+template <int K, bool NC>
 __device__ __forceinline__ void b_mat1_mg(float2 (&p)[NS], float2 (&l)[NS], const float2* __restrict__ M, bool tok,
                                           int rm, int rv, float (&g)[8]) {
   const float2 u0 = M[0], u1 = M[1], u2 = M[2], u3 = M[3];
@@ -179,7 +180,7 @@ __device__ __forceinline__ void b_mat1_mg(float2 (&p)[NS], float2 (&l)[NS], cons
   for (int j = 0; j < NS; ++j) {
     if (j & (1 << K)) continue;
     const int j1 = j | (1 << K);
-    const bool ok = tok && ((j & rm) == rv);
+    const bool ok = NC || (tok && ((j & rm) == rv));
     const float2 x0 = p[j], x1 = p[j1];
     const float2 y0 = cfmac(u2, x1, cmulc(u0, x0));
     const float2 y1 = cfmac(u3, x1, cmulc(u1, x0));
@@ -192,10 +193,17 @@ __device__ __forceinline__ void b_mat1_mg(float2 (&p)[NS], float2 (&l)[NS], cons
     m11 = cfmac(ok ? y1 : z, l1, m11);
     const float2 n0 = cfmac(u2, l1, cmulc(u0, l0));
     const float2 n1 = cfmac(u3, l1, cmulc(u1, l0));
-    p[j] = csel(ok, y0, x0);
-    p[j1] = csel(ok, y1, x1);
-    l[j] = csel(ok, n0, l0);
-    l[j1] = csel(ok, n1, l1);
+    if (NC) {
+      p[j] = y0;
+      p[j1] = y1;
+      l[j] = n0;
+      l[j1] = n1;
+    } else {
+      p[j] = csel(ok, y0, x0);
+      p[j1] = csel(ok, y1, x1);
+      l[j] = csel(ok, n0, l0);
+      l[j1] = csel(ok, n1, l1);
+    }
   }
   g[0] += m00.x; g[1] += m00.y; g[2] += m01.x; g[3] += m01.y;
   g[4] += m10.x; g[5] += m10.y; g[6] += m11.x; g[7] += m11.y;
@@ -483,11 +491,20 @@ __device__ __forceinline__ void bwd_op(float2 (&p)[NS], float2 (&l)[NS], const i
       return;
     }
     if (fl & F_MG) {
-      switch (w[3]) {
-        case 0: b_mat1_mg<0>(p, l, M, tok, rm, rv, g8); break;
-        case 1: b_mat1_mg<1>(p, l, M, tok, rm, rv, g8); break;
-        case 2: b_mat1_mg<2>(p, l, M, tok, rm, rv, g8); break;
-        default: b_mat1_mg<3>(p, l, M, tok, rm, rv, g8); break;
+      if (fl & F_NOCTRL) {
+        switch (w[3]) {
+          case 0: b_mat1_mg<0, true>(p, l, M, tok, rm, rv, g8); break;
+          case 1: b_mat1_mg<1, true>(p, l, M, tok, rm, rv, g8); break;
+          case 2: b_mat1_mg<2, true>(p, l, M, tok, rm, rv, g8); break;
+          default: b_mat1_mg<3, true>(p, l, M, tok, rm, rv, g8); break;
+        }
+      } else {
+        switch (w[3]) {
+          case 0: b_mat1_mg<0, false>(p, l, M, tok, rm, rv, g8); break;
+          case 1: b_mat1_mg<1, false>(p, l, M, tok, rm, rv, g8); break;
+          case 2: b_mat1_mg<2, false>(p, l, M, tok, rm, rv, g8); break;
+          default: b_mat1_mg<3, false>(p, l, M, tok, rm, rv, g8); break;
+        }
       }
       return;
     }

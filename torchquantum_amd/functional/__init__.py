@@ -46,16 +46,27 @@ def _normalize_params(name: str, params):
     return params
 
 
+def _wire_list(wires):
+    if isinstance(wires, (int, np.integer)):
+        return [int(wires)]
+    if isinstance(wires, (list, tuple)) and all(type(w) is int for w in wires):
+        return list(wires)
+    return [int(w) for w in np.array(wires).reshape(-1)]
+
+
 def gate_wrapper(name, q_device, wires, params=None, n_wires=None, static=False, parent_graph=None,
                  inverse=False, comp_method="bmm"):
     """Record one gate on ``q_device`` (the reference's choke point,
     torchquantum/functional/gate_wrapper.py:324-446)."""
     cname = G.canonical(name)
-    s = G.spec(cname)
+    _record(name, cname, G.spec(cname), q_device, wires, params, inverse, comp_method)
+
+
+def _record(name, cname, s, q_device, wires, params, inverse, comp_method):
     if comp_method not in ("bmm", "einsum"):
         raise ValueError(f"unknown comp_method '{comp_method}'")
     params = _normalize_params(cname, params)
-    wires = [wires] if isinstance(wires, (int, np.integer)) else [int(w) for w in np.array(wires).reshape(-1)]
+    wires = _wire_list(wires)
     if getattr(q_device, "record_op", False):
         q_device.op_history.append({
             "name": name,
@@ -108,9 +119,12 @@ def gate_wrapper(name, q_device, wires, params=None, n_wires=None, static=False,
 
 
 def _make_gate(name: str):
+    cname = G.canonical(name)
+    spec = G.spec(cname)
+
     def gate(q_device, wires, params=None, n_wires=None, static=False, parent_graph=None, inverse=False,
              comp_method="bmm"):
-        gate_wrapper(name, q_device, wires, params, n_wires, static, parent_graph, inverse, comp_method)
+        _record(name, cname, spec, q_device, wires, params, inverse, comp_method)
 
     gate.__name__ = name
     gate.__qualname__ = name
