@@ -249,6 +249,85 @@ __global__ __launch_bounds__(AJ_THREADS) void k_adj_diag4(float4* __restrict__ p
   block_accum(g, P, grad + row * gstride + gslot);
 }
 
+// 2-qubit gate, bit 0 free: items are float4 quadruples (positions in float4 bits)
+template <bool BATCHED, bool NT>
+__global__ __launch_bounds__(AJ_THREADS) void k_adj_2q4(float4* __restrict__ psi, float4* __restrict__ lam, int n,
+                                                       uint64_t items_row, InsertSpec ins4, uint64_t cval4,
+                                                       uint64_t mhi4, uint64_t mlo4, const float2* __restrict__ mat,
+                                                       int64_t bstride, int P, float* __restrict__ grad,
+                                                       int64_t gstride, int gslot, int reps) {
+  const uint64_t row = blockIdx.y;
+  const float2* M = BATCHED ? mat + row * bstride : mat;
+  float g[4] = {0.f, 0.f, 0.f, 0.f};
+  const uint64_t rbase = row << (n - 1);
+  uint64_t q = (uint64_t)blockIdx.x * AJ_THREADS * (uint64_t)reps + threadIdx.x;
+  for (int rr = 0; rr < reps; ++rr, q += AJ_THREADS) {
+    if (q >= items_row) break;
+    const uint64_t b = rbase + (insert_zeros(q, ins4) | cval4);
+    const uint64_t idx[4] = {b, b | mlo4, b | mhi4, b | mhi4 | mlo4};
+    float4 xv[4], lv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      xv[r] = ld4<NT>(psi + idx[r]);
+      lv[r] = ld4<NT>(lam + idx[r]);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {   // the two amplitudes of each float4
+      float2 x[4], l[4], y[4], m[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        x[r] = h ? make_float2(xv[r].z, xv[r].w) : make_float2(xv[r].x, xv[r].y);
+        l[r] = h ? make_float2(lv[r].z, lv[r].w) : make_float2(lv[r].x, lv[r].y);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float2 acc = cmulc(M[r], x[0]);
+#pragma unroll
+        for (int c = 1; c < 4; ++c) acc = cfmac(M[4 * c + r], x[c], acc);
+        y[r] = acc;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k < P) {
+          const float2* D = M + 16 * (k + 1);
+          float t = 0.f;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            float2 v = cmul(D[4 * r], y[0]);
+#pragma unroll
+            for (int c = 1; c < 4; ++c) v = cfma(D[4 * r + c], y[c], v);
+            t += redot(l[r], v);
+          }
+          g[k] += t;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float2 acc = cmulc(M[r], l[0]);
+#pragma unroll
+        for (int c = 1; c < 4; ++c) acc = cfmac(M[4 * c + r], l[c], acc);
+        m[r] = acc;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (h) {
+          xv[r].z = y[r].x; xv[r].w = y[r].y;
+          lv[r].z = m[r].x; lv[r].w = m[r].y;
+        } else {
+          xv[r].x = y[r].x; xv[r].y = y[r].y;
+          lv[r].x = m[r].x; lv[r].y = m[r].y;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      st4<NT>(psi + idx[r], xv[r]);
+      st4<NT>(lam + idx[r], lv[r]);
+    }
+  }
+  block_accum(g, P, grad + row * gstride + gslot);
+}
+
 // tiles per row for the float4 adjoint kernels: 4 items per thread, more when
 // the grid would pass 2^22 workgroups per row
 static dim3 adj_tiles(uint64_t items_row, int64_t bsz, int& reps) {
@@ -341,8 +420,25 @@ TQSV_API int tqsv_adjoint_2q(hipStream_t s, void* psi, void* lam, int64_t bsz, i
   const uint64_t mhi = 1ull << bit_hi, mlo = 1ull << bit_lo;
   if ((cmask & (mhi | mlo)) || (n < 64 && (cmask >> n)) || (cval & ~cmask)) { set_error("adjoint_2q: bad control mask"); return E_ARG; }
   InsertSpec ins;
-  if (!build_insert_adj(ins, cmask | mhi | mlo, n)) { set_error("adjoint_2q: too many controls"); return E_LIMIT; }
-  const uint64_t items_row = 1ull << (n - 2 - __builtin_popcountll(cmask));
+  const int c = __builtin_popcountll(cmask);
+  const uint64_t all = cmask | mhi | mlo;
+  if (!(all & 1ull)) {
+    if (!build_insert_adj(ins, all >> 1, n - 1)) { set_error("adjoint_2q: too many controls"); return E_LIMIT; }
+    const uint64_t items_row = 1ull << (n - 3 - c);
+    int reps;
+    const dim3 grid = adj_tiles(items_row, bsz, reps);
+    const bool nt = stream_nt(2 * bsz, n);
+#define ADJ2Q4(B, NTV)                                                                                        \
+  k_adj_2q4<B, NTV><<<grid, AJ_THREADS, 0, s>>>((float4*)psi, (float4*)lam, n, items_row, ins, cval >> 1,    \
+                                                mhi >> 1, mlo >> 1, (const float2*)mat, mat_bstride, P, grad, \
+                                                grad_stride, gslot, reps)
+    if (mat_bstride) { if (nt) ADJ2Q4(true, true); else ADJ2Q4(true, false); }
+    else { if (nt) ADJ2Q4(false, true); else ADJ2Q4(false, false); }
+#undef ADJ2Q4
+    return check_launch("tqsv_adjoint_2q");
+  }
+  if (!build_insert_adj(ins, all, n)) { set_error("adjoint_2q: too many controls"); return E_LIMIT; }
+  const uint64_t items_row = 1ull << (n - 2 - c);
   const dim3 grid = adj_grid(items_row, bsz);
   if (mat_bstride)
     k_adj_2q<true><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, mhi, mlo,

@@ -331,64 +331,115 @@ def embed(u: np.ndarray, op_bits: Sequence[int], block_bits: Sequence[int]) -> n
 # ----------------------------------------------------------------------------
 # matrix values (host, float64)
 # ----------------------------------------------------------------------------
-class GateValues:
-    """Gate matrices for the current parameter values, evaluated on the host.
+class _GVLayout:
+    """The structural part of GateValues for one plan (it depends only on the
+    tape structure, which the plan cache key fixes): tape ops grouped by
+    (gate, width, has-params), each op's group key, first row in its group,
+    row count and whether its derivatives are wanted."""
 
-    Tape ops are grouped by (gate, width, has-params); each group's matrices
-    are built in one vectorised call into ``UG[key]`` ([rows, d, d]
-    complex128, inverse ops already conjugate-transposed) and, when any
-    member is trainable, derivative matrices ``DG[key]`` ([rows, P, d, d]).
-    ``gkey[i]``, ``grow[i]``, ``rows[i]`` locate tape op i; ``U[i]`` / ``D[i]``
-    are views.  Parameters living on the GPU are fetched with one copy."""
-
-    def __init__(self, tape: Sequence[TapeOp], want_grad: bool):
+    def __init__(self, tape: Sequence[TapeOp]):
         T = len(tape)
-        self.U: List[np.ndarray] = [None] * T
-        self.D: List[Optional[np.ndarray]] = [None] * T
-        self.gkey: List[tuple] = [None] * T
+        self.gkey: List[Optional[tuple]] = [None] * T
         self.grow = np.zeros(T, dtype=np.int64)
         self.rows = np.ones(T, dtype=np.int64)
-        self.UG: Dict[tuple, np.ndarray] = {}
-        self.DG: Dict[tuple, Optional[np.ndarray]] = {}
-        host = _host_params(tape)
+        self.dwant = np.zeros(T, dtype=bool)
+        self.mat_ops: List[int] = []
         groups: Dict[tuple, List[int]] = {}
         for i, op in enumerate(tape):
             if op.is_matrix_param:
-                m = host[i].astype(np.complex128)
-                m = m.reshape((-1,) + m.shape[-2:]) if m.ndim == 3 else m[None]
-                if op.inverse:
-                    m = np.conj(np.swapaxes(m, -1, -2))
-                key = ("__mat__", i)
-                self.UG[key] = m
-                self.DG[key] = None
-                self.gkey[i] = key
-                self.rows[i] = m.shape[0]
-                self.U[i] = m
+                self.mat_ops.append(i)
+                self.gkey[i] = ("__mat__", i)
+                self.rows[i] = op.params.shape[0] if op.params.dim() == 3 else 1
                 continue
             key = (op.name, len(op.wires), op.n_params() > 0)
             groups.setdefault(key, []).append(i)
+        self.groups = []
         for key, idx in groups.items():
+            has_p = key[2]
+            inv_rows = None
+            if has_p:
+                off = 0
+                inv = []
+                for i in idx:
+                    r = tape[i].params.shape[0]
+                    self.grow[i], self.rows[i] = off, r
+                    self.dwant[i] = tape[i].wants_grad()
+                    inv.extend([tape[i].inverse] * r)
+                    off += r
+                inv_rows = np.array(inv, dtype=bool)
+                P = tape[idx[0]].params.shape[-1]
+            else:
+                P = 0
+                for i in idx:
+                    self.grow[i] = 1 if tape[i].inverse else 0
+            for i in idx:
+                self.gkey[i] = key
+            self.groups.append((key, idx, P, inv_rows, bool(self.dwant[idx].any()) if has_p else False))
+
+
+class _GroupViews:
+    """gv.U[i] / gv.D[i]: tape op i's rows of its group's matrices (a view)."""
+
+    def __init__(self, gv: "GateValues", deriv: bool):
+        self.gv, self.deriv = gv, deriv
+
+    def __getitem__(self, i: int):
+        gv = self.gv
+        key = gv.gkey[i]
+        if key is None:
+            return None
+        if self.deriv:
+            if not gv.dwant[i]:
+                return None
+            M = gv.DG.get(key)
+        else:
+            M = gv.UG.get(key)
+        if M is None:
+            return None
+        g, r = int(gv.grow[i]), int(gv.rows[i])
+        return M[g:g + r]
+
+
+class GateValues:
+    """Gate matrices for the current parameter values, evaluated on the host.
+
+    Tape ops are grouped by (gate, width, has-params) (``_GVLayout``); each
+    group's matrices are built in one vectorised call into ``UG[key]``
+    ([rows, d, d] complex128, inverse ops already conjugate-transposed) and,
+    when any member is trainable, derivative matrices ``DG[key]``
+    ([rows, P, d, d]).  ``gkey[i]``, ``grow[i]``, ``rows[i]`` locate tape op
+    i; ``U[i]`` / ``D[i]`` are views.  Parameters living on the GPU are
+    fetched with one copy (or reused from their host copies)."""
+
+    def __init__(self, tape: Sequence[TapeOp], want_grad: bool, layout: Optional[_GVLayout] = None):
+        lay = layout if layout is not None else _GVLayout(tape)
+        self.gkey, self.grow, self.rows = lay.gkey, lay.grow, lay.rows
+        self.dwant = lay.dwant if want_grad else np.zeros(len(tape), dtype=bool)
+        self.UG: Dict[tuple, np.ndarray] = {}
+        self.DG: Dict[tuple, Optional[np.ndarray]] = {}
+        self.U = _GroupViews(self, False)
+        self.D = _GroupViews(self, True)
+        host = _host_params(tape)
+        for i in lay.mat_ops:
+            op = tape[i]
+            m = host[i].astype(np.complex128)
+            m = m.reshape((-1,) + m.shape[-2:]) if m.ndim == 3 else m[None]
+            if op.inverse:
+                m = np.conj(np.swapaxes(m, -1, -2))
+            self.UG[lay.gkey[i]] = m
+            self.DG[lay.gkey[i]] = None
+        for key, idx, P, inv, any_grad in lay.groups:
             name, k, has_p = key
             builder = G.width_builder(name, k)
             if has_p:
-                ps = [host[i] for i in idx]
-                rows = [p.shape[0] for p in ps]
-                P = ps[0].shape[-1]
-                allp = torch.from_numpy(np.concatenate([p.reshape(-1, P) for p in ps], axis=0))
+                allp = torch.from_numpy(np.concatenate([host[i].reshape(-1, P) for i in idx], axis=0))
                 U = builder(allp).numpy()
                 D = None
-                if want_grad and any(tape[i].wants_grad() for i in idx):
+                if want_grad and any_grad:
                     def one(q, _b=builder):
                         return torch.view_as_real(_b(q.unsqueeze(0))[0])
                     J = torch.func.vmap(torch.func.jacfwd(one))(allp)        # [N, d, d, 2, P]
                     D = torch.complex(J[..., 0, :], J[..., 1, :]).movedim(-1, 1).numpy()  # [N,P,d,d]
-                inv = np.zeros(U.shape[0], dtype=bool)
-                off = 0
-                for i, r in zip(idx, rows):
-                    self.grow[i], self.rows[i] = off, r
-                    if tape[i].inverse:
-                        inv[off:off + r] = True
-                    off += r
                 if inv.any():
                     U = U.copy()
                     U[inv] = np.conj(np.swapaxes(U[inv], -1, -2))
@@ -396,19 +447,10 @@ class GateValues:
                         D = D.copy()
                         D[inv] = np.conj(np.swapaxes(D[inv], -1, -2))
                 self.UG[key], self.DG[key] = U, D
-                for i in idx:
-                    g, r = self.grow[i], self.rows[i]
-                    self.U[i] = U[g:g + r]
-                    self.D[i] = D[g:g + r] if (D is not None and tape[i].wants_grad()) else None
-                    self.gkey[i] = key
             else:
                 u = builder(torch.zeros(1, 1, dtype=torch.float64)).numpy()
-                U = np.concatenate([u, np.conj(np.swapaxes(u, -1, -2))], axis=0)   # row 0: U, row 1: U^dag
-                self.UG[key], self.DG[key] = U, None
-                for i in idx:
-                    self.grow[i] = 1 if tape[i].inverse else 0
-                    self.U[i] = U[self.grow[i]:self.grow[i] + 1]
-                    self.gkey[i] = key
+                self.UG[key] = np.concatenate([u, np.conj(np.swapaxes(u, -1, -2))], axis=0)   # row 0: U, 1: U^dag
+                self.DG[key] = None
 
 
 class _HostMirror:
@@ -516,18 +558,36 @@ def _host_params(tape: Sequence[TapeOp]) -> List[Optional[np.ndarray]]:
     concatenation and one device-to-host copy."""
     out: List[Optional[np.ndarray]] = [None] * len(tape)
     dev: Dict[tuple, List[int]] = {}
+    cpu_real: List[int] = []
     for i, op in enumerate(tape):
-        if op.params is None:
+        p = op.params
+        if p is None:
             continue
-        p = op.params.detach()
         if p.device.type == "cpu":
-            out[i] = p.numpy().astype(np.complex128 if p.is_complex() else np.float64)
+            if p.is_complex():
+                out[i] = p.detach().numpy().astype(np.complex128)
+            else:
+                cpu_real.append(i)
             continue
-        hit = _MIRROR.get(op.params)
+        hit = _MIRROR.get(p)
         if hit is not None:
             out[i] = hit
         else:
             dev.setdefault((p.device, p.dtype), []).append(i)
+    if cpu_real:
+        # real host parameters ([rows, P]): one concatenation and one
+        # conversion per parameter width P
+        byP: Dict[int, List[int]] = {}
+        for i in cpu_real:
+            byP.setdefault(tape[i].params.shape[-1], []).append(i)
+        with torch.no_grad():
+            for P, idx in byP.items():
+                flat = torch.cat([tape[i].params for i in idx], 0).numpy().astype(np.float64)
+                off = 0
+                for i in idx:
+                    r = tape[i].params.shape[0]
+                    out[i] = flat[off:off + r]
+                    off += r
     for (d, dt), idx in dev.items():
         flat = torch.cat([tape[i].params.detach().reshape(-1) for i in idx]).cpu().numpy()
         flat = flat.astype(np.complex128 if np.iscomplexobj(flat) else np.float64)
@@ -956,6 +1016,7 @@ class Compiled:
         self.steps: List[tuple] = []
         self.perms: Dict[int, np.ndarray] = {}
         self._fill = None
+        self._gv_layout: Optional[_GVLayout] = None
         self.mg_items = np.zeros(0, dtype=MG_ITEM)
         self.grad_width = max(self.n_slots, 1)
         if opts.mode == "fused" and n >= R_BITS:
@@ -1038,7 +1099,9 @@ class Compiled:
 
     def pool_host(self, tape: Sequence[TapeOp], with_grad: bool, with_adj: bool) -> np.ndarray:
         """The matrix pool (complex64) for the current parameter values."""
-        gv = GateValues(tape, with_grad)
+        if self._gv_layout is None:
+            self._gv_layout = _GVLayout(tape)
+        gv = GateValues(tape, with_grad, self._gv_layout)
         host = np.zeros(self.pool_len, dtype=np.complex64)
         if self._fill is None:
             self._fill = self._build_fill(tape)
