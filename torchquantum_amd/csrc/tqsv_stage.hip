@@ -54,7 +54,8 @@ __device__ __forceinline__ void f_mat1(float2 (&a)[NS], const float2* __restrict
   for (int j = 0; j < NS; ++j) {
     if (j & (1 << K)) continue;
     const int j1 = j | (1 << K);
-    const bool ok = tok && ((j & rm) == rv);
+    if ((j & rm) != rv) continue;  // register controls: a wave-uniform skip
+    const bool ok = tok;
     const float2 x0 = a[j], x1 = a[j1];
     const float2 y0 = cfma(u1, x1, cmul(u0, x0));
     const float2 y1 = cfma(u3, x1, cmul(u2, x0));
@@ -72,7 +73,8 @@ __device__ __forceinline__ void f_mat2(float2 (&a)[NS], const float2* __restrict
   for (int j = 0; j < NS; ++j) {
     if (j & ((1 << KH) | (1 << KL))) continue;
     const int idx[4] = {j, j | (1 << KL), j | (1 << KH), j | (1 << KH) | (1 << KL)};
-    const bool ok = tok && ((j & rm) == rv);
+    if ((j & rm) != rv) continue;  // register controls: a wave-uniform skip
+    const bool ok = tok;
     float2 x[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) x[r] = a[idx[r]];
@@ -126,7 +128,8 @@ __device__ __forceinline__ void f_diag(float2 (&a)[NS], const float2* __restrict
   const float2 p0 = M[0], p1 = M[1];
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
-    const bool ok = tok && ((j & rm) == rv);
+    if ((j & rm) != rv) continue;  // register controls: a wave-uniform skip
+    const bool ok = tok;
     const int par = (tpar + __popc(j & rpm)) & 1;
     const float2 f = par ? p1 : p0;
     a[j] = csel(ok, cmul(f, a[j]), a[j]);
@@ -143,7 +146,8 @@ __device__ __forceinline__ void b_mat1(float2 (&p)[NS], float2 (&l)[NS], const f
   for (int j = 0; j < NS; ++j) {
     if (j & (1 << K)) continue;
     const int j1 = j | (1 << K);
-    const bool ok = tok && ((j & rm) == rv);
+    if ((j & rm) != rv) continue;  // register controls: a wave-uniform skip
+    const bool ok = tok;
     const float2 x0 = p[j], x1 = p[j1];
     const float2 y0 = cfmac(u2, x1, cmulc(u0, x0));
     const float2 y1 = cfmac(u3, x1, cmulc(u1, x0));
@@ -180,7 +184,8 @@ __device__ __forceinline__ void b_mat1_mg(float2 (&p)[NS], float2 (&l)[NS], cons
   for (int j = 0; j < NS; ++j) {
     if (j & (1 << K)) continue;
     const int j1 = j | (1 << K);
-    const bool ok = NC || (tok && ((j & rm) == rv));
+    if (!NC && (j & rm) != rv) continue;  // register controls: a wave-uniform skip
+    const bool ok = NC || tok;
     const float2 x0 = p[j], x1 = p[j1];
     const float2 y0 = cfmac(u2, x1, cmulc(u0, x0));
     const float2 y1 = cfmac(u3, x1, cmulc(u1, x0));
@@ -219,7 +224,8 @@ __device__ __forceinline__ void b_mat2(float2 (&p)[NS], float2 (&l)[NS], const f
   for (int j = 0; j < NS; ++j) {
     if (j & ((1 << KH) | (1 << KL))) continue;
     const int idx[4] = {j, j | (1 << KL), j | (1 << KH), j | (1 << KH) | (1 << KL)};
-    const bool ok = tok && ((j & rm) == rv);
+    if ((j & rm) != rv) continue;  // register controls: a wave-uniform skip
+    const bool ok = tok;
     float2 x[4], y[4], lm[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) { x[r] = p[idx[r]]; lm[r] = l[idx[r]]; }
@@ -281,7 +287,8 @@ __device__ __forceinline__ void b_diag(float2 (&p)[NS], float2 (&l)[NS], const f
   const float2 p0 = M[0], p1 = M[1];
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
-    const bool ok = tok && ((j & rm) == rv);
+    if ((j & rm) != rv) continue;  // register controls: a wave-uniform skip
+    const bool ok = tok;
     const int par = (tpar + __popc(j & rpm)) & 1;
     const float2 f = par ? p1 : p0;
     const float2 y = cmulc(f, p[j]);
@@ -340,7 +347,8 @@ __device__ __forceinline__ void f_xperm(float2 (&a)[NS], bool tok, int rm, int r
   for (int j = 0; j < NS; ++j) {
     if (j & (1 << K)) continue;
     const int j1 = j | (1 << K);
-    const bool ok = tok && ((j & rm) == rv);
+    if ((j & rm) != rv) continue;  // register controls: a wave-uniform skip
+    const bool ok = tok;
     const float2 x0 = a[j], x1 = a[j1];
     a[j] = csel(ok, x1, x0);
     a[j1] = csel(ok, x0, x1);
@@ -351,7 +359,8 @@ __device__ __forceinline__ void f_xperm(float2 (&a)[NS], bool tok, int rm, int r
 __device__ __forceinline__ void f_sign(float2 (&a)[NS], bool tok, int rm, int rv) {
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
-    const bool ok = tok && ((j & rm) == rv);
+    if ((j & rm) != rv) continue;  // register controls: a wave-uniform skip
+    const bool ok = tok;
     a[j] = make_float2(ok ? -a[j].x : a[j].x, ok ? -a[j].y : a[j].y);
   }
 }
