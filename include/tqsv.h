@@ -91,6 +91,27 @@ int tqsv_stage_forward(tqsv_stream_t s, void* state_out, const void* state_in, i
 int tqsv_stage_adjoint(tqsv_stream_t s, void* psi, void* lam, int64_t bsz, int n, const int* prog_host,
                        const void* prog_dev, const void* mats, float* grad, int64_t grad_stride, int grid_cap);
 
+/* ---- run-time compiled stages: the same stage program turned by the host
+ *      planner into straight-line HIP source (every op unrolled with its bits,
+ *      controls and matrix offsets as constants), compiled here with hipRTC
+ *      for gfx950, and launched with the launch shape and argument meaning of
+ *      tqsv_stage_forward / tqsv_stage_adjoint above (same reference
+ *      counterparts).  Needs 2^T >= 64 threads per group (m >= 10). ------- */
+/* source -> code object (no GPU needed).  code may be NULL to query *size;
+ * log (optional) receives the compiler log. */
+int tqsv_jit_compile(const char* src, void* code, int64_t cap, int64_t* size, char* log, int64_t log_cap);
+/* load a code object on the current device; funcs[i] = kernel names[i]. */
+int tqsv_jit_load(const void* code, int64_t size, int n, const char* const* names, void** module, void** funcs);
+int tqsv_jit_unload(void* module);
+/* func: kernel compiled from prog_host's program (parameters: out, in, mats,
+ * init_zero, n_groups, groups_per_wg). */
+int tqsv_jit_stage_forward(tqsv_stream_t s, const void* func, void* state_out, const void* state_in, int64_t bsz,
+                           int n, const int* prog_host, const void* mats, int init_zero, int grid_cap);
+/* func: adjoint kernel compiled from prog_host's program (parameters: psi,
+ * lam, mats, grad, grad_stride, n_groups, groups_per_wg). */
+int tqsv_jit_stage_adjoint(tqsv_stream_t s, const void* func, void* psi, void* lam, int64_t bsz, int n,
+                           const int* prog_host, const void* mats, float* grad, int64_t grad_stride, int grid_cap);
+
 /* ---- per-gate adjoint steps (the unfused gradient path); mat holds U then
  *      P derivative matrices dU/dtheta_p (P <= 4); grad slots gslot..gslot+P-1.
  *      Replace autograd through apply_unitary_bmm (gate_wrapper.py:127-136). */

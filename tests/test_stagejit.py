@@ -1,0 +1,110 @@
+"""Code generation of the run-time compiled stage kernels, checked on the CPU
+(hipRTC cross-compiles for gfx950 without a GPU): the generated HIP compiles,
+carries no packed-FP32 instructions, and reflects the stage programs it was
+generated from.  Their numerics are checked on the GPU (test_gpu_jit.py)."""
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from circuits import apply_ops_tq, random_ops
+
+
+def _plan(n, bsz, ops, **kw):
+    import torchquantum_amd as tq
+    from torchquantum_amd import engine as E
+    q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cpu")
+    apply_ops_tq(q, ops, "cpu", requires_grad=True)
+    return E.Compiled(q._tape, n, bsz, tq.EngineOptions(**kw), "cpu")
+
+
+def _stages(c):
+    return [st for kind, (base, st) in [(k, p) for k, p in c.steps if k == "stage"]]
+
+
+def _objdump():
+    for p in (shutil.which("llvm-objdump"), "/opt/rocm/llvm/bin/llvm-objdump"):
+        if p and Path(p).exists():
+            return p
+    return None
+
+
+def test_generated_stages_compile(tmp_path):
+    """Forward and adjoint kernels of a random 12-qubit circuit with dense
+    3-bit blocks, controls and per-row parameters compile for gfx950; the
+    code object has no packed-FP32 instructions."""
+    from torchquantum_amd import stagejit as J
+    rng = np.random.default_rng(3)
+    ops = random_ops(12, 40, 2, rng)
+    c = _plan(12, 2, ops, fusion_width=3)
+    stages = [s for s in _stages(c) if J.supported(s.words)]
+    assert stages
+    # the stage with the most distinct op kinds
+    def kinds(s):
+        return len({(w[0] & 15, w[0] & ~0xF0) for w in J.StageProgram(s.words).ops})
+    pick = [max(stages, key=kinds)]
+    src = J.PRELUDE + "".join(J.gen_forward(s.words, f"kf{i}") + J.gen_adjoint(s.words, f"ka{i}")
+                              for i, s in enumerate(pick))
+    code = J.compile_source(src)
+    assert len(code) > 1000
+    od = _objdump()
+    if od is None:
+        pytest.skip("llvm-objdump not found")
+    f = tmp_path / "stage.co"
+    f.write_bytes(code)
+    dis = subprocess.run([od, "-d", "--mcpu=gfx950", str(f)], capture_output=True, text=True, check=True).stdout
+    assert "v_fma_f32" in dis or "v_fmac_f32" in dis
+    assert "v_pk_fma_f32" not in dis and "v_pk_mul_f32" not in dis and "v_pk_add_f32" not in dis
+
+
+def test_compile_error_is_reported():
+    from torchquantum_amd import stagejit as J
+    with pytest.raises(RuntimeError, match="compile failed"):
+        J.compile_source('extern "C" __global__ void k() { undefined_thing(); }')
+
+
+def test_supported_needs_full_waves():
+    """Stages with fewer than 64 threads per group (m < 10) stay on the
+    interpreting kernels."""
+    from torchquantum_amd import stagejit as J
+    rng = np.random.default_rng(4)
+    small = _plan(8, 1, random_ops(8, 10, 1, rng))
+    assert not any(J.supported(s.words) for s in _stages(small))
+    big = _plan(11, 1, random_ops(11, 10, 1, rng))
+    assert all(J.supported(s.words) for s in _stages(big))
+
+
+def test_generated_code_follows_program():
+    """Structure of the generated forward code: one LDS round trip per remap
+    op, no selects for uncontrolled ops, renaming (no code) for an X on a
+    register bit without thread-bit controls."""
+    import torchquantum_amd as tq
+    import torchquantum_amd.functional as tqf
+    from torchquantum_amd import engine as E, stagejit as J
+    n = 12
+    q = tq.QuantumDevice(n_wires=n, bsz=1, device="cpu")
+    for w in range(n):
+        tqf.ry(q, w, params=torch.tensor([[0.1 * (w + 1)]]))
+    c = E.Compiled(q._tape, n, 1, tq.EngineOptions(), "cpu")
+    for s in _stages(c):
+        sp = J.StageProgram(s.words)
+        src = J.gen_forward(s.words, "k")
+        n_remap = sum(1 for w in sp.ops if (w[0] & 15) == E.OP_REMAP)
+        assert src.count("__syncthreads();") == 2 * n_remap
+        assert "csel" not in src
+    # a lone X on wire 0 (bit 11, a register bit of the canonical layout):
+    # no instructions, the stores name the loaded values exchanged in pairs
+    q = tq.QuantumDevice(n_wires=n, bsz=1, device="cpu")
+    tqf.paulix(q, 0)
+    c = E.Compiled(q._tape, n, 1, tq.EngineOptions(merge_1q=False), "cpu")
+    (s,) = _stages(c)
+    sp = J.StageProgram(s.words)
+    assert [w[0] & 15 for w in sp.ops] == [E.OP_MAT1]
+    K = sp.ops[0][3]
+    src = J.gen_forward(s.words, "k")
+    assert "const float2 v" not in src
+    for j in range(16):
+        assert f"dst[{sp.roff[j]}] = i{j ^ (1 << K)};" in src

@@ -25,6 +25,7 @@ SOURCES = [
     "tqsv_batch.hip",
     "tqsv_mlp.hip",
     "tqsv_sample.hip",
+    "tqsv_jit.hip",
 ]
 
 ARCH = os.environ.get("TQSV_ARCH", "gfx950")
@@ -76,7 +77,8 @@ def build(verbose: bool = False, jobs: int = 6) -> Path:
         if failed:
             raise RuntimeError("hipcc failed:\n" + "\n".join(failed))
         tmp_lib = tmp / "libtqsv.so"
-        cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp_lib)]
+        cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp_lib),
+               "-lhiprtc"]
         r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
         if r.returncode != 0:
             raise RuntimeError("link failed:\n" + r.stdout.decode(errors="replace"))

@@ -917,7 +917,8 @@ TQSV_API int tqsv_mg_to_grad(hipStream_t s, float* grad, int64_t grad_stride, in
   return check_launch("tqsv_mg_to_grad");
 }
 
-static int stage_launch_shape(const int* prog_host, int n, int64_t bsz, int grid_cap, int& T, int64_t& n_groups,
+// Launch shape of a stage program (also used by the run-time compiled stages).
+int stage_launch_shape(const int* prog_host, int n, int64_t bsz, int grid_cap, int& T, int64_t& n_groups,
                               int& grid, int64_t& gpw, size_t& shm) {
   const int m = prog_host[H_M];
   T = prog_host[H_T];

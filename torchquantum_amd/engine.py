@@ -65,6 +65,12 @@ class EngineOptions:
     grid_cap:    workgroups per stage launch (0 = auto).
     adjoint_mg:  adjoint sweep accumulates 2x2 matrix gradients for ops with
                  >= 2 trainable parameters (else one inner product per parameter).
+    jit:         "auto" | "on" | "off": run stages as kernels compiled at run
+                 time from their programs (straight-line code, hipRTC) instead
+                 of the program-interpreting kernels.  "auto" compiles for
+                 states of >= 2^22 amplitudes (bsz * 2^n), where the one-time
+                 compile (about 2 s per stage and direction, on host threads)
+                 is repaid within a few passes.
     """
     mode: str = "fused"
     stage_bits: int = 0
@@ -73,13 +79,15 @@ class EngineOptions:
     fusion_width: int = 1
     grid_cap: int = 0
     adjoint_mg: bool = True
+    jit: str = "auto"
 
     def key(self):
         return (self.mode, self.stage_bits, self.low_bits, self.merge_1q, self.fusion_width, self.grid_cap,
-                self.adjoint_mg)
+                self.adjoint_mg, self.jit)
 
 
 DEFAULT_OPTIONS = EngineOptions()
+JIT_MIN_AMPS = 1 << 22
 
 
 # ----------------------------------------------------------------------------
@@ -1017,6 +1025,7 @@ class Compiled:
         self.perms: Dict[int, np.ndarray] = {}
         self._fill = None
         self._gv_layout: Optional[_GVLayout] = None
+        self._jit: Dict[str, Optional[Dict[int, int]]] = {}
         self.mg_items = np.zeros(0, dtype=MG_ITEM)
         self.grad_width = max(self.n_slots, 1)
         if opts.mode == "fused" and n >= R_BITS:
@@ -1087,6 +1096,37 @@ class Compiled:
         if run:
             out.append(("gates", np.stack(run)))
         return out
+
+    def jit_kernels(self, kind: str) -> Optional[Dict[int, int]]:
+        """Compiled stage kernels of this plan ({stage word offset: kernel
+        handle}; kind "f" forward, "a" adjoint), or None when stages run on
+        the interpreting kernels (option off, small state, nothing eligible)."""
+        if kind in self._jit:
+            return self._jit[kind]
+        table = None
+        mode = self.opts.jit
+        if mode not in ("auto", "on", "off"):
+            raise ValueError(f"EngineOptions.jit must be 'auto', 'on' or 'off', not {mode!r}")
+        want = mode == "on" or (mode == "auto" and (self.bsz << self.n) >= JIT_MIN_AMPS)
+        if want and torch.device(self.device).type == "cuda" and self.prog_host is not None:
+            from . import stagejit
+            stages = [(base, st.words) for kind_, (base, st) in
+                      [(k, p) for k, p in self.steps if k == "stage"] if stagejit.supported(st.words)]
+            if stages:
+                dev = torch.device(self.device)
+                index = dev.index if dev.index is not None else torch.cuda.current_device()
+                try:
+                    funcs = stagejit.get_kernels([w for _, w in stages], kind, index)
+                except (RuntimeError, OSError) as e:
+                    if mode == "on":
+                        raise
+                    import warnings
+                    warnings.warn(f"run-time stage compile unavailable, using the interpreting kernels: {e}")
+                    funcs = None
+                if funcs is not None:
+                    table = {base: f for (base, _), f in zip(stages, funcs)}
+        self._jit[kind] = table
+        return table
 
     # ---------------------------------------------------------------- values
     def pool(self, tape: Sequence[TapeOp], with_grad: bool, with_adj: bool) -> torch.Tensor:
@@ -1320,14 +1360,21 @@ def forward(c: Compiled, state_in: torch.Tensor, pool: torch.Tensor, init_zero: 
     if init_zero and (not c.steps or c.steps[0][0] != "stage"):
         N.call("tqsv_init_basis", stream, out.data_ptr(), bsz, n, 0)
         fresh = True
+    jit = c.jit_kernels("f") if c.prog_host is not None else None
     for kind, payload in c.fwd_steps:
         if kind == "stage":
             base, stg = payload
             src = out if fresh else cur_in
             hp = c.prog_host[base:]
-            N.call("tqsv_stage_forward", stream, out.data_ptr(), (src.data_ptr() if src is not None else None),
-                   bsz, n, hp.ctypes.data_as(N.c_int_p), c.prog_dev.data_ptr() + 4 * base, pool.data_ptr(),
-                   1 if (init_zero and not fresh) else 0, c.opts.grid_cap)
+            f = jit.get(base) if jit else None
+            if f is not None:
+                N.call("tqsv_jit_stage_forward", stream, f, out.data_ptr(),
+                       (src.data_ptr() if src is not None else None), bsz, n, hp.ctypes.data_as(N.c_int_p),
+                       pool.data_ptr(), 1 if (init_zero and not fresh) else 0, c.opts.grid_cap)
+            else:
+                N.call("tqsv_stage_forward", stream, out.data_ptr(), (src.data_ptr() if src is not None else None),
+                       bsz, n, hp.ctypes.data_as(N.c_int_p), c.prog_dev.data_ptr() + 4 * base, pool.data_ptr(),
+                       1 if (init_zero and not fresh) else 0, c.opts.grid_cap)
             fresh = True
         else:
             if not fresh:
@@ -1353,13 +1400,19 @@ def adjoint(c: Compiled, psi: torch.Tensor, lam: torch.Tensor, pool: torch.Tenso
     grad = torch.zeros((bsz, c.grad_width), dtype=torch.float32, device=c.device)
     stream = N.stream_of(psi)
     gstride = grad.shape[1]
+    jit = c.jit_kernels("a") if c.prog_host is not None else None
     for kind, payload in reversed(c.steps):
         if kind == "stage":
             base, stg = payload
             hp = c.prog_host[base:]
-            N.call("tqsv_stage_adjoint", stream, psi.data_ptr(), lam.data_ptr(), bsz, n,
-                   hp.ctypes.data_as(N.c_int_p), c.prog_dev.data_ptr() + 4 * base, pool.data_ptr(),
-                   N.fptr(grad), gstride, c.opts.grid_cap)
+            f = jit.get(base) if jit else None
+            if f is not None:
+                N.call("tqsv_jit_stage_adjoint", stream, f, psi.data_ptr(), lam.data_ptr(), bsz, n,
+                       hp.ctypes.data_as(N.c_int_p), pool.data_ptr(), N.fptr(grad), gstride, c.opts.grid_cap)
+            else:
+                N.call("tqsv_stage_adjoint", stream, psi.data_ptr(), lam.data_ptr(), bsz, n,
+                       hp.ctypes.data_as(N.c_int_p), c.prog_dev.data_ptr() + 4 * base, pool.data_ptr(),
+                       N.fptr(grad), gstride, c.opts.grid_cap)
             continue
         i = payload
         k = c.kops[i]

@@ -1,0 +1,802 @@
+"""Run-time compiled stage kernels: straight-line HIP code per stage program.
+
+The interpreter kernels (``k_stage_fwd`` / ``k_stage_adj`` in
+csrc/tqsv_stage.hip) decode every op of a stage program at run time.  This
+module turns one stage program -- the same int32 words the planner builds in
+``engine.encode_stage`` -- into HIP source in which every op is unrolled code
+with its register bits, controls and matrix offsets as constants, compiles it
+for gfx950 with hipRTC (``tqsv_jit_compile``) and launches it through
+``tqsv_jit_stage_forward`` / ``tqsv_jit_stage_adjoint``.
+
+What the generated code computes is what the interpreter computes, op for op
+(the per-gate bmm chain of torchquantum/functional/gate_wrapper.py:96-140 and,
+for the adjoint, its autograd backward :127-136).  What changes is how:
+
+* the slot -> variable assignment is tracked while generating, so an
+  exchange (X / CNOT on a register bit with no thread-bit control) is a pure
+  renaming and costs no instruction;
+* a control on thread bits selects the identity matrix per lane once per op
+  (8 selects) instead of selecting every result (4 per amplitude pair);
+* register-bit controls drop the pairs that fail them at generation time;
+* the LDS addresses of every remap are hoisted out of the group loop.
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import os
+import threading
+from concurrent.futures import ThreadPoolExecutor
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from .engine import (H_NOPS, H_M, H_T, H_NNL, H_OFF_OPS, H_NGP, H_OFF_GSLOT, H_POS, OP_MAT1, OP_MAT2, OP_DIAG,
+                     OP_MATK, OP_REMAP, F_BATCHED, F_NOCTRL, F_REAL, F_XPERM, F_SIGN, F_MG, R_BITS)
+
+NS = 1 << R_BITS
+MIN_T = 6            # compiled stages need full 64-lane waves
+KERNELS_PER_SOURCE = 4
+
+PRELUDE = r"""
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 acc) {
+  acc.x = fmaf(a.x, b.x, acc.x);
+  acc.x = fmaf(-a.y, b.y, acc.x);
+  acc.y = fmaf(a.x, b.y, acc.y);
+  acc.y = fmaf(a.y, b.x, acc.y);
+  return acc;
+}
+__device__ __forceinline__ float2 cfmac(float2 a, float2 b, float2 acc) {
+  acc.x = fmaf(a.x, b.x, acc.x);
+  acc.x = fmaf(a.y, b.y, acc.x);
+  acc.y = fmaf(a.x, b.y, acc.y);
+  acc.y = fmaf(-a.y, b.x, acc.y);
+  return acc;
+}
+__device__ __forceinline__ float redot(float2 a, float2 b) { return fmaf(a.x, b.x, a.y * b.y); }
+__device__ __forceinline__ float2 csel(bool c, float2 a, float2 b) {
+  return make_float2(c ? a.x : b.x, c ? a.y : b.y);
+}
+// NV values per lane -> every lane holds the wave total of value index
+// bits (5, 4, 3)[: log2 NV] of its lane id (halving exchanges, then a tree).
+template <int NV>
+__device__ __forceinline__ float wave_reduce_t(const float (&v)[NV], int lane) {
+  float a[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) a[i] = v[i];
+  int cnt = NV;
+  int sh = 32;
+#pragma unroll
+  for (int step = 0; step < 3; ++step) {
+    if (cnt == 1) break;
+    const int half = cnt >> 1;
+    const bool hi = lane & sh;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i < half) {
+        const float keep = hi ? a[i + half] : a[i];
+        const float send = hi ? a[i] : a[i + half];
+        a[i] = keep + __shfl_xor(send, sh, 64);
+      }
+    }
+    cnt = half;
+    sh >>= 1;
+  }
+  float c = a[0];
+  for (int o = sh; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+  return c;
+}
+"""
+
+ONE = "make_float2(1.f, 0.f)"
+ZERO = "make_float2(0.f, 0.f)"
+
+
+def swz(l: int) -> int:
+    """LDS index swizzle of the interpreter (XOR-linear)."""
+    return l ^ ((l >> 5) & 31) ^ ((l >> 10) & 31)
+
+
+class StageProgram:
+    """Decoded stage words (format: csrc/tqsv_stage.hip)."""
+
+    def __init__(self, words: np.ndarray):
+        h = [int(x) for x in np.asarray(words, dtype=np.int64)]
+        self.words = h
+        self.n_ops, self.m, self.T, self.nnl = h[H_NOPS], h[H_M], h[H_T], h[H_NNL]
+        self.n = self.m + self.nnl
+        T, m, n = self.T, self.m, self.n
+        self.tpos = h[H_POS:H_POS + T]
+        self.rpos = h[H_POS + T:H_POS + m]
+        self.nlpos = h[H_POS + m:H_POS + n]
+        off = h[H_OFF_OPS]
+        self.ops = [h[off + 8 * i:off + 8 * i + 8] for i in range(self.n_ops)]
+        self.ngp = h[H_NGP]
+        self.gslots = h[h[H_OFF_GSLOT]:h[H_OFF_GSLOT] + self.ngp]
+        self.roff = []
+        for j in range(NS):
+            o = 0
+            for k in range(R_BITS):
+                if (j >> k) & 1:
+                    o |= 1 << self.rpos[k]
+            self.roff.append(o)
+
+    def remap_layouts(self, w):
+        rec = self.words[w[7]:w[7] + 2 * (self.T + R_BITS)]
+        k = self.T + R_BITS
+        return rec[:k], rec[k:]
+
+
+def supported(words: np.ndarray) -> bool:
+    """Whether a stage program can be compiled (full waves, no register
+    controls on dense k-bit ops)."""
+    sp = StageProgram(words)
+    if sp.T < MIN_T:
+        return False
+    for w in sp.ops:
+        if (w[0] & 15) == OP_MATK and (w[6] & 255):
+            return False
+    return True
+
+
+class _Gen:
+    def __init__(self, sp: StageProgram):
+        self.sp = sp
+        self.lines: List[str] = []
+        self.k = 0
+        self.ind = 2
+        self.addr: Dict[tuple, str] = {}     # thread layout -> hoisted LDS index variable
+
+    def line(self, s: str):
+        self.lines.append("  " * self.ind + s)
+
+    def var(self, expr: str, ty: str = "float2") -> str:
+        name = f"v{self.k}"
+        self.k += 1
+        self.line(f"const {ty} {name} = {expr};")
+        return name
+
+    # ---- pieces shared by both directions
+    def mat_base(self, w) -> str:
+        if w[0] & F_BATCHED:
+            return self.var(f"mats + {w[1]} + (long long)row * {w[2]}", "float2*")
+        return self.var(f"mats + {w[1]}", "float2*")
+
+    def thread_ok(self, w) -> Optional[str]:
+        tm, tv = w[4], w[5]
+        if tm == 0:
+            return None
+        return self.var(f"(tid & {tm}) == {tv}", "bool")
+
+    @staticmethod
+    def reg_ctrl(w):
+        return w[6] & 255, (w[6] >> 8) & 255
+
+    def select_identity(self, tk: Optional[str], u: List[str], d: int) -> List[str]:
+        if tk is None:
+            return u
+        out = []
+        for q, x in enumerate(u):
+            r, c = divmod(q, d)
+            out.append(self.var(f"csel({tk}, {x}, {ONE if r == c else ZERO})"))
+        return out
+
+    def lane_index(self, layout: Sequence[int]) -> str:
+        """Hoisted LDS index of this thread's part of a layout."""
+        key = tuple(layout[:self.sp.T])
+        return self.addr.setdefault(key, f"ra{len(self.addr)}")
+
+    def hoisted(self) -> List[str]:
+        out = []
+        for key, name in self.addr.items():
+            terms = [f"((tid & {1 << i}) ? {swz(int(key[i]))}u : 0u)" for i in range(len(key))]
+            out.append(f"  const unsigned {name} = {' ^ '.join(terms) if terms else '0u'};")
+        return out
+
+    def remap(self, a: List[str], frm: Sequence[int], to: Sequence[int], lds: str = "lds") -> List[str]:
+        T = self.sp.T
+        ra, rb = self.lane_index(frm), self.lane_index(to)
+        self.line("__syncthreads();")
+        for j in range(NS):
+            lr = 0
+            for k in range(R_BITS):
+                if (j >> k) & 1:
+                    lr |= frm[T + k]
+            self.line(f"{lds}[{ra} ^ {swz(lr)}u] = {a[j]};")
+        self.line("__syncthreads();")
+        out = []
+        for j in range(NS):
+            lr = 0
+            for k in range(R_BITS):
+                if (j >> k) & 1:
+                    lr |= to[T + k]
+            out.append(self.var(f"{lds}[{rb} ^ {swz(lr)}u]"))
+        return out
+
+    # ---- forward ops (k_stage_fwd's fwd_op)
+    def f_xperm(self, w, a: List[str]):
+        K = w[3]
+        rm, rv = self.reg_ctrl(w)
+        tk = self.thread_ok(w)
+        for j in range(NS):
+            if j & (1 << K) or (j & rm) != rv:
+                continue
+            j1 = j | (1 << K)
+            if tk is None:
+                a[j], a[j1] = a[j1], a[j]
+            else:
+                x0, x1 = a[j], a[j1]
+                a[j] = self.var(f"csel({tk}, {x1}, {x0})")
+                a[j1] = self.var(f"csel({tk}, {x0}, {x1})")
+
+    def f_mat1(self, w, a: List[str]):
+        K = w[3]
+        rm, rv = self.reg_ctrl(w)
+        M = self.mat_base(w)
+        if w[0] & F_REAL:          # uncontrolled real 2x2 (RY, H ...)
+            r = [self.var(f"{M}[{i}].x", "float") for i in range(4)]
+            for j in range(NS):
+                if j & (1 << K):
+                    continue
+                j1 = j | (1 << K)
+                x0, x1 = a[j], a[j1]
+                a[j] = self.var(f"make_float2(fmaf({r[0]}, {x0}.x, {r[1]} * {x1}.x), "
+                                f"fmaf({r[0]}, {x0}.y, {r[1]} * {x1}.y))")
+                a[j1] = self.var(f"make_float2(fmaf({r[2]}, {x0}.x, {r[3]} * {x1}.x), "
+                                 f"fmaf({r[2]}, {x0}.y, {r[3]} * {x1}.y))")
+            return
+        u = [self.var(f"{M}[{i}]") for i in range(4)]
+        u = self.select_identity(self.thread_ok(w), u, 2)
+        for j in range(NS):
+            if j & (1 << K) or (j & rm) != rv:
+                continue
+            j1 = j | (1 << K)
+            x0, x1 = a[j], a[j1]
+            a[j] = self.var(f"cfma({u[1]}, {x1}, cmul({u[0]}, {x0}))")
+            a[j1] = self.var(f"cfma({u[3]}, {x1}, cmul({u[2]}, {x0}))")
+
+    def f_sign(self, w, a: List[str]):
+        rm, rv = self.reg_ctrl(w)
+        tk = self.thread_ok(w)
+        for j in range(NS):
+            if (j & rm) != rv:
+                continue
+            x = a[j]
+            if tk is None:
+                a[j] = self.var(f"make_float2(-{x}.x, -{x}.y)")
+            else:
+                a[j] = self.var(f"make_float2({tk} ? -{x}.x : {x}.x, {tk} ? -{x}.y : {x}.y)")
+
+    def diag_factors(self, w, M: str, tk: Optional[str], conj_unused=False):
+        """(fA, fB): the phase for amplitudes whose register-bit parity is even
+        / odd, per lane (thread-bit parity folded in; identity where the
+        thread-bit controls fail)."""
+        tpm = w[3]
+        p0, p1 = self.var(f"{M}[0]"), self.var(f"{M}[1]")
+        if tpm:
+            tp = self.var(f"(__popc(tid & {tpm}) & 1) != 0", "bool")
+            fA = self.var(f"{tp} ? {p1} : {p0}")
+            fB = self.var(f"{tp} ? {p0} : {p1}")
+        else:
+            tp, fA, fB = None, p0, p1
+        if tk is not None:
+            fA = self.var(f"csel({tk}, {fA}, {ONE})")
+            fB = self.var(f"csel({tk}, {fB}, {ONE})")
+        return tp, fA, fB
+
+    def f_diag(self, w, a: List[str]):
+        rm, rv = self.reg_ctrl(w)
+        rpm = (w[6] >> 16) & 255
+        M = self.mat_base(w)
+        _, fA, fB = self.diag_factors(w, M, self.thread_ok(w))
+        for j in range(NS):
+            if (j & rm) != rv:
+                continue
+            f = fB if bin(j & rpm).count("1") & 1 else fA
+            a[j] = self.var(f"cmul({f}, {a[j]})")
+
+    @staticmethod
+    def quads(KH: int, KL: int):
+        for j in range(NS):
+            if j & ((1 << KH) | (1 << KL)):
+                continue
+            yield j, [j, j | (1 << KL), j | (1 << KH), j | (1 << KH) | (1 << KL)]
+
+    def f_mat2(self, w, a: List[str]):
+        KH, KL = w[3] & 15, (w[3] >> 4) & 15
+        rm, rv = self.reg_ctrl(w)
+        M = self.mat_base(w)
+        u = [self.var(f"{M}[{q}]") for q in range(16)]
+        tk = self.thread_ok(w)
+        for j, idx in self.quads(KH, KL):
+            if (j & rm) != rv:
+                continue
+            x = [a[i] for i in idx]
+            ys = []
+            for r in range(4):
+                e = f"cmul({u[4 * r]}, {x[0]})"
+                for c in range(1, 4):
+                    e = f"cfma({u[4 * r + c]}, {x[c]}, {e})"
+                y = self.var(e)
+                if tk is not None:
+                    y = self.var(f"csel({tk}, {y}, {x[r]})")
+                ys.append(y)
+            for r in range(4):
+                a[idx[r]] = ys[r]
+
+    @staticmethod
+    def matk_slots(KB: int, EX: int):
+        bases = [0] if KB == 4 else [0, 1 << EX]
+        for base in bases:
+            slots = []
+            for r in range(1 << KB):
+                s, q = base, 0
+                for k in range(R_BITS):
+                    if KB == 3 and k == EX:
+                        continue
+                    if (r >> q) & 1:
+                        s |= 1 << k
+                    q += 1
+                slots.append(s)
+            yield slots
+
+    def f_matk(self, w, a: List[str], adjoint: bool = False):
+        KB = 4 if w[3] == 4 else 3
+        EX = w[3] if KB == 3 else 0
+        D = 1 << KB
+        M = self.mat_base(w)
+        tk = self.thread_ok(w)
+        for slots in self.matk_slots(KB, EX):
+            x = [a[s] for s in slots]
+            ys = []
+            for r in range(D):
+                if adjoint:     # U^dagger: conj(M[c][r])
+                    e = f"cmulc({M}[{r}], {x[0]})"
+                    for c in range(1, D):
+                        e = f"cfmac({M}[{c * D + r}], {x[c]}, {e})"
+                else:
+                    e = f"cmul({M}[{r * D}], {x[0]})"
+                    for c in range(1, D):
+                        e = f"cfma({M}[{r * D + c}], {x[c]}, {e})"
+                y = self.var(e)
+                if tk is not None:
+                    y = self.var(f"csel({tk}, {y}, {x[r]})")
+                ys.append(y)
+            for r in range(D):
+                a[slots[r]] = ys[r]
+
+    def fwd_op(self, w, a: List[str]):
+        t = w[0] & 15
+        if t == OP_MAT1:
+            if w[0] & F_XPERM:
+                self.f_xperm(w, a)
+            else:
+                self.f_mat1(w, a)
+        elif t == OP_DIAG:
+            if w[0] & F_SIGN:
+                self.f_sign(w, a)
+            else:
+                self.f_diag(w, a)
+        elif t == OP_MAT2:
+            self.f_mat2(w, a)
+        elif t == OP_MATK:
+            self.f_matk(w, a)
+        else:
+            raise ValueError(f"unknown stage op {t}")
+
+    # ---- adjoint ops (k_stage_adj's bwd_op); each returns per-lane partials
+    def acc(self, cur: Optional[str], term: str) -> str:
+        return self.var(term if cur is None else f"{cur} + {term}", "float")
+
+    def lane_gate(self, tk: Optional[str], g: List[Optional[str]]) -> List[str]:
+        out = []
+        for x in g:
+            if x is None:
+                out.append("0.f")
+            elif tk is None:
+                out.append(x)
+            else:
+                out.append(self.var(f"{tk} ? {x} : 0.f", "float"))
+        return out
+
+    def b_mat1(self, w, p: List[str], l: List[str], P: int) -> List[str]:
+        K = w[3]
+        rm, rv = self.reg_ctrl(w)
+        M = self.mat_base(w)
+        tk = self.thread_ok(w)
+        u = self.select_identity(tk, [self.var(f"{M}[{i}]") for i in range(4)], 2)
+        Dm = [[f"{M}[{4 * (q + 1) + i}]" for i in range(4)] for q in range(P)]
+        g: List[Optional[str]] = [None] * P
+        for j in range(NS):
+            if j & (1 << K) or (j & rm) != rv:
+                continue
+            j1 = j | (1 << K)
+            x0, x1 = p[j], p[j1]
+            y0 = self.var(f"cfmac({u[2]}, {x1}, cmulc({u[0]}, {x0}))")
+            y1 = self.var(f"cfmac({u[3]}, {x1}, cmulc({u[1]}, {x0}))")
+            l0, l1 = l[j], l[j1]
+            for q in range(P):
+                D = Dm[q]
+                v0 = self.var(f"cfma({D[1]}, {y1}, cmul({D[0]}, {y0}))")
+                v1 = self.var(f"cfma({D[3]}, {y1}, cmul({D[2]}, {y0}))")
+                g[q] = self.acc(g[q], f"(redot({l0}, {v0}) + redot({l1}, {v1}))")
+            n0 = self.var(f"cfmac({u[2]}, {l1}, cmulc({u[0]}, {l0}))")
+            n1 = self.var(f"cfmac({u[3]}, {l1}, cmulc({u[1]}, {l0}))")
+            p[j], p[j1], l[j], l[j1] = y0, y1, n0, n1
+        return self.lane_gate(tk, g)
+
+    def b_mat1_mg(self, w, p: List[str], l: List[str]) -> List[str]:
+        K = w[3]
+        nc = bool(w[0] & F_NOCTRL)
+        rm, rv = (0, 0) if nc else self.reg_ctrl(w)
+        M = self.mat_base(w)
+        tk = None if nc else self.thread_ok(w)
+        u = self.select_identity(tk, [self.var(f"{M}[{i}]") for i in range(4)], 2)
+        m = [None] * 4     # m00, m01, m10, m11
+        for j in range(NS):
+            if j & (1 << K) or (j & rm) != rv:
+                continue
+            j1 = j | (1 << K)
+            x0, x1 = p[j], p[j1]
+            y0 = self.var(f"cfmac({u[2]}, {x1}, cmulc({u[0]}, {x0}))")
+            y1 = self.var(f"cfmac({u[3]}, {x1}, cmulc({u[1]}, {x0}))")
+            l0, l1 = l[j], l[j1]
+            for idx, (yy, ll) in enumerate(((y0, l0), (y1, l0), (y0, l1), (y1, l1))):
+                m[idx] = self.var(f"cfmac({yy}, {ll}, {m[idx] if m[idx] else ZERO})")
+            n0 = self.var(f"cfmac({u[2]}, {l1}, cmulc({u[0]}, {l0}))")
+            n1 = self.var(f"cfmac({u[3]}, {l1}, cmulc({u[1]}, {l0}))")
+            p[j], p[j1], l[j], l[j1] = y0, y1, n0, n1
+        g: List[Optional[str]] = []
+        for x in m:
+            if x is None:
+                g += [None, None]
+            else:
+                g += [f"{x}.x", f"{x}.y"]
+        return self.lane_gate(tk, g)
+
+    def b_mat2(self, w, p: List[str], l: List[str], P: int) -> List[str]:
+        KH, KL = w[3] & 15, (w[3] >> 4) & 15
+        rm, rv = self.reg_ctrl(w)
+        M = self.mat_base(w)
+        tk = self.thread_ok(w)
+        u = self.select_identity(tk, [self.var(f"{M}[{q}]") for q in range(16)], 4)
+        g: List[Optional[str]] = [None] * P
+        for j, idx in self.quads(KH, KL):
+            if (j & rm) != rv:
+                continue
+            x = [p[i] for i in idx]
+            lm = [l[i] for i in idx]
+            y = []
+            for r in range(4):
+                e = f"cmulc({u[r]}, {x[0]})"
+                for c in range(1, 4):
+                    e = f"cfmac({u[4 * c + r]}, {x[c]}, {e})"
+                y.append(self.var(e))
+            for q in range(P):
+                base = 16 * (q + 1)
+                t = None
+                for r in range(4):
+                    e = f"cmul({M}[{base + 4 * r}], {y[0]})"
+                    for c in range(1, 4):
+                        e = f"cfma({M}[{base + 4 * r + c}], {y[c]}, {e})"
+                    v = self.var(e)
+                    t = self.acc(t, f"redot({lm[r]}, {v})")
+                g[q] = self.acc(g[q], t)
+            for r in range(4):
+                e = f"cmulc({u[r]}, {lm[0]})"
+                for c in range(1, 4):
+                    e = f"cfmac({u[4 * c + r]}, {lm[c]}, {e})"
+                l[idx[r]] = self.var(e)
+                p[idx[r]] = y[r]
+        return self.lane_gate(tk, g)
+
+    def b_diag(self, w, p: List[str], l: List[str], P: int) -> List[str]:
+        rm, rv = self.reg_ctrl(w)
+        rpm = (w[6] >> 16) & 255
+        M = self.mat_base(w)
+        tk = self.thread_ok(w)
+        tp, fA, fB = self.diag_factors(w, M, tk)
+        dA, dB = [], []
+        for q in range(P):
+            e0, e1 = f"{M}[{2 * (q + 1)}]", f"{M}[{2 * (q + 1) + 1}]"
+            if tp is None:
+                dA.append(self.var(e0))
+                dB.append(self.var(e1))
+            else:
+                dA.append(self.var(f"{tp} ? {e1} : {e0}"))
+                dB.append(self.var(f"{tp} ? {e0} : {e1}"))
+        g: List[Optional[str]] = [None] * P
+        for j in range(NS):
+            if (j & rm) != rv:
+                continue
+            odd = bin(j & rpm).count("1") & 1
+            f = fB if odd else fA
+            y = self.var(f"cmulc({f}, {p[j]})")
+            lj = l[j]
+            for q in range(P):
+                d = dB[q] if odd else dA[q]
+                g[q] = self.acc(g[q], f"redot({lj}, cmul({d}, {y}))")
+            p[j] = y
+            l[j] = self.var(f"cmulc({f}, {lj})")
+        return self.lane_gate(tk, g)
+
+    def bwd_op(self, w, p: List[str], l: List[str]) -> Optional[List[str]]:
+        t = w[0] & 15
+        fl = w[0]
+        P = ((fl >> 4) & 15) if w[7] >= 0 else 0
+        if t == OP_MAT1:
+            if fl & F_XPERM:
+                self.f_xperm(w, p)
+                self.f_xperm(w, l)
+                return None
+            if fl & F_MG:
+                return self.b_mat1_mg(w, p, l)
+            return self.b_mat1(w, p, l, P)
+        if t == OP_DIAG:
+            if fl & F_SIGN:
+                self.f_sign(w, p)
+                self.f_sign(w, l)
+                return None
+            return self.b_diag(w, p, l, P)
+        if t == OP_MAT2:
+            return self.b_mat2(w, p, l, P)
+        if t == OP_MATK:
+            self.f_matk(w, p, adjoint=True)
+            self.f_matk(w, l, adjoint=True)
+            return None
+        raise ValueError(f"unknown stage op {t}")
+
+    def reduce(self, w, g: List[str]):
+        Pp = 8 if (w[0] & F_MG) else ((w[0] >> 4) & 15)
+        if Pp > 4:
+            nv, sh, cond = 8, 3, "(lane & 7) == 0"
+        elif Pp > 2:
+            nv, sh, cond = 4, 4, f"(lane & 15) == 0 && (lane >> 4) < {Pp}"
+        elif Pp == 2:
+            nv, sh, cond = 2, 5, "(lane & 31) == 0"
+        else:
+            nv, sh, cond = 1, 6, "lane == 0"
+        # wave_reduce_t of the interpreter, unrolled on named values: log2(nv)
+        # halving exchanges (each lane keeps one half, forwards the other),
+        # then a shuffle tree on the one value left
+        a = (list(g) + ["0.f"] * nv)[:nv]
+        cnt, s = nv, 32
+        while cnt > 1:
+            half = cnt >> 1
+            hi = self.var(f"(lane & {s}) != 0", "bool")
+            nxt = []
+            for i in range(half):
+                send = self.var(f"{hi} ? {a[i]} : {a[i + half]}", "float")
+                keep = f"({hi} ? {a[i + half]} : {a[i]})"
+                nxt.append(self.var(f"{keep} + __shfl_xor({send}, {s}, 64)", "float"))
+            a, cnt, s = nxt, half, s >> 1
+        c = a[0]
+        while s >= 1:
+            c = self.var(f"{c} + __shfl_xor({c}, {s}, 64)", "float")
+            s >>= 1
+        self.line(f"if ({cond}) atomicAdd(&gacc[{w[7]} + (lane >> {sh})], {c});")
+
+
+def _head(sp: StageProgram, g: _Gen) -> List[str]:
+    """Group-loop prologue: row, group offset, base index."""
+    terms = [f"((unsigned)((rest >> {i}) & 1ull) << {sp.nlpos[i]})" for i in range(sp.nnl)]
+    return [
+        f"    const unsigned long long row = (unsigned long long)g >> {sp.nnl};",
+        f"    const unsigned long long rest = (unsigned long long)g & {(1 << sp.nnl) - 1}ull;",
+        f"    const unsigned goff = {' | '.join(terms) if terms else '0u'};",
+        f"    const unsigned long long base = (row << {sp.n}) + goff + toff;",
+    ]
+
+
+def _toff(sp: StageProgram) -> str:
+    terms = [f"((((unsigned)tid >> {i}) & 1u) << {sp.tpos[i]})" for i in range(sp.T)]
+    return " | ".join(terms) if terms else "0u"
+
+
+def gen_forward(words: np.ndarray, name: str) -> str:
+    """HIP source of one forward stage kernel (k_stage_fwd semantics)."""
+    sp = StageProgram(words)
+    if sp.T < MIN_T:
+        raise ValueError("compiled stages need T >= 6")
+    g = _Gen(sp)
+    a = [f"i{j}" for j in range(NS)]
+    has_remap = False
+    for w in sp.ops:
+        if (w[0] & 15) == OP_REMAP:
+            frm, to = sp.remap_layouts(w)
+            a = g.remap(a, frm, to)
+            has_remap = True
+        else:
+            g.fwd_op(w, a)
+    body = g.lines
+    nt = 1 << sp.T
+    src = [f'extern "C" __global__ void __launch_bounds__({nt}) {name}(float2* __restrict__ out, '
+           "const float2* __restrict__ in, const float2* __restrict__ mats, int init_zero, long long n_groups, "
+           "long long gpw) {"]
+    if has_remap:
+        src.append(f"  __shared__ float2 lds[{1 << sp.m}];")
+    src.append("  const int tid = threadIdx.x;")
+    src.append(f"  const unsigned toff = {_toff(sp)};")
+    src += g.hoisted()
+    src += ["  const long long g0 = (long long)blockIdx.x * gpw;",
+            "  long long g1 = g0 + gpw;",
+            "  if (g1 > n_groups) g1 = n_groups;",
+            "  for (long long g = g0; g < g1; ++g) {"]
+    src += _head(sp, g)
+    src.append(f"    float2 {', '.join(f'i{j}' for j in range(NS))};")
+    src.append("    if (init_zero) {")
+    for j in range(NS):
+        src.append(f"      i{j} = make_float2((goff + toff + {sp.roff[j]}u) == 0u ? 1.f : 0.f, 0.f);")
+    src.append("    } else {")
+    src.append("      const float2* src = in + base;")
+    for j in range(NS):
+        src.append(f"      i{j} = src[{sp.roff[j]}];")
+    src.append("    }")
+    src += body
+    src.append("    float2* dst = out + base;")
+    for j in range(NS):
+        src.append(f"    dst[{sp.roff[j]}] = {a[j]};")
+    src += ["  }", "}"]
+    return "\n".join(src) + "\n"
+
+
+def gen_adjoint(words: np.ndarray, name: str) -> str:
+    """HIP source of one adjoint stage kernel (k_stage_adj semantics)."""
+    sp = StageProgram(words)
+    if sp.T < MIN_T:
+        raise ValueError("compiled stages need T >= 6")
+    g = _Gen(sp)
+    p = [f"p{j}" for j in range(NS)]
+    l = [f"l{j}" for j in range(NS)]
+    has_remap = False
+    for w in reversed(sp.ops):
+        if (w[0] & 15) == OP_REMAP:
+            frm, to = sp.remap_layouts(w)
+            p = g.remap(p, to, frm)
+            l = g.remap(l, to, frm)
+            has_remap = True
+            continue
+        parts = g.bwd_op(w, p, l)
+        if w[7] >= 0 and parts is not None:
+            g.reduce(w, parts)
+    nt = 1 << sp.T
+    ngp = sp.ngp
+    src = []
+    if ngp:
+        src.append(f"__constant__ int {name}_gs[{ngp}] = {{{', '.join(str(int(x)) for x in sp.gslots)}}};")
+    src.append(f'extern "C" __global__ void __launch_bounds__({nt}) {name}(float2* __restrict__ psi, '
+               "float2* __restrict__ lam, const float2* __restrict__ mats, float* __restrict__ grad, "
+               "long long grad_stride, long long n_groups, long long gpw) {")
+    if has_remap:
+        src.append(f"  __shared__ float2 lds[{1 << sp.m}];")
+    if ngp:
+        src.append(f"  __shared__ float gacc[{ngp}];")
+        src.append(f"  for (int t = threadIdx.x; t < {ngp}; t += {nt}) gacc[t] = 0.f;")
+    src.append("  const int tid = threadIdx.x;")
+    src.append("  const int lane = tid & 63;")
+    src.append(f"  const unsigned toff = {_toff(sp)};")
+    src += g.hoisted()
+    src += ["  const long long g0 = (long long)blockIdx.x * gpw;",
+            "  long long g1 = g0 + gpw;",
+            "  if (g1 > n_groups) g1 = n_groups;",
+            "  long long cur_row = -1;",
+            "  __syncthreads();",
+            "  for (long long g = g0; g < g1; ++g) {"]
+    src += _head(sp, g)
+    if ngp:
+        src += ["    if ((long long)row != cur_row) {",
+                "      if (cur_row >= 0) {",
+                "        __syncthreads();",
+                f"        for (int t = tid; t < {ngp}; t += {nt}) {{",
+                "          const float v = gacc[t];",
+                f"          if (v != 0.f) atomicAdd(grad + cur_row * grad_stride + {name}_gs[t], v);",
+                "          gacc[t] = 0.f;",
+                "        }",
+                "        __syncthreads();",
+                "      }",
+                "      cur_row = (long long)row;",
+                "    }"]
+    src.append("    const float2* sp_ = psi + base;")
+    src.append("    const float2* sl_ = lam + base;")
+    for j in range(NS):
+        src.append(f"    const float2 p{j} = sp_[{sp.roff[j]}];")
+    for j in range(NS):
+        src.append(f"    const float2 l{j} = sl_[{sp.roff[j]}];")
+    src += g.lines
+    src.append("    float2* dp = psi + base;")
+    src.append("    float2* dl = lam + base;")
+    for j in range(NS):
+        src.append(f"    dp[{sp.roff[j]}] = {p[j]};")
+    for j in range(NS):
+        src.append(f"    dl[{sp.roff[j]}] = {l[j]};")
+    src.append("  }")
+    if ngp:
+        src += ["  if (cur_row >= 0) {",
+                "    __syncthreads();",
+                f"    for (int t = tid; t < {ngp}; t += {nt}) {{",
+                "      const float v = gacc[t];",
+                f"      if (v != 0.f) atomicAdd(grad + cur_row * grad_stride + {name}_gs[t], v);",
+                "    }",
+                "  }"]
+    src.append("}")
+    return "\n".join(src) + "\n"
+
+
+# ----------------------------------------------------------------------------
+# compile + cache
+# ----------------------------------------------------------------------------
+def kernel_name(kind: str, words: np.ndarray) -> str:
+    h = hashlib.sha1(np.ascontiguousarray(words, dtype=np.int32).tobytes()).hexdigest()[:20]
+    return f"tqsv_j{kind}_{h}"
+
+
+def compile_source(src: str) -> bytes:
+    """hipRTC-compile a source (no GPU needed); returns the code object."""
+    from . import _native as N
+    lib = N.lib()
+    size = ctypes.c_int64(0)
+    log = ctypes.create_string_buffer(1 << 16)
+    srcb = src.encode()
+    rc = lib.tqsv_jit_compile(srcb, None, 0, ctypes.byref(size), log, len(log))
+    if rc != 0:
+        raise RuntimeError(f"stage kernel compile failed: {log.value.decode(errors='replace')[:4000]}")
+    buf = ctypes.create_string_buffer(size.value)
+    rc = lib.tqsv_jit_compile(srcb, buf, size.value, ctypes.byref(size), None, 0)
+    if rc != 0:
+        raise RuntimeError(f"stage kernel compile failed: {lib.tqsv_last_error().decode(errors='replace')}")
+    return buf.raw[:size.value]
+
+
+_lock = threading.Lock()
+_funcs: Dict[tuple, int] = {}          # (device index, kernel name) -> hipFunction_t
+_modules: List[int] = []                # loaded modules (kept for the process lifetime)
+
+
+def _jobs() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 4
+    return max(1, min(8, n))
+
+
+def get_kernels(programs: Sequence[np.ndarray], kind: str, device_index: int) -> List[int]:
+    """hipFunction_t handles (as ints) of the compiled stage kernels for the
+    given programs; kind "f" = forward, "a" = adjoint.  Compiles (in parallel
+    host threads) and loads what is not cached yet."""
+    from . import _native as N
+    gen = gen_forward if kind == "f" else gen_adjoint
+    names = [kernel_name(kind, w) for w in programs]
+    with _lock:
+        missing = {}
+        for nm, w in zip(names, programs):
+            if (device_index, nm) not in _funcs and nm not in missing:
+                missing[nm] = w
+        if missing:
+            items = list(missing.items())
+            chunks = [items[i:i + KERNELS_PER_SOURCE] for i in range(0, len(items), KERNELS_PER_SOURCE)]
+            sources = [PRELUDE + "\n".join(gen(w, nm) for nm, w in ch) for ch in chunks]
+            if len(sources) > 1:
+                with ThreadPoolExecutor(max_workers=min(_jobs(), len(sources))) as ex:
+                    codes = list(ex.map(compile_source, sources))
+            else:
+                codes = [compile_source(s) for s in sources]
+            lib = N.lib()
+            for ch, code in zip(chunks, codes):
+                cnames = (ctypes.c_char_p * len(ch))(*[nm.encode() for nm, _ in ch])
+                fptrs = (ctypes.c_void_p * len(ch))()
+                mod = ctypes.c_void_p()
+                buf = ctypes.create_string_buffer(code, len(code))
+                rc = lib.tqsv_jit_load(buf, len(code), len(ch), cnames, ctypes.byref(mod), fptrs)
+                if rc != 0:
+                    raise RuntimeError(f"stage kernel load failed: {lib.tqsv_last_error().decode(errors='replace')}")
+                _modules.append(mod.value)
+                for (nm, _), f in zip(ch, fptrs):
+                    _funcs[(device_index, nm)] = f
+        return [_funcs[(device_index, nm)] for nm in names]
