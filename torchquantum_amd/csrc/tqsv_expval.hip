@@ -11,6 +11,9 @@
 // Reductions: per-thread partial sums -> 64-lane shuffle tree -> LDS across
 // waves -> one partial per (row, chunk, term) -> a second tiny kernel sums
 // the chunks in float64.  No float atomics, so results are deterministic.
+#include <algorithm>
+#include <vector>
+
 #include "tqsv_common.h"
 
 namespace tqsv {
@@ -194,23 +197,24 @@ struct PTerms {
   int nt;
   uint32_t x[EV_MAXT];
   uint32_t z[EV_MAXT];
+  int col[EV_MAXT];   // term column in coef
 };
 
 __global__ __launch_bounds__(EV_THREADS) void k_pauli_apply(float2* __restrict__ out, const float2* __restrict__ in,
                                                             int n, uint64_t total, PTerms pt,
                                                             const float2* __restrict__ coef, int64_t coef_stride,
-                                                            int coef_off, int accumulate) {
+                                                            int accumulate) {
   for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total;
        g += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t row = g >> n;
     const uint32_t j = (uint32_t)(g & ((1ull << n) - 1ull));
     const float2* r = in + (row << n);
-    const float2* cf = coef + row * coef_stride + coef_off;
+    const float2* cf = coef + row * coef_stride;
     float2 acc = accumulate ? out[g] : make_float2(0.f, 0.f);
     for (int t = 0; t < pt.nt; ++t) {
       const uint32_t k = j ^ pt.x[t];
       const float2 v = r[k];
-      const float2 c = cf[t];
+      const float2 c = cf[pt.col[t]];
       const float2 cv = cmul(c, v);
       const bool neg = __popc(k & pt.z[t]) & 1;
       acc.x += neg ? -cv.x : cv.x;
@@ -218,6 +222,246 @@ __global__ __launch_bounds__(EV_THREADS) void k_pauli_apply(float2* __restrict__
     }
     out[g] = acc;
   }
+}
+
+
+// ---- off-diagonal terms grouped by a local bit set -------------------------
+// A "local set" L of m <= 12 bit positions (the 6 lowest always among them,
+// for coalescing) splits each row into groups of 2^m amplitudes that differ
+// only in L.  Every term whose flip mask x lies inside L maps a group onto
+// itself, so one pass that stages each group in LDS serves all of them:
+// per amplitude one LDS read per term instead of one pass over the state per
+// term (expectations) or one uncached gather per term (their backward).
+constexpr int PG_THREADS = 256;
+constexpr int PG_MAXM = 12;
+constexpr int PG_MAXT = 32;     // terms per local set (apply)
+constexpr int PG_MAXT_EV = 16;  // terms per local set (expectations: two accumulators each)
+
+struct PGroup {
+  int m, nnl, nt, lo;        // local bits, non-local bits, terms, identity-mapped low bits
+  int lpos[PG_MAXM];         // local bit i -> bit position (lpos[i] = i for i < lo)
+  int nlpos[32];             // non-local positions, ascending
+  uint32_t xl[PG_MAXT];      // flip mask in local numbering
+  uint32_t zl[PG_MAXT];      // phase mask, local part in local numbering
+  uint32_t zg[PG_MAXT];      // phase mask, non-local part (bit positions)
+  int col[PG_MAXT];          // term column in coef / out
+};
+
+__device__ __forceinline__ uint32_t pg_scatter(const PGroup& g, uint32_t l) {
+  uint32_t o = l & ((1u << g.lo) - 1u);
+  for (int i = g.lo; i < g.m; ++i) o |= ((l >> i) & 1u) << g.lpos[i];
+  return o;
+}
+
+__device__ __forceinline__ uint32_t pg_group_offset(const PGroup& g, uint64_t rest) {
+  uint32_t o = 0;
+  for (int i = 0; i < g.nnl; ++i) o |= (uint32_t)((rest >> i) & 1ull) << g.nlpos[i];
+  return o;
+}
+
+// Stage one group (2^m amplitudes) into LDS; lane l of the group holds local
+// indices l = tid + 256 r.  Returns nothing; buf is complete after the barrier.
+__device__ __forceinline__ void pg_load(const PGroup& g, const float2* __restrict__ rin, float2* buf, uint32_t tsc,
+                                        int tid, uint32_t size) {
+  uint32_t r = 0;
+  for (uint32_t l = tid; l < size; l += PG_THREADS, ++r) buf[l] = rin[tsc | pg_scatter(g, r << 8)];
+}
+
+// out (+)= sum_t coef[row, col_t] (-1)^{popcount((j^x_t) & z_t)} psi_{j ^ x_t}
+// for the terms of one local set; one workgroup walks gpw consecutive groups.
+// Term-outer order: each lane keeps its NR = 2^m / 256 outputs in registers,
+// and a term's coefficient (with the group's non-local phase folded in) is
+// wave-uniform, so a real or imaginary coefficient -- the usual case, i^{nY}
+// times a real weight -- costs two FMA per amplitude.
+template <int NR>
+__global__ __launch_bounds__(PG_THREADS) void k_pauli_group_apply(float2* __restrict__ out,
+                                                                  const float2* __restrict__ in, int n,
+                                                                  PGroup g, const float2* __restrict__ coef,
+                                                                  int64_t coef_stride, int accumulate,
+                                                                  int64_t n_groups, int64_t gpw) {
+  __shared__ float2 buf[NR * PG_THREADS];
+  const int tid = threadIdx.x;
+  const uint32_t size = NR * PG_THREADS;
+  // local index l = tid + 256 r: its bit positions are those of tid | those of r << 8
+  const uint32_t tsc = pg_scatter(g, (uint32_t)tid);
+  uint32_t rsc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) rsc[r] = pg_scatter(g, (uint32_t)r << 8);
+  const int64_t g0 = (int64_t)blockIdx.x * gpw;
+  const int64_t g1 = g0 + gpw < n_groups ? g0 + gpw : n_groups;
+  const uint64_t rmask = (1ull << g.nnl) - 1ull;
+  for (int64_t grp = g0; grp < g1; ++grp) {
+    const uint64_t row = (uint64_t)grp >> g.nnl;
+    const uint32_t goff = pg_group_offset(g, (uint64_t)grp & rmask);
+    const float2* rin = in + (row << n) + goff;
+    float2* rout = out + (row << n) + goff;
+    __syncthreads();   // the previous group's LDS reads are done
+#pragma unroll
+    for (int r = 0; r < NR; ++r) buf[tid + r * PG_THREADS] = rin[tsc | rsc[r]];
+    __syncthreads();
+    float2 acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = accumulate ? rout[tsc | rsc[r]] : make_float2(0.f, 0.f);
+    const float2* cf = coef + row * coef_stride;
+    for (int t = 0; t < g.nt; ++t) {
+      float2 c = cf[g.col[t]];
+      if (__popc(goff & g.zg[t]) & 1) c = make_float2(-c.x, -c.y);
+      const uint32_t xl = g.xl[t], zl = g.zl[t];
+      // source amplitude k = l ^ x with its phase sign folded into the coefficient
+#define TQSV_PG_SRC(r)                                                   \
+  const uint32_t k = ((uint32_t)tid + (uint32_t)(r) * PG_THREADS) ^ xl; \
+  const float2 v = buf[k];                                              \
+  const float sg = (__popc(k & zl) & 1) ? -1.f : 1.f;
+      if (c.y == 0.f) {          // real coefficient (wave-uniform branch)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          TQSV_PG_SRC(r)
+          const float cr = sg * c.x;
+          acc[r].x = fmaf(cr, v.x, acc[r].x);
+          acc[r].y = fmaf(cr, v.y, acc[r].y);
+        }
+      } else if (c.x == 0.f) {   // imaginary coefficient
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          TQSV_PG_SRC(r)
+          const float ci = sg * c.y;
+          acc[r].x = fmaf(-ci, v.y, acc[r].x);
+          acc[r].y = fmaf(ci, v.x, acc[r].y);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          TQSV_PG_SRC(r)
+          acc[r] = cfma(make_float2(sg * c.x, sg * c.y), v, acc[r]);
+        }
+      }
+#undef TQSV_PG_SRC
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) rout[tsc | rsc[r]] = acc[r];
+  }
+}
+
+// Partials of the terms of one local set over the gpw groups of one
+// workgroup (all in one row).  With a = psi_j, b = psi_{j^x}:
+// <P> = Re(i^{nY} S), S = sum_j (-1)^{popcount(j & z)} conj(b) a, so only
+// Re S (nY even) or Im S (nY odd) is accumulated: two FMA per amplitude.
+// Per-lane running sums live in LDS (one slot per term and lane), so the
+// term loop stays a plain loop.  part[blockIdx.x * 2 nt + 2 t] = the partial.
+template <int NR>
+__global__ __launch_bounds__(PG_THREADS) void k_pauli_group_expval(const float2* __restrict__ in, int n,
+                                                                   PGroup g, int64_t gpw,
+                                                                   float* __restrict__ part) {
+  __shared__ float2 buf[NR * PG_THREADS];
+  __shared__ float lacc[PG_MAXT_EV][PG_THREADS];
+  __shared__ float red[PG_THREADS / 64];
+  const int tid = threadIdx.x;
+  const uint32_t tsc = pg_scatter(g, (uint32_t)tid);
+  uint32_t rsc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) rsc[r] = pg_scatter(g, (uint32_t)r << 8);
+  const int64_t g0 = (int64_t)blockIdx.x * gpw;
+  const uint64_t rmask = (1ull << g.nnl) - 1ull;
+  for (int t = 0; t < g.nt; ++t) lacc[t][tid] = 0.f;
+  for (int64_t grp = g0; grp < g0 + gpw; ++grp) {
+    const uint32_t goff = pg_group_offset(g, (uint64_t)grp & rmask);
+    const float2* rin = in + (((uint64_t)grp >> g.nnl) << n) + goff;
+    __syncthreads();
+    float2 a[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      a[r] = rin[tsc | rsc[r]];
+      buf[tid + r * PG_THREADS] = a[r];
+    }
+    __syncthreads();
+    for (int t = 0; t < g.nt; ++t) {
+      const uint32_t xl = g.xl[t], zl = g.zl[t];
+      const bool odd = __popc(xl & zl) & 1;                  // nY odd: Im S
+      float s = 0.f;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const uint32_t l = (uint32_t)tid + (uint32_t)r * PG_THREADS;
+        const float2 b = buf[l ^ xl];
+        const float d = odd ? fmaf(b.x, a[r].y, -b.y * a[r].x) : fmaf(b.x, a[r].x, b.y * a[r].y);
+        s += (__popc(l & zl) & 1) ? -d : d;
+      }
+      lacc[t][tid] += (__popc(goff & g.zg[t]) & 1) ? -s : s;
+    }
+  }
+  float* o = part + (uint64_t)blockIdx.x * 2 * g.nt;
+  for (int t = 0; t < g.nt; ++t) {
+    const float sr = block_sum(lacc[t][tid], red);
+    if (tid == 0) o[2 * t] = sr;
+  }
+}
+
+// Host side: split off-diagonal terms into local sets.  Each set keeps the
+// lowest lo = min(6, n) bits plus up to m - lo higher bits that its terms flip;
+// it is filled up to m bits with the lowest unused positions (n >= 8, so m >= 8:
+// every lane of the 256-thread workgroup owns 2^m / 256 amplitudes).  Terms that flip
+// more than m - lo bits above lo are left for the per-term kernels.
+struct PGPlan {
+  std::vector<PGroup> sets;
+  std::vector<int> set_of_term;   // -1: not grouped
+};
+
+static PGPlan pg_plan(int n, int nterms, const uint64_t* xmask, const uint64_t* zmask, bool with_diag, int maxt) {
+  PGPlan P;
+  P.set_of_term.assign(nterms, -1);
+  const int m = n < PG_MAXM ? n : PG_MAXM;
+  const int lo = n < 6 ? n : 6;
+  std::vector<uint32_t> hi_of(nterms, 0);
+  for (int t = 0; t < nterms; ++t) hi_of[t] = (uint32_t)(xmask[t] >> lo) << lo;
+  std::vector<int> todo;
+  for (int t = 0; t < nterms; ++t)
+    if ((xmask[t] || with_diag) && __builtin_popcount(hi_of[t]) <= m - lo) todo.push_back(t);
+  while (!todo.empty()) {
+    uint32_t H = 0;
+    std::vector<int> members, rest;
+    for (int t : todo) {
+      const uint32_t h = H | hi_of[t];
+      if (__builtin_popcount(h) <= m - lo && (int)members.size() < maxt) {
+        H = h;
+        members.push_back(t);
+      } else {
+        rest.push_back(t);
+      }
+    }
+    PGroup g{};
+    g.m = m;
+    g.lo = lo;
+    int nl = 0;
+    for (int i = 0; i < lo; ++i) g.lpos[nl++] = i;
+    uint32_t L = (1u << lo) - 1u;
+    for (int b = lo; b < n && nl < m; ++b)
+      if ((H >> b) & 1u) { g.lpos[nl++] = b; L |= 1u << b; }
+    for (int b = lo; b < n && nl < m; ++b)
+      if (!((L >> b) & 1u)) { g.lpos[nl++] = b; L |= 1u << b; }
+    // local numbering must keep positions ascending past lo (scatter order is free,
+    // but sorted keeps consecutive local indices close in memory)
+    std::sort(g.lpos + lo, g.lpos + m);
+    g.nnl = 0;
+    for (int b = 0; b < n; ++b)
+      if (!((L >> b) & 1u)) g.nlpos[g.nnl++] = b;
+    auto to_local = [&](uint64_t mask) {
+      uint32_t v = 0;
+      for (int i = 0; i < m; ++i)
+        if ((mask >> g.lpos[i]) & 1ull) v |= 1u << i;
+      return v;
+    };
+    g.nt = (int)members.size();
+    for (int q = 0; q < g.nt; ++q) {
+      const int t = members[q];
+      g.xl[q] = to_local(xmask[t]);
+      g.zl[q] = to_local(zmask[t]);
+      g.zg[q] = (uint32_t)(zmask[t] & ~(uint64_t)L);
+      g.col[q] = t;
+      P.set_of_term[t] = (int)P.sets.size();
+    }
+    P.sets.push_back(g);
+    todo.swap(rest);
+  }
+  return P;
 }
 
 }  // namespace tqsv
@@ -297,9 +541,43 @@ TQSV_API int tqsv_expval_pauli(hipStream_t s, const void* state, int64_t bsz, in
     if (n < 64 && ((xmask[t] | zmask[t]) >> n)) { set_error("tqsv_expval_pauli: mask beyond n bits"); return E_ARG; }
     if (xmask[t] == 0) idx[cnt++] = t;
   }
-  // off-diagonal terms, one pass each
+  // off-diagonal terms: grouped by local set (one pass per set), the rest
+  // (flip masks spanning too many high bits) one pass each
+  std::vector<char> done(nterms, 0);
+  if (n >= 8) {
+    const PGPlan P = pg_plan(n, nterms, xmask, zmask, false, PG_MAXT_EV);
+    for (const PGroup& g : P.sets) {
+      const int64_t per_row = 1ll << g.nnl;
+      int64_t wpr = ((int64_t)num_cu() * 4) / bsz;
+      if (wpr < 1) wpr = 1;
+      if (wpr > per_row) wpr = per_row;
+      while (per_row % wpr) --wpr;        // per_row is a power of two: wpr becomes one too
+      const int64_t gpw = per_row / wpr;
+      const int64_t grid = bsz * wpr;
+      if (grid > 0x7fffffffll || ws_bytes < grid * 2 * g.nt * (int64_t)sizeof(float)) continue;
+      switch (g.m) {
+        case 8: k_pauli_group_expval<1><<<(unsigned)grid, PG_THREADS, 0, s>>>((const float2*)state, n, g, gpw, (float*)workspace); break;
+        case 9: k_pauli_group_expval<2><<<(unsigned)grid, PG_THREADS, 0, s>>>((const float2*)state, n, g, gpw, (float*)workspace); break;
+        case 10: k_pauli_group_expval<4><<<(unsigned)grid, PG_THREADS, 0, s>>>((const float2*)state, n, g, gpw, (float*)workspace); break;
+        case 11: k_pauli_group_expval<8><<<(unsigned)grid, PG_THREADS, 0, s>>>((const float2*)state, n, g, gpw, (float*)workspace); break;
+        default: k_pauli_group_expval<16><<<(unsigned)grid, PG_THREADS, 0, s>>>((const float2*)state, n, g, gpw, (float*)workspace); break;
+      }
+      int rc = check_launch("tqsv_expval_pauli/group");
+      if (rc) return rc;
+      for (int q = 0; q < g.nt; ++q) {
+        const int t = g.col[q];
+        // partial = Re S (nY even) or Im S (nY odd); <P> = Re(i^{nY} S)
+        const int ny = __builtin_popcount((uint32_t)(xmask[t] & zmask[t]));
+        const float sc = (ny & 3) == 0 || (ny & 3) == 3 ? 1.f : -1.f;
+        k_sum_chunks<<<(unsigned)((bsz + 255) / 256), 256, 0, s>>>((const float*)workspace + 2 * q, (int)wpr,
+                                                                   2 * g.nt, 0, bsz, sc, 0.f, out, out_stride, t);
+        if ((rc = check_launch("tqsv_expval_pauli/groupsum"))) return rc;
+        done[t] = 1;
+      }
+    }
+  }
   for (int t = 0; t < nterms; ++t) {
-    if (xmask[t] == 0) continue;
+    if (xmask[t] == 0 || done[t]) continue;
     const uint32_t x = (uint32_t)xmask[t], z = (uint32_t)zmask[t];
     const int top = 31 - __builtin_clz(x);
     const int ny = __builtin_popcount(x & z);
@@ -329,19 +607,55 @@ TQSV_API int tqsv_pauli_apply(hipStream_t s, void* out, const void* in, int64_t 
   if (!out || !in || !coef || (nterms > 0 && (!xmask || !zmask))) { set_error("tqsv_pauli_apply: null pointer"); return E_ARG; }
   if (out == in) { set_error("tqsv_pauli_apply: out must not alias in"); return E_ARG; }
   if (bsz < 1 || n < 1 || n > 32 || nterms < 1) { set_error("tqsv_pauli_apply: bad sizes"); return E_ARG; }
+  for (int t = 0; t < nterms; ++t)
+    if ((xmask[t] | zmask[t]) >> n) { set_error("tqsv_pauli_apply: mask beyond n bits"); return E_ARG; }
   const uint64_t total = (uint64_t)bsz << n;
-  for (int t0 = 0; t0 < nterms; t0 += EV_MAXT) {
+  int acc = accumulate ? 1 : 0;
+  // terms grouped by local set: one LDS-staged pass per set
+  std::vector<char> done(nterms, 0);
+  if (n >= 8) {
+    const PGPlan P = pg_plan(n, nterms, xmask, zmask, true, PG_MAXT);
+    const int64_t per_row = 1ll << (n - (n < PG_MAXM ? n : PG_MAXM));
+    const int64_t n_groups = bsz * per_row;
+    int64_t grid = (int64_t)num_cu() * 4;
+    if (grid > n_groups) grid = n_groups;
+    const int64_t gpw = (n_groups + grid - 1) / grid;
+    grid = (n_groups + gpw - 1) / gpw;
+    for (const PGroup& g : P.sets) {
+#define TQSV_PG_APPLY(NR_)                                                                                     \
+  k_pauli_group_apply<NR_><<<(unsigned)grid, PG_THREADS, 0, s>>>((float2*)out, (const float2*)in, n, g,          \
+                                                                 (const float2*)coef, nterms, acc, n_groups, gpw)
+      switch (g.m) {
+        case 8: TQSV_PG_APPLY(1); break;
+        case 9: TQSV_PG_APPLY(2); break;
+        case 10: TQSV_PG_APPLY(4); break;
+        case 11: TQSV_PG_APPLY(8); break;
+        default: TQSV_PG_APPLY(16); break;
+      }
+#undef TQSV_PG_APPLY
+      int rc = check_launch("tqsv_pauli_apply/group");
+      if (rc) return rc;
+      acc = 1;
+      for (int q = 0; q < g.nt; ++q) done[g.col[q]] = 1;
+    }
+  }
+  // the rest: gathered per amplitude, EV_MAXT terms per launch
+  std::vector<int> rest;
+  for (int t = 0; t < nterms; ++t)
+    if (!done[t]) rest.push_back(t);
+  for (size_t t0 = 0; t0 < rest.size(); t0 += EV_MAXT) {
     PTerms pt;
-    pt.nt = nterms - t0 < EV_MAXT ? nterms - t0 : EV_MAXT;
+    pt.nt = (int)(rest.size() - t0 < (size_t)EV_MAXT ? rest.size() - t0 : (size_t)EV_MAXT);
     for (int q = 0; q < pt.nt; ++q) {
-      if (n < 64 && ((xmask[t0 + q] | zmask[t0 + q]) >> n)) { set_error("tqsv_pauli_apply: mask beyond n bits"); return E_ARG; }
-      pt.x[q] = (uint32_t)xmask[t0 + q];
-      pt.z[q] = (uint32_t)zmask[t0 + q];
+      pt.x[q] = (uint32_t)xmask[rest[t0 + q]];
+      pt.z[q] = (uint32_t)zmask[rest[t0 + q]];
+      pt.col[q] = rest[t0 + q];
     }
     k_pauli_apply<<<grid_cap(total), EV_THREADS, 0, s>>>((float2*)out, (const float2*)in, n, total, pt,
-                                                         (const float2*)coef, nterms, t0, (accumulate || t0 > 0) ? 1 : 0);
+                                                         (const float2*)coef, nterms, acc);
     int rc = check_launch("tqsv_pauli_apply");
     if (rc) return rc;
+    acc = 1;
   }
   return OK;
 }

@@ -37,6 +37,12 @@ from .engine import (H_NOPS, H_M, H_T, H_NNL, H_OFF_OPS, H_NGP, H_OFF_GSLOT, H_P
 NS = 1 << R_BITS
 MIN_T = 6            # compiled stages need full 64-lane waves
 KERNELS_PER_SOURCE = 4
+# Code-shape knobs: workgroups per CU the compiler must fit (0 = its own
+# choice; bounds the registers per lane), and whether the adjoint moves psi
+# and lam through LDS in one exchange (two LDS buffers) instead of two.
+FWD_MIN_WG = 0
+ADJ_MIN_WG = 0
+ADJ_PAIRED_REMAP = False
 
 PRELUDE = r"""
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -62,35 +68,6 @@ __device__ __forceinline__ float2 cfmac(float2 a, float2 b, float2 acc) {
 __device__ __forceinline__ float redot(float2 a, float2 b) { return fmaf(a.x, b.x, a.y * b.y); }
 __device__ __forceinline__ float2 csel(bool c, float2 a, float2 b) {
   return make_float2(c ? a.x : b.x, c ? a.y : b.y);
-}
-// NV values per lane -> every lane holds the wave total of value index
-// bits (5, 4, 3)[: log2 NV] of its lane id (halving exchanges, then a tree).
-template <int NV>
-__device__ __forceinline__ float wave_reduce_t(const float (&v)[NV], int lane) {
-  float a[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) a[i] = v[i];
-  int cnt = NV;
-  int sh = 32;
-#pragma unroll
-  for (int step = 0; step < 3; ++step) {
-    if (cnt == 1) break;
-    const int half = cnt >> 1;
-    const bool hi = lane & sh;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (i < half) {
-        const float keep = hi ? a[i + half] : a[i];
-        const float send = hi ? a[i] : a[i + half];
-        a[i] = keep + __shfl_xor(send, sh, 64);
-      }
-    }
-    cnt = half;
-    sh >>= 1;
-  }
-  float c = a[0];
-  for (int o = sh; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
-  return c;
 }
 """
 
@@ -218,6 +195,29 @@ class _Gen:
                     lr |= to[T + k]
             out.append(self.var(f"{lds}[{rb} ^ {swz(lr)}u]"))
         return out
+
+    def remap2(self, p: List[str], l: List[str], frm: Sequence[int], to: Sequence[int]):
+        """remap of psi and lam through two LDS buffers in one exchange."""
+        T = self.sp.T
+        ra, rb = self.lane_index(frm), self.lane_index(to)
+        self.line("__syncthreads();")
+        for j in range(NS):
+            lr = 0
+            for k in range(R_BITS):
+                if (j >> k) & 1:
+                    lr |= frm[T + k]
+            self.line(f"lds[{ra} ^ {swz(lr)}u] = {p[j]};")
+            self.line(f"lds2[{ra} ^ {swz(lr)}u] = {l[j]};")
+        self.line("__syncthreads();")
+        po, lo = [], []
+        for j in range(NS):
+            lr = 0
+            for k in range(R_BITS):
+                if (j >> k) & 1:
+                    lr |= to[T + k]
+            po.append(self.var(f"lds[{rb} ^ {swz(lr)}u]"))
+            lo.append(self.var(f"lds2[{rb} ^ {swz(lr)}u]"))
+        return po, lo
 
     # ---- forward ops (k_stage_fwd's fwd_op)
     def f_xperm(self, w, a: List[str]):
@@ -583,15 +583,33 @@ class _Gen:
         self.line(f"if ({cond}) atomicAdd(&gacc[{w[7]} + (lane >> {sh})], {c});")
 
 
+def _split_addr(sp: StageProgram) -> bool:
+    """Amplitude addresses as (wave-uniform 64-bit base + constant) plus one
+    32-bit per-lane byte offset (toff * 8), so that every load and store of a
+    group shares one offset register; needs toff * 8 < 2^32."""
+    return max(sp.tpos, default=0) <= 28
+
+
 def _head(sp: StageProgram, g: _Gen) -> List[str]:
     """Group-loop prologue: row, group offset, base index."""
     terms = [f"((unsigned)((rest >> {i}) & 1ull) << {sp.nlpos[i]})" for i in range(sp.nnl)]
-    return [
+    out = [
         f"    const unsigned long long row = (unsigned long long)g >> {sp.nnl};",
         f"    const unsigned long long rest = (unsigned long long)g & {(1 << sp.nnl) - 1}ull;",
         f"    const unsigned goff = {' | '.join(terms) if terms else '0u'};",
-        f"    const unsigned long long base = (row << {sp.n}) + goff + toff;",
+        f"    const unsigned long long gbase = (row << {sp.n}) + goff;",
     ]
+    if not _split_addr(sp):
+        out.append("    const unsigned long long base = gbase + toff;")
+    return out
+
+
+def _amp(sp: StageProgram, arr: str, j: int, const: bool) -> str:
+    """lvalue of amplitude slot j of this lane's group in array arr."""
+    if _split_addr(sp):
+        c = "const " if const else ""
+        return f"*({c}float2*)(({c}char*)({arr} + gbase) + {8 * sp.roff[j]}ull + toff8)"
+    return f"{arr}[base + {sp.roff[j]}]"
 
 
 def _toff(sp: StageProgram) -> str:
@@ -616,13 +634,15 @@ def gen_forward(words: np.ndarray, name: str) -> str:
             g.fwd_op(w, a)
     body = g.lines
     nt = 1 << sp.T
-    src = [f'extern "C" __global__ void __launch_bounds__({nt}) {name}(float2* __restrict__ out, '
+    lb = f"{nt}, {FWD_MIN_WG}" if FWD_MIN_WG else f"{nt}"
+    src = [f'extern "C" __global__ void __launch_bounds__({lb}) {name}(float2* __restrict__ out, '
            "const float2* __restrict__ in, const float2* __restrict__ mats, int init_zero, long long n_groups, "
            "long long gpw) {"]
     if has_remap:
         src.append(f"  __shared__ float2 lds[{1 << sp.m}];")
     src.append("  const int tid = threadIdx.x;")
     src.append(f"  const unsigned toff = {_toff(sp)};")
+    src.append("  const unsigned toff8 = toff << 3;")
     src += g.hoisted()
     src += ["  const long long g0 = (long long)blockIdx.x * gpw;",
             "  long long g1 = g0 + gpw;",
@@ -634,14 +654,12 @@ def gen_forward(words: np.ndarray, name: str) -> str:
     for j in range(NS):
         src.append(f"      i{j} = make_float2((goff + toff + {sp.roff[j]}u) == 0u ? 1.f : 0.f, 0.f);")
     src.append("    } else {")
-    src.append("      const float2* src = in + base;")
     for j in range(NS):
-        src.append(f"      i{j} = src[{sp.roff[j]}];")
+        src.append(f"      i{j} = {_amp(sp, 'in', j, True)};")
     src.append("    }")
     src += body
-    src.append("    float2* dst = out + base;")
     for j in range(NS):
-        src.append(f"    dst[{sp.roff[j]}] = {a[j]};")
+        src.append(f"    {_amp(sp, 'out', j, False)} = {a[j]};")
     src += ["  }", "}"]
     return "\n".join(src) + "\n"
 
@@ -658,8 +676,11 @@ def gen_adjoint(words: np.ndarray, name: str) -> str:
     for w in reversed(sp.ops):
         if (w[0] & 15) == OP_REMAP:
             frm, to = sp.remap_layouts(w)
-            p = g.remap(p, to, frm)
-            l = g.remap(l, to, frm)
+            if ADJ_PAIRED_REMAP:
+                p, l = g.remap2(p, l, to, frm)
+            else:
+                p = g.remap(p, to, frm)
+                l = g.remap(l, to, frm)
             has_remap = True
             continue
         parts = g.bwd_op(w, p, l)
@@ -670,17 +691,21 @@ def gen_adjoint(words: np.ndarray, name: str) -> str:
     src = []
     if ngp:
         src.append(f"__constant__ int {name}_gs[{ngp}] = {{{', '.join(str(int(x)) for x in sp.gslots)}}};")
-    src.append(f'extern "C" __global__ void __launch_bounds__({nt}) {name}(float2* __restrict__ psi, '
+    lb = f"{nt}, {ADJ_MIN_WG}" if ADJ_MIN_WG else f"{nt}"
+    src.append(f'extern "C" __global__ void __launch_bounds__({lb}) {name}(float2* __restrict__ psi, '
                "float2* __restrict__ lam, const float2* __restrict__ mats, float* __restrict__ grad, "
                "long long grad_stride, long long n_groups, long long gpw) {")
     if has_remap:
         src.append(f"  __shared__ float2 lds[{1 << sp.m}];")
+        if ADJ_PAIRED_REMAP:
+            src.append(f"  __shared__ float2 lds2[{1 << sp.m}];")
     if ngp:
         src.append(f"  __shared__ float gacc[{ngp}];")
         src.append(f"  for (int t = threadIdx.x; t < {ngp}; t += {nt}) gacc[t] = 0.f;")
     src.append("  const int tid = threadIdx.x;")
     src.append("  const int lane = tid & 63;")
     src.append(f"  const unsigned toff = {_toff(sp)};")
+    src.append("  const unsigned toff8 = toff << 3;")
     src += g.hoisted()
     src += ["  const long long g0 = (long long)blockIdx.x * gpw;",
             "  long long g1 = g0 + gpw;",
@@ -702,19 +727,15 @@ def gen_adjoint(words: np.ndarray, name: str) -> str:
                 "      }",
                 "      cur_row = (long long)row;",
                 "    }"]
-    src.append("    const float2* sp_ = psi + base;")
-    src.append("    const float2* sl_ = lam + base;")
     for j in range(NS):
-        src.append(f"    const float2 p{j} = sp_[{sp.roff[j]}];")
+        src.append(f"    const float2 p{j} = {_amp(sp, 'psi', j, True)};")
     for j in range(NS):
-        src.append(f"    const float2 l{j} = sl_[{sp.roff[j]}];")
+        src.append(f"    const float2 l{j} = {_amp(sp, 'lam', j, True)};")
     src += g.lines
-    src.append("    float2* dp = psi + base;")
-    src.append("    float2* dl = lam + base;")
     for j in range(NS):
-        src.append(f"    dp[{sp.roff[j]}] = {p[j]};")
+        src.append(f"    {_amp(sp, 'psi', j, False)} = {p[j]};")
     for j in range(NS):
-        src.append(f"    dl[{sp.roff[j]}] = {l[j]};")
+        src.append(f"    {_amp(sp, 'lam', j, False)} = {l[j]};")
     src.append("  }")
     if ngp:
         src += ["  if (cur_row >= 0) {",
@@ -732,7 +753,9 @@ def gen_adjoint(words: np.ndarray, name: str) -> str:
 # compile + cache
 # ----------------------------------------------------------------------------
 def kernel_name(kind: str, words: np.ndarray) -> str:
-    h = hashlib.sha1(np.ascontiguousarray(words, dtype=np.int32).tobytes()).hexdigest()[:20]
+    hs = hashlib.sha1(np.ascontiguousarray(words, dtype=np.int32).tobytes())
+    hs.update(repr((FWD_MIN_WG, ADJ_MIN_WG, ADJ_PAIRED_REMAP)).encode())
+    h = hs.hexdigest()[:20]
     return f"tqsv_j{kind}_{h}"
 
 
