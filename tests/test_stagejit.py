@@ -107,4 +107,4 @@ def test_generated_code_follows_program():
     src = J.gen_forward(s.words, "k")
     assert "const float2 v" not in src
     for j in range(16):
-        assert f"dst[{sp.roff[j]}] = i{j ^ (1 << K)};" in src
+        assert f"{J._amp(sp, 'out', j, False)} = i{j ^ (1 << K)};" in src
