@@ -59,6 +59,8 @@ def time_forward(n, bsz, ops, mode, steps, warmup, tq):
 
     for _ in range(warmup):
         step()
+    if warmup and tq.wait_for_compiled_stages():   # compiled stages (jit="auto") are ready: run them once
+        step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -98,6 +100,8 @@ def time_gradient(n, bsz, layers, mode, steps, warmup, tq):
         return loss
 
     for _ in range(warmup):
+        step()
+    if warmup and tq.wait_for_compiled_stages():
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -268,6 +272,16 @@ def config5(args, tq):
     res = {"workload": f"{n} qubits, bsz {bsz}, 20 x [RY all, CZ even+odd] + Toffoli/CSWAP every 5 layers",
            "gates": len(ops)}
     with torch.no_grad():
+        # one untimed fused pass plans the circuit and starts the background
+        # compile of its stage kernels; the timed passes run the compiled ones
+        log("[config5] fused forward (warm-up, compile)")
+        t0 = time.perf_counter()
+        q = build("fused")
+        tq.expval_joint_analytical_batch(q, obs[:4])
+        del q
+        torch.cuda.empty_cache()
+        tq.wait_for_compiled_stages()
+        res["warmup_and_compile_s"] = time.perf_counter() - t0
         for mode in ("fused", "pergate"):
             log(f"[config5] {mode} forward")
             q = build(mode)
@@ -287,6 +301,15 @@ def config5(args, tq):
                 res["fused_vs_pergate_amp_max_abs"] = float((q.get_states_1d()[:, :: (1 << 12)] - sample).abs().max())
             del q
             torch.cuda.empty_cache()
+    log("[config5] fused gradient (warm-up, compile)")
+    q = build("fused")
+    loss = tq.expval_pauli_sum(q, obs, [1.0] * len(obs)).mean()
+    loss.backward()
+    del q, loss
+    for p in params:
+        if p is not None:
+            p.grad = None
+    tq.wait_for_compiled_stages()
     log("[config5] fused gradient")
     q = build("fused")
     torch.cuda.synchronize()

@@ -125,3 +125,36 @@ def test_jit_config3_family_gradient():
         out[jit] = (float(loss.detach()), torch.cat([p.grad.reshape(-1) for p in params]).cpu().numpy())
     assert abs(out["on"][0] - out["off"][0]) < 1e-5
     assert np.abs(out["on"][1] - out["off"][1]).max() < 1e-5
+
+
+@gpu
+def test_jit_auto_compiles_in_background():
+    """jit="auto" on a 2^22-amplitude state: the first pass runs the
+    interpreting kernels while the stage kernels compile on a background
+    thread; after wait_for_compiled_stages() the same plan runs compiled
+    kernels, with the same amplitudes (1e-6) and gradients (1e-5)."""
+    import torchquantum_amd as tq
+    from torchquantum_amd import engine as E
+    E._plan_cache.clear()
+    rng = np.random.default_rng(9)
+    n, bsz = 18, 16
+    names = ["rx", "ry", "rz", "u3", "cnot", "cz", "crx", "rzz", "h", "swap"]
+    ops = random_ops(n, 40, bsz, rng, names=names)
+    obs, coef = config3_observables(n)
+    outs = []
+    for _ in range(2):
+        q = tq.QuantumDevice(n_wires=n, bsz=bsz, device=DEV)
+        ts = apply_ops_tq(q, ops, DEV, requires_grad=True)
+        loss = tq.expval_pauli_sum(q, obs, coef).sum()
+        st = q.get_states_1d().detach().cpu().numpy()
+        loss.backward()
+        g = np.concatenate([t.grad.detach().cpu().numpy().reshape(-1) for t in ts])
+        outs.append((st, g))
+        if not outs[1:]:
+            plans = [c for c in E._plan_cache.values() if c.bsz == bsz and c.n == n]
+            assert plans and all("f" not in c._jit for c in plans)   # first pass: interpreting kernels
+            assert tq.wait_for_compiled_stages(timeout=600)
+    plans = [c for c in E._plan_cache.values() if c.bsz == bsz and c.n == n]
+    assert any(c._jit.get("f") for c in plans) and any(c._jit.get("a") for c in plans)
+    assert np.abs(outs[0][0] - outs[1][0]).max() < 1e-6
+    assert np.abs(outs[0][1] - outs[1][1]).max() < 1e-5

@@ -108,3 +108,17 @@ def test_generated_code_follows_program():
     assert "const float2 v" not in src
     for j in range(16):
         assert f"{J._amp(sp, 'out', j, False)} = i{j ^ (1 << K)};" in src
+
+
+def test_code_object_disk_cache(tmp_path, monkeypatch):
+    """A compiled source is stored under TQSV_JIT_CACHE and read back."""
+    from torchquantum_amd import stagejit as J
+    monkeypatch.setenv("TQSV_JIT_CACHE", str(tmp_path))
+    src = 'extern "C" __global__ void k(float* p) { p[threadIdx.x] *= 2.f; }\n'
+    a = J.compile_source(src)
+    files = list(tmp_path.glob("*.co"))
+    assert len(files) == 1 and files[0].read_bytes() == a
+    files[0].write_bytes(b"cached")          # a second compile returns the stored bytes
+    assert J.compile_source(src) == b"cached"
+    monkeypatch.setenv("TQSV_JIT_CACHE", "off")
+    assert J.compile_source(src) == a

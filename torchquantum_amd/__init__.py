@@ -19,7 +19,7 @@ from .measurement import (expval, MeasureAll, expval_joint_analytical, expval_jo
                           expval_pauli_sum, MeasureMultiPauliSum, gen_bitstrings, measure, sample_indices,
                           find_observable_groups, expval_joint_sampling, expval_joint_sampling_grouping)
 from .layer import (Op1QAllLayer, Op2QAllLayer, EntanglePairwise, LayerTemplate0, RXYZCXLayer0, RandomLayer)
-from .engine import EngineOptions
+from .engine import EngineOptions, wait_for_compiled_stages
 from . import device as _device_mod
 
 _device_mod._bind_methods()

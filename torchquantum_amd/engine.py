@@ -68,9 +68,11 @@ class EngineOptions:
     jit:         "auto" | "on" | "off": run stages as kernels compiled at run
                  time from their programs (straight-line code, hipRTC) instead
                  of the program-interpreting kernels.  "auto" compiles for
-                 states of >= 2^22 amplitudes (bsz * 2^n), where the one-time
-                 compile (about 2 s per stage and direction, on host threads)
-                 is repaid within a few passes.
+                 states of >= 2^22 amplitudes (bsz * 2^n) on a background
+                 thread (about 0.5-2 s per stage and direction; code objects
+                 are cached on disk) while the interpreting kernels run, and
+                 switches once they are ready (wait_for_compiled_stages()
+                 blocks until then).  "on" compiles before the first pass.
     """
     mode: str = "fused"
     stage_bits: int = 0
@@ -88,6 +90,15 @@ class EngineOptions:
 
 DEFAULT_OPTIONS = EngineOptions()
 JIT_MIN_AMPS = 1 << 22
+
+
+def wait_for_compiled_stages(timeout: Optional[float] = None) -> bool:
+    """Block until the background compiles of stage kernels (jit="auto") have
+    finished; later passes of those plans then run the compiled kernels.
+    Returns False if the timeout passed first."""
+    import sys
+    sj = sys.modules.get(__package__ + ".stagejit")
+    return True if sj is None else sj.wait_all(timeout)
 
 
 # ----------------------------------------------------------------------------
@@ -1026,6 +1037,7 @@ class Compiled:
         self._fill = None
         self._gv_layout: Optional[_GVLayout] = None
         self._jit: Dict[str, Optional[Dict[int, int]]] = {}
+        self._jit_jobs: Dict[str, object] = {}
         self.mg_items = np.zeros(0, dtype=MG_ITEM)
         self.grad_width = max(self.n_slots, 1)
         if opts.mode == "fused" and n >= R_BITS:
@@ -1100,31 +1112,41 @@ class Compiled:
     def jit_kernels(self, kind: str) -> Optional[Dict[int, int]]:
         """Compiled stage kernels of this plan ({stage word offset: kernel
         handle}; kind "f" forward, "a" adjoint), or None when stages run on
-        the interpreting kernels (option off, small state, nothing eligible)."""
+        the interpreting kernels: option off, small state, nothing eligible,
+        or -- with jit="auto" -- while the background compile is running.
+        jit="on" compiles on the calling thread before returning."""
         if kind in self._jit:
             return self._jit[kind]
-        table = None
         mode = self.opts.jit
         if mode not in ("auto", "on", "off"):
             raise ValueError(f"EngineOptions.jit must be 'auto', 'on' or 'off', not {mode!r}")
         want = mode == "on" or (mode == "auto" and (self.bsz << self.n) >= JIT_MIN_AMPS)
-        if want and torch.device(self.device).type == "cuda" and self.prog_host is not None:
-            from . import stagejit
-            stages = [(base, st.words) for kind_, (base, st) in
-                      [(k, p) for k, p in self.steps if k == "stage"] if stagejit.supported(st.words)]
-            if stages:
-                dev = torch.device(self.device)
-                index = dev.index if dev.index is not None else torch.cuda.current_device()
-                try:
-                    funcs = stagejit.get_kernels([w for _, w in stages], kind, index)
-                except (RuntimeError, OSError) as e:
-                    if mode == "on":
-                        raise
-                    import warnings
-                    warnings.warn(f"run-time stage compile unavailable, using the interpreting kernels: {e}")
-                    funcs = None
-                if funcs is not None:
-                    table = {base: f for (base, _), f in zip(stages, funcs)}
+        if not (want and torch.device(self.device).type == "cuda" and self.prog_host is not None):
+            self._jit[kind] = None
+            return None
+        from . import stagejit
+        stages = [(base, st.words) for kind_, (base, st) in
+                  [(k, p) for k, p in self.steps if k == "stage"] if stagejit.supported(st.words)]
+        if not stages:
+            self._jit[kind] = None
+            return None
+        dev = torch.device(self.device)
+        index = dev.index if dev.index is not None else torch.cuda.current_device()
+        if mode == "on":
+            funcs = stagejit.get_kernels([w for _, w in stages], kind, index)
+        else:
+            job = self._jit_jobs.get(kind)
+            if job is None:
+                self._jit_jobs[kind] = stagejit.submit([w for _, w in stages], kind, index)
+                return None
+            if not job.done.is_set():
+                return None
+            funcs = job.result
+            if job.error is not None:
+                import warnings
+                warnings.warn(f"run-time stage compile failed, staying on the interpreting kernels: {job.error}")
+                funcs = None
+        table = None if funcs is None else {base: f for (base, _), f in zip(stages, funcs)}
         self._jit[kind] = table
         return table
 

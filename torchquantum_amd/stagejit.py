@@ -22,11 +22,13 @@ for the adjoint, its autograd backward :127-136).  What changes is how:
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import hashlib
 import os
+import queue
 import threading
-from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
 from typing import Dict, List, Optional, Sequence
 
 import numpy as np
@@ -759,9 +761,46 @@ def kernel_name(kind: str, words: np.ndarray) -> str:
     return f"tqsv_j{kind}_{h}"
 
 
-def compile_source(src: str) -> bytes:
-    """hipRTC-compile a source (no GPU needed); returns the code object."""
+def _cache_dir() -> Optional[Path]:
+    """Directory of the on-disk code-object cache (TQSV_JIT_CACHE; "0" or
+    "off" disables it; default ~/.cache/torchquantum_amd/stage-kernels)."""
+    d = os.environ.get("TQSV_JIT_CACHE")
+    if d is not None and d.strip().lower() in ("", "0", "off", "none"):
+        return None
+    base = Path(d) if d else Path.home() / ".cache" / "torchquantum_amd" / "stage-kernels"
+    try:
+        base.mkdir(parents=True, exist_ok=True)
+    except OSError:
+        return None
+    return base
+
+
+def _lib_tag() -> str:
+    """Identifies the library build (its compile options travel with it)."""
     from . import _native as N
+    try:
+        st = N.LIB_PATH.stat()
+        return f"{st.st_size}:{st.st_mtime_ns}"
+    except OSError:
+        return "?"
+
+
+def compile_source(src: str) -> bytes:
+    """hipRTC-compile a source (no GPU needed); returns the code object.
+    Code objects are cached on disk by a hash of the source and the library
+    build."""
+    from . import _native as N
+    cache = _cache_dir()
+    path = None
+    if cache is not None:
+        h = hashlib.sha1((_lib_tag() + "\n" + src).encode()).hexdigest()
+        path = cache / f"{h}.co"
+        try:
+            data = path.read_bytes()
+            if data:
+                return data
+        except OSError:
+            pass
     lib = N.lib()
     size = ctypes.c_int64(0)
     log = ctypes.create_string_buffer(1 << 16)
@@ -773,26 +812,28 @@ def compile_source(src: str) -> bytes:
     rc = lib.tqsv_jit_compile(srcb, buf, size.value, ctypes.byref(size), None, 0)
     if rc != 0:
         raise RuntimeError(f"stage kernel compile failed: {lib.tqsv_last_error().decode(errors='replace')}")
-    return buf.raw[:size.value]
+    code = buf.raw[:size.value]
+    if path is not None:
+        try:
+            tmp = path.with_suffix(f".{os.getpid()}.{threading.get_ident()}.tmp")
+            tmp.write_bytes(code)
+            os.replace(tmp, path)
+        except OSError:
+            pass
+    return code
 
 
 _lock = threading.Lock()
+_shutdown = threading.Event()
 _funcs: Dict[tuple, int] = {}          # (device index, kernel name) -> hipFunction_t
 _modules: List[int] = []                # loaded modules (kept for the process lifetime)
 
 
-def _jobs() -> int:
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 4
-    return max(1, min(8, n))
-
-
 def get_kernels(programs: Sequence[np.ndarray], kind: str, device_index: int) -> List[int]:
     """hipFunction_t handles (as ints) of the compiled stage kernels for the
-    given programs; kind "f" = forward, "a" = adjoint.  Compiles (in parallel
-    host threads) and loads what is not cached yet."""
+    given programs; kind "f" = forward, "a" = adjoint.  Compiles and loads
+    (on the calling thread) what is not loaded yet.  hipRTC serialises
+    compiles within a process, so sources are compiled one after another."""
     from . import _native as N
     gen = gen_forward if kind == "f" else gen_adjoint
     names = [kernel_name(kind, w) for w in programs]
@@ -801,17 +842,19 @@ def get_kernels(programs: Sequence[np.ndarray], kind: str, device_index: int) ->
         for nm, w in zip(names, programs):
             if (device_index, nm) not in _funcs and nm not in missing:
                 missing[nm] = w
-        if missing:
-            items = list(missing.items())
-            chunks = [items[i:i + KERNELS_PER_SOURCE] for i in range(0, len(items), KERNELS_PER_SOURCE)]
-            sources = [PRELUDE + "\n".join(gen(w, nm) for nm, w in ch) for ch in chunks]
-            if len(sources) > 1:
-                with ThreadPoolExecutor(max_workers=min(_jobs(), len(sources))) as ex:
-                    codes = list(ex.map(compile_source, sources))
-            else:
-                codes = [compile_source(s) for s in sources]
-            lib = N.lib()
+    if missing:
+        items = list(missing.items())
+        chunks = [items[i:i + KERNELS_PER_SOURCE] for i in range(0, len(items), KERNELS_PER_SOURCE)]
+        codes = []
+        for ch in chunks:
+            if _shutdown.is_set():
+                raise RuntimeError("stage kernel compile cancelled at interpreter exit")
+            codes.append(compile_source(PRELUDE + "\n".join(gen(w, nm) for nm, w in ch)))
+        lib = N.lib()
+        with _lock:
             for ch, code in zip(chunks, codes):
+                if all((device_index, nm) in _funcs for nm, _ in ch):
+                    continue
                 cnames = (ctypes.c_char_p * len(ch))(*[nm.encode() for nm, _ in ch])
                 fptrs = (ctypes.c_void_p * len(ch))()
                 mod = ctypes.c_void_p()
@@ -822,4 +865,78 @@ def get_kernels(programs: Sequence[np.ndarray], kind: str, device_index: int) ->
                 _modules.append(mod.value)
                 for (nm, _), f in zip(ch, fptrs):
                     _funcs[(device_index, nm)] = f
+    with _lock:
         return [_funcs[(device_index, nm)] for nm in names]
+
+
+# ----------------------------------------------------------------------------
+# background compiles: the interpreting kernels run until these are ready
+# ----------------------------------------------------------------------------
+class CompileJob:
+    """One plan's stage kernels being compiled on the background thread."""
+
+    def __init__(self, programs, kind, device_index):
+        self.programs, self.kind, self.device_index = list(programs), kind, device_index
+        self.done = threading.Event()
+        self.result: Optional[List[int]] = None
+        self.error: Optional[BaseException] = None
+
+
+_jobs: "queue.Queue[CompileJob]" = queue.Queue()
+_pending: List[CompileJob] = []
+_worker: Optional[threading.Thread] = None
+
+
+def _work():
+    import torch
+    while True:
+        job = _jobs.get()
+        try:
+            torch.cuda.set_device(job.device_index)
+            job.result = get_kernels(job.programs, job.kind, job.device_index)
+        except BaseException as e:   # reported to the plan that asked
+            job.error = e
+        finally:
+            job.done.set()
+            with _lock:
+                if job in _pending:
+                    _pending.remove(job)
+
+
+def submit(programs: Sequence[np.ndarray], kind: str, device_index: int) -> CompileJob:
+    """Queue a compile on the background thread; returns at once."""
+    global _worker
+    job = CompileJob(programs, kind, device_index)
+    with _lock:
+        _pending.append(job)
+        if _worker is None or not _worker.is_alive():
+            _worker = threading.Thread(target=_work, name="tqsv-stage-compile", daemon=True)
+            _worker.start()
+    _jobs.put(job)
+    return job
+
+
+def wait_all(timeout: Optional[float] = None) -> bool:
+    """Block until every queued background compile has finished (True) or the
+    timeout has passed (False)."""
+    import time
+    end = None if timeout is None else time.monotonic() + timeout
+    while True:
+        with _lock:
+            jobs = list(_pending)
+        if not jobs:
+            return True
+        for j in jobs:
+            left = None if end is None else max(0.0, end - time.monotonic())
+            if not j.done.wait(left):
+                return False
+
+
+def _at_exit():
+    """Let a background compile stop at its next source boundary and finish
+    before the HIP runtime is torn down."""
+    _shutdown.set()
+    wait_all(timeout=60)
+
+
+atexit.register(_at_exit)
