@@ -395,6 +395,35 @@ __global__ __launch_bounds__(PG_THREADS) void k_pauli_group_expval(const float2*
   }
 }
 
+constexpr int PG_WG_PER_CU = 4;   // workgroups per CU of the grouped-term launches
+
+template <int NR>
+static int pg_apply_launch(hipStream_t s, float2* out, const float2* in, int n, const PGroup& g, const float2* coef,
+                           int64_t coef_stride, int acc, int64_t bsz) {
+  const int64_t n_groups = bsz << g.nnl;
+  int64_t grid = (int64_t)num_cu() * PG_WG_PER_CU;
+  if (grid > n_groups) grid = n_groups;
+  const int64_t gpw = (n_groups + grid - 1) / grid;
+  grid = (n_groups + gpw - 1) / gpw;
+  k_pauli_group_apply<NR><<<(unsigned)grid, PG_THREADS, 0, s>>>(out, in, n, g, coef, coef_stride, acc, n_groups, gpw);
+  return check_launch("tqsv_pauli_apply/group");
+}
+
+template <int NR>
+static int pg_expval_launch(hipStream_t s, const float2* in, int n, const PGroup& g, int64_t bsz, void* ws,
+                            int64_t ws_bytes, int64_t& wpr) {
+  const int64_t per_row = 1ll << g.nnl;
+  wpr = ((int64_t)num_cu() * PG_WG_PER_CU) / bsz;
+  if (wpr < 1) wpr = 1;
+  if (wpr > per_row) wpr = per_row;
+  while (per_row % wpr) --wpr;        // per_row is a power of two: wpr becomes one too
+  const int64_t gpw = per_row / wpr;
+  const int64_t grid = bsz * wpr;
+  if (grid > 0x7fffffffll || ws_bytes < grid * 2 * g.nt * (int64_t)sizeof(float)) return 1;   // caller falls back
+  k_pauli_group_expval<NR><<<(unsigned)grid, PG_THREADS, 0, s>>>(in, n, g, gpw, (float*)ws);
+  return check_launch("tqsv_expval_pauli/group");
+}
+
 // Host side: split off-diagonal terms into local sets.  Each set keeps the
 // lowest lo = min(6, n) bits plus up to m - lo higher bits that its terms flip;
 // it is filled up to m bits with the lowest unused positions (n >= 8, so m >= 8:
@@ -547,22 +576,16 @@ TQSV_API int tqsv_expval_pauli(hipStream_t s, const void* state, int64_t bsz, in
   if (n >= 8) {
     const PGPlan P = pg_plan(n, nterms, xmask, zmask, false, PG_MAXT_EV);
     for (const PGroup& g : P.sets) {
-      const int64_t per_row = 1ll << g.nnl;
-      int64_t wpr = ((int64_t)num_cu() * 4) / bsz;
-      if (wpr < 1) wpr = 1;
-      if (wpr > per_row) wpr = per_row;
-      while (per_row % wpr) --wpr;        // per_row is a power of two: wpr becomes one too
-      const int64_t gpw = per_row / wpr;
-      const int64_t grid = bsz * wpr;
-      if (grid > 0x7fffffffll || ws_bytes < grid * 2 * g.nt * (int64_t)sizeof(float)) continue;
+      int64_t wpr = 0;
+      int rc;
       switch (g.m) {
-        case 8: k_pauli_group_expval<1><<<(unsigned)grid, PG_THREADS, 0, s>>>((const float2*)state, n, g, gpw, (float*)workspace); break;
-        case 9: k_pauli_group_expval<2><<<(unsigned)grid, PG_THREADS, 0, s>>>((const float2*)state, n, g, gpw, (float*)workspace); break;
-        case 10: k_pauli_group_expval<4><<<(unsigned)grid, PG_THREADS, 0, s>>>((const float2*)state, n, g, gpw, (float*)workspace); break;
-        case 11: k_pauli_group_expval<8><<<(unsigned)grid, PG_THREADS, 0, s>>>((const float2*)state, n, g, gpw, (float*)workspace); break;
-        default: k_pauli_group_expval<16><<<(unsigned)grid, PG_THREADS, 0, s>>>((const float2*)state, n, g, gpw, (float*)workspace); break;
+        case 8: rc = pg_expval_launch<1>(s, (const float2*)state, n, g, bsz, workspace, ws_bytes, wpr); break;
+        case 9: rc = pg_expval_launch<2>(s, (const float2*)state, n, g, bsz, workspace, ws_bytes, wpr); break;
+        case 10: rc = pg_expval_launch<4>(s, (const float2*)state, n, g, bsz, workspace, ws_bytes, wpr); break;
+        case 11: rc = pg_expval_launch<8>(s, (const float2*)state, n, g, bsz, workspace, ws_bytes, wpr); break;
+        default: rc = pg_expval_launch<16>(s, (const float2*)state, n, g, bsz, workspace, ws_bytes, wpr); break;
       }
-      int rc = check_launch("tqsv_expval_pauli/group");
+      if (rc == 1) continue;   // workspace too small: these terms take the per-term path
       if (rc) return rc;
       for (int q = 0; q < g.nt; ++q) {
         const int t = g.col[q];
@@ -615,25 +638,18 @@ TQSV_API int tqsv_pauli_apply(hipStream_t s, void* out, const void* in, int64_t 
   std::vector<char> done(nterms, 0);
   if (n >= 8) {
     const PGPlan P = pg_plan(n, nterms, xmask, zmask, true, PG_MAXT);
-    const int64_t per_row = 1ll << (n - (n < PG_MAXM ? n : PG_MAXM));
-    const int64_t n_groups = bsz * per_row;
-    int64_t grid = (int64_t)num_cu() * 4;
-    if (grid > n_groups) grid = n_groups;
-    const int64_t gpw = (n_groups + grid - 1) / grid;
-    grid = (n_groups + gpw - 1) / gpw;
     for (const PGroup& g : P.sets) {
-#define TQSV_PG_APPLY(NR_)                                                                                     \
-  k_pauli_group_apply<NR_><<<(unsigned)grid, PG_THREADS, 0, s>>>((float2*)out, (const float2*)in, n, g,          \
-                                                                 (const float2*)coef, nterms, acc, n_groups, gpw)
+      float2* o = (float2*)out;
+      const float2* x = (const float2*)in;
+      const float2* c = (const float2*)coef;
+      int rc;
       switch (g.m) {
-        case 8: TQSV_PG_APPLY(1); break;
-        case 9: TQSV_PG_APPLY(2); break;
-        case 10: TQSV_PG_APPLY(4); break;
-        case 11: TQSV_PG_APPLY(8); break;
-        default: TQSV_PG_APPLY(16); break;
+        case 8: rc = pg_apply_launch<1>(s, o, x, n, g, c, nterms, acc, bsz); break;
+        case 9: rc = pg_apply_launch<2>(s, o, x, n, g, c, nterms, acc, bsz); break;
+        case 10: rc = pg_apply_launch<4>(s, o, x, n, g, c, nterms, acc, bsz); break;
+        case 11: rc = pg_apply_launch<8>(s, o, x, n, g, c, nterms, acc, bsz); break;
+        default: rc = pg_apply_launch<16>(s, o, x, n, g, c, nterms, acc, bsz); break;
       }
-#undef TQSV_PG_APPLY
-      int rc = check_launch("tqsv_pauli_apply/group");
       if (rc) return rc;
       acc = 1;
       for (int q = 0; q < g.nt; ++q) done[g.col[q]] = 1;

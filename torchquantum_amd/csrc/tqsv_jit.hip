@@ -23,11 +23,12 @@
 
 int stage_launch_shape(const int* prog_host, int n, int64_t bsz, int grid_cap, int& T, int64_t& n_groups,
                        int& grid, int64_t& gpw, size_t& shm);
-
 using namespace tqsv;
 
 namespace {
 
+
+constexpr int kHdrT = 2;    // stage header word: thread bits T (2^T threads per group)
 constexpr int kHdrNgp = 5;  // stage header word: number of gradient parameters
 
 // gfx950 code, no packed-FP32 instructions (the same flags as the library)
@@ -121,9 +122,10 @@ TQSV_API int tqsv_jit_stage_forward(hipStream_t s, const void* func, void* state
   int T, grid;
   int64_t n_groups, gpw;
   size_t shm;
+  const int Th = prog_host[kHdrT];
+  if (Th < 6 || Th > 9) { set_error("tqsv_jit_stage_forward: compiled stages need 64..512 threads (6 <= T <= 9)"); return E_ARG; }
   int rc = stage_launch_shape(prog_host, n, bsz, grid_cap, T, n_groups, grid, gpw, shm);
   if (rc) return rc;
-  if (T < 6) { set_error("tqsv_jit_stage_forward: compiled stages need >= 64 threads (T >= 6)"); return E_ARG; }
   void* out = state_out;
   const void* in = state_in;
   const void* m = mats;
@@ -150,9 +152,10 @@ TQSV_API int tqsv_jit_stage_adjoint(hipStream_t s, const void* func, void* psi, 
   int T, grid;
   int64_t n_groups, gpw;
   size_t shm;
+  const int Th = prog_host[kHdrT];
+  if (Th < 6 || Th > 9) { set_error("tqsv_jit_stage_adjoint: compiled stages need 64..512 threads (6 <= T <= 9)"); return E_ARG; }
   int rc = stage_launch_shape(prog_host, n, bsz, grid_cap, T, n_groups, grid, gpw, shm);
   if (rc) return rc;
-  if (T < 6) { set_error("tqsv_jit_stage_adjoint: compiled stages need >= 64 threads (T >= 6)"); return E_ARG; }
   void* p = psi;
   void* l = lam;
   const void* m = mats;
