@@ -71,6 +71,28 @@ __device__ __forceinline__ float redot(float2 a, float2 b) { return fmaf(a.x, b.
 __device__ __forceinline__ float2 csel(bool c, float2 a, float2 b) {
   return make_float2(c ? a.x : b.x, c ? a.y : b.y);
 }
+// Cross-lane sums without the LDS crossbar (gfx950): v_permlane32_swap
+// exchanges the upper 32 lanes of a with the lower 32 lanes of b, so a' + b'
+// holds, in lanes 0-31, a summed over lanes (l, l + 32) and, in lanes 32-63,
+// the same for b.  v_permlane16_swap does this for the odd 16-lane rows of a
+// and the even rows of b.  DPP (row_ror:8, row_half_mirror, quad_perm) then
+// reduces within a 16-lane row.
+__device__ __forceinline__ float sum32(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float sum16(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float dpp_ror8(float v) { return __builtin_amdgcn_update_dpp(0.f, v, 0x128, 0xf, 0xf, false); }
+// all-reduce over the 8-lane halves of each row
+__device__ __forceinline__ float allsum8(float v) {
+  v += __builtin_amdgcn_update_dpp(0.f, v, 0x141, 0xf, 0xf, false);   // row_half_mirror: l <-> 7 - l
+  v += __builtin_amdgcn_update_dpp(0.f, v, 0x4E, 0xf, 0xf, false);    // quad_perm [2,3,0,1]: l ^ 2
+  v += __builtin_amdgcn_update_dpp(0.f, v, 0xB1, 0xf, 0xf, false);    // quad_perm [1,0,3,2]: l ^ 1
+  return v;
+}
 """
 
 ONE = "make_float2(1.f, 0.f)"
@@ -555,6 +577,11 @@ class _Gen:
         raise ValueError(f"unknown stage op {t}")
 
     def reduce(self, w, g: List[str]):
+        """Wave totals of a gradient op's per-lane partials, added into the
+        workgroup's LDS accumulators.  Lane l ends with the total of value
+        l >> 3 (8 values), l >> 4 (4), l >> 5 (2) or of the only value --
+        the assignment of the interpreter's halving exchange -- built from
+        permlane swaps and DPP, with no LDS traffic and no selects."""
         Pp = 8 if (w[0] & F_MG) else ((w[0] >> 4) & 15)
         if Pp > 4:
             nv, sh, cond = 8, 3, "(lane & 7) == 0"
@@ -564,24 +591,29 @@ class _Gen:
             nv, sh, cond = 2, 5, "(lane & 31) == 0"
         else:
             nv, sh, cond = 1, 6, "lane == 0"
-        # wave_reduce_t of the interpreter, unrolled on named values: log2(nv)
-        # halving exchanges (each lane keeps one half, forwards the other),
-        # then a shuffle tree on the one value left
         a = (list(g) + ["0.f"] * nv)[:nv]
-        cnt, s = nv, 32
-        while cnt > 1:
-            half = cnt >> 1
-            hi = self.var(f"(lane & {s}) != 0", "bool")
-            nxt = []
-            for i in range(half):
-                send = self.var(f"{hi} ? {a[i]} : {a[i + half]}", "float")
-                keep = f"({hi} ? {a[i + half]} : {a[i]})"
-                nxt.append(self.var(f"{keep} + __shfl_xor({send}, {s}, 64)", "float"))
-            a, cnt, s = nxt, half, s >> 1
-        c = a[0]
-        while s >= 1:
-            c = self.var(f"{c} + __shfl_xor({c}, {s}, 64)", "float")
-            s >>= 1
+        f = lambda e: self.var(e, "float")
+        if nv == 8:
+            w4 = [f(f"sum32({a[i]}, {a[i + 4]})") for i in range(4)]      # lanes < 32: value i, else i + 4
+            u0 = f(f"sum16({w4[0]}, {w4[2]})")                          # rows: values 0, 2, 4, 6
+            u1 = f(f"sum16({w4[1]}, {w4[3]})")                          # rows: values 1, 3, 5, 7
+            s0 = f(f"{u0} + dpp_ror8({u0})")
+            s1 = f(f"{u1} + dpp_ror8({u1})")
+            z = f(f"(lane & 8) ? {s1} : {s0}")                       # half h of row r: value 2 r + h
+            c = f(f"allsum8({z})")
+        elif nv == 4:
+            w0 = f(f"sum32({a[0]}, {a[2]})")
+            w1 = f(f"sum32({a[1]}, {a[3]})")
+            u = f(f"sum16({w0}, {w1})")                                  # row r: value r
+            c = f(f"allsum8({u} + dpp_ror8({u}))")
+        elif nv == 2:
+            w0 = f(f"sum32({a[0]}, {a[1]})")                             # lanes < 32: value 0
+            u = f(f"sum16({w0}, {w0})")
+            c = f(f"allsum8({u} + dpp_ror8({u}))")
+        else:
+            w0 = f(f"sum32({a[0]}, {a[0]})")
+            u = f(f"sum16({w0}, {w0})")
+            c = f(f"allsum8({u} + dpp_ror8({u}))")
         self.line(f"if ({cond}) atomicAdd(&gacc[{w[7]} + (lane >> {sh})], {c});")
 
 
