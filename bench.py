@@ -45,14 +45,14 @@ def build_config2(q, ops, tqf):
                                        torch.tensor(p, dtype=torch.float32, device=q.device))
 
 
-def time_forward(n, bsz, ops, mode, steps, warmup, tq):
+def time_forward(n, bsz, ops, mode, steps, warmup, tq, jit="auto"):
     import torchquantum_amd.functional as tqf
     params = [None if op["params"] is None else torch.tensor(op["params"], dtype=torch.float32, device="cuda")
               for op in ops]
 
     def step():
         q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
-        q.engine_options = tq.EngineOptions(mode=mode)
+        q.engine_options = tq.EngineOptions(mode=mode, jit=jit)
         for op, p in zip(ops, params):
             tqf.func_name_dict[op["name"]](q, op["wires"], params=p)
         return tq.MeasureAll(tq.PauliZ)(q)
@@ -70,7 +70,7 @@ def time_forward(n, bsz, ops, mode, steps, warmup, tq):
     return dt, out
 
 
-def time_gradient(n, bsz, layers, mode, steps, warmup, tq):
+def time_gradient(n, bsz, layers, mode, steps, warmup, tq, jit="auto"):
     from circuits import config3_ops, config3_observables
     import torchquantum_amd.functional as tqf
     ops = config3_ops(seed=0, n=n, layers=layers)
@@ -85,7 +85,7 @@ def time_gradient(n, bsz, layers, mode, steps, warmup, tq):
         for p in params:
             p.grad = None
         q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
-        q.engine_options = tq.EngineOptions(mode=mode)
+        q.engine_options = tq.EngineOptions(mode=mode, jit=jit)
         for w in range(n):
             tqf.ry(q, w, params=phi)
         k = 0
@@ -395,6 +395,8 @@ def main():
     log("[bench] config2 fused forward")
     dt_f, ev_f = time_forward(n, bsz, ops, "fused", args.steps, args.warmup, tq)
     log(f"[bench] fused {dt_f * 1e3:.2f} ms/step")
+    dt_fi, _ = time_forward(n, bsz, ops, "fused", args.steps, args.warmup, tq, jit="off")
+    log(f"[bench] fused, program-interpreting stage kernels {dt_fi * 1e3:.2f} ms/step")
     parity = float((ev_pg - ev_f).abs().max())
 
     log("[bench] kernel roofline")
@@ -411,12 +413,15 @@ def main():
         log(f"[bench] adjoint gradient fused n={an} bsz={ab}")
         dt_af, loss_f, g_f = time_gradient(an, ab, 10, "fused", args.adj_steps, 1, tq)
         log(f"[bench] adjoint fused {dt_af:.3f} s/grad")
+        dt_ai, _, _ = time_gradient(an, ab, 10, "fused", 1, 1, tq, jit="off")
+        log(f"[bench] adjoint fused, program-interpreting stage kernels {dt_ai:.3f} s/grad")
         log(f"[bench] adjoint per-gate n={an} bsz={ab}")
         dt_ap, loss_p, g_p = time_gradient(an, ab, 10, "pergate", args.adj_steps, 1, tq)
         log(f"[bench] adjoint per-gate {dt_ap:.3f} s/grad")
         extra["adjoint"] = {
             "workload": f"{an} qubits, bsz {ab}, 10 x [RX,RY,RZ all wires, CNOT ring], 720 params",
             "fused_grads_per_s": 1.0 / dt_af, "pergate_grads_per_s": 1.0 / dt_ap,
+            "fused_interpreting_stage_kernels_grads_per_s": 1.0 / dt_ai,
             "speedup_vs_pergate": dt_ap / dt_af,
             "grad_max_abs_diff_fused_vs_pergate": float((g_f - g_p).abs().max()),
             "loss": loss_f,
@@ -462,7 +467,8 @@ def main():
         "config": {"workload": "20 qubits, bsz 16, 20 x [RY all wires, CZ even+odd pairs], <Z_i>, per-gate "
                                "kernels, forward", "n_qubits": n, "bsz": bsz, "gates": len(ops)},
         "fused": {"amp_updates_per_s": upd_per_step / dt_f, "ms_per_step": dt_f * 1e3,
-                  "speedup_vs_pergate": dt_pg / dt_f, "expval_max_abs_diff_vs_pergate": parity},
+                  "speedup_vs_pergate": dt_pg / dt_f, "expval_max_abs_diff_vs_pergate": parity,
+                  "ms_per_step_interpreting_stage_kernels": dt_fi * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (load_traffic() or {}).get("k_1q_20q_bsz16_bytes_per_launch"),
