@@ -291,13 +291,15 @@ __global__ __launch_bounds__(256) void k_diag(float2* __restrict__ st, uint64_t 
   }
 }
 
-// bit 0 free of controls: items are float4 (two adjacent amplitudes, each
-// with its own parity) with the control bits removed (ins4/cval4 in float4
-// bit positions; no controls = every float4 of the state)
+// items are float4 (two adjacent amplitudes, each with its own parity) with
+// the control bits above bit 0 removed (ins4/cval4 in float4 bit positions;
+// no controls = every float4 of the state).  c0: -1 = bit 0 is no control;
+// 0 / 1 = bit 0 is a control with that value, so only that amplitude of the
+// pair changes (CZ or CCZ on wire n-1 still moves 16-byte items).
 template <bool BATCHED, bool NT>
 __global__ __launch_bounds__(256) void k_diag4(float4* __restrict__ st, uint64_t n_items, InsertSpec ins4,
                                               uint64_t cval4, uint64_t pmask, const float2* __restrict__ ph,
-                                              int64_t bstride, int lg_items_row, int reps) {
+                                              int64_t bstride, int lg_items_row, int reps, int c0) {
   uint64_t it = tile_first(reps);
   for (int r = 0; r < reps; ++r, it += 256) {
     if (it >= n_items) return;
@@ -308,8 +310,9 @@ __global__ __launch_bounds__(256) void k_diag4(float4* __restrict__ st, uint64_t
     const int par0 = __popcll(i & pmask) & 1;
     const int par1 = __popcll((i | 1ull) & pmask) & 1;
     const float4 v = ld4<NT>(st + f);
-    const float2 a = cmul(par0 ? f1 : f0, make_float2(v.x, v.y));
-    const float2 b = cmul(par1 ? f1 : f0, make_float2(v.z, v.w));
+    float2 a = make_float2(v.x, v.y), b = make_float2(v.z, v.w);
+    if (c0 != 1) a = cmul(par0 ? f1 : f0, a);
+    if (c0 != 0) b = cmul(par1 ? f1 : f0, b);
     st4<NT>(st + f, make_float4(a.x, a.y, b.x, b.y));
   }
 }
@@ -687,13 +690,16 @@ TQSV_API int tqsv_apply_diag(hipStream_t s, void* state, int64_t bsz, int n, uin
   const bool b = ph_bstride != 0;
   const float2* p = (const float2*)phases;
   InsertSpec ins;
-  if (n >= 1 && !(cmask & 1ull)) {
-    if (!build_insert(ins, cmask >> 1, n - 1)) { set_error("too many control bits (max 12)"); return E_LIMIT; }
-    const int lg = n - 1 - c;
+  if (n >= 1) {
+    // float4 items; a control on bit 0 is applied inside the item
+    const uint64_t cm4 = (cmask & ~1ull) >> 1;
+    const int c0 = (cmask & 1ull) ? (int)(cval & 1ull) : -1;
+    if (!build_insert(ins, cm4, n - 1)) { set_error("too many control bits (max 12)"); return E_LIMIT; }
+    const int lg = n - 1 - __builtin_popcountll(cm4);
     const uint64_t n_items = (uint64_t)bsz << lg;
     const Tiles t = tiles_for(n_items);
-    LAUNCH_BN(k_diag4, b, stream_nt(bsz, n), t, s, (float4*)state, n_items, ins, cval >> 1, pmask, p, ph_bstride,
-              lg, t.reps);
+    LAUNCH_BN(k_diag4, b, stream_nt(bsz, n), t, s, (float4*)state, n_items, ins, (cval & ~1ull) >> 1, pmask, p,
+              ph_bstride, lg, t.reps, c0);
     return check_launch("tqsv_apply_diag");
   }
   if (!build_insert(ins, cmask, n)) { set_error("too many control bits (max 12)"); return E_LIMIT; }
