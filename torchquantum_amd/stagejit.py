@@ -39,12 +39,6 @@ from .engine import (H_NOPS, H_M, H_T, H_NNL, H_OFF_OPS, H_NGP, H_OFF_GSLOT, H_P
 NS = 1 << R_BITS
 MIN_T = 6            # compiled stages need full 64-lane waves
 KERNELS_PER_SOURCE = 4
-# Code-shape knobs: workgroups per CU the compiler must fit (0 = its own
-# choice; bounds the registers per lane), and whether the adjoint moves psi
-# and lam through LDS in one exchange (two LDS buffers) instead of two.
-FWD_MIN_WG = 0
-ADJ_MIN_WG = 0
-ADJ_PAIRED_REMAP = False
 
 PRELUDE = r"""
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -220,29 +214,6 @@ class _Gen:
             out.append(self.var(f"{lds}[{rb} ^ {swz(lr)}u]"))
         return out
 
-    def remap2(self, p: List[str], l: List[str], frm: Sequence[int], to: Sequence[int]):
-        """remap of psi and lam through two LDS buffers in one exchange."""
-        T = self.sp.T
-        ra, rb = self.lane_index(frm), self.lane_index(to)
-        self.line("__syncthreads();")
-        for j in range(NS):
-            lr = 0
-            for k in range(R_BITS):
-                if (j >> k) & 1:
-                    lr |= frm[T + k]
-            self.line(f"lds[{ra} ^ {swz(lr)}u] = {p[j]};")
-            self.line(f"lds2[{ra} ^ {swz(lr)}u] = {l[j]};")
-        self.line("__syncthreads();")
-        po, lo = [], []
-        for j in range(NS):
-            lr = 0
-            for k in range(R_BITS):
-                if (j >> k) & 1:
-                    lr |= to[T + k]
-            po.append(self.var(f"lds[{rb} ^ {swz(lr)}u]"))
-            lo.append(self.var(f"lds2[{rb} ^ {swz(lr)}u]"))
-        return po, lo
-
     # ---- forward ops (k_stage_fwd's fwd_op)
     def f_xperm(self, w, a: List[str]):
         K = w[3]
@@ -297,7 +268,7 @@ class _Gen:
             else:
                 a[j] = self.var(f"make_float2({tk} ? -{x}.x : {x}.x, {tk} ? -{x}.y : {x}.y)")
 
-    def diag_factors(self, w, M: str, tk: Optional[str], conj_unused=False):
+    def diag_factors(self, w, M: str, tk: Optional[str]):
         """(fA, fB): the phase for amplitudes whose register-bit parity is even
         / odd, per lane (thread-bit parity folded in; identity where the
         thread-bit controls fail)."""
@@ -668,8 +639,7 @@ def gen_forward(words: np.ndarray, name: str) -> str:
             g.fwd_op(w, a)
     body = g.lines
     nt = 1 << sp.T
-    lb = f"{nt}, {FWD_MIN_WG}" if FWD_MIN_WG else f"{nt}"
-    src = [f'extern "C" __global__ void __launch_bounds__({lb}) {name}(float2* __restrict__ out, '
+    src = [f'extern "C" __global__ void __launch_bounds__({nt}) {name}(float2* __restrict__ out, '
            "const float2* __restrict__ in, const float2* __restrict__ mats, int init_zero, long long n_groups, "
            "long long gpw) {"]
     if has_remap:
@@ -710,11 +680,8 @@ def gen_adjoint(words: np.ndarray, name: str) -> str:
     for w in reversed(sp.ops):
         if (w[0] & 15) == OP_REMAP:
             frm, to = sp.remap_layouts(w)
-            if ADJ_PAIRED_REMAP:
-                p, l = g.remap2(p, l, to, frm)
-            else:
-                p = g.remap(p, to, frm)
-                l = g.remap(l, to, frm)
+            p = g.remap(p, to, frm)
+            l = g.remap(l, to, frm)
             has_remap = True
             continue
         parts = g.bwd_op(w, p, l)
@@ -725,14 +692,11 @@ def gen_adjoint(words: np.ndarray, name: str) -> str:
     src = []
     if ngp:
         src.append(f"__constant__ int {name}_gs[{ngp}] = {{{', '.join(str(int(x)) for x in sp.gslots)}}};")
-    lb = f"{nt}, {ADJ_MIN_WG}" if ADJ_MIN_WG else f"{nt}"
-    src.append(f'extern "C" __global__ void __launch_bounds__({lb}) {name}(float2* __restrict__ psi, '
+    src.append(f'extern "C" __global__ void __launch_bounds__({nt}) {name}(float2* __restrict__ psi, '
                "float2* __restrict__ lam, const float2* __restrict__ mats, float* __restrict__ grad, "
                "long long grad_stride, long long n_groups, long long gpw) {")
     if has_remap:
         src.append(f"  __shared__ float2 lds[{1 << sp.m}];")
-        if ADJ_PAIRED_REMAP:
-            src.append(f"  __shared__ float2 lds2[{1 << sp.m}];")
     if ngp:
         src.append(f"  __shared__ float gacc[{ngp}];")
         src.append(f"  for (int t = threadIdx.x; t < {ngp}; t += {nt}) gacc[t] = 0.f;")
@@ -787,9 +751,7 @@ def gen_adjoint(words: np.ndarray, name: str) -> str:
 # compile + cache
 # ----------------------------------------------------------------------------
 def kernel_name(kind: str, words: np.ndarray) -> str:
-    hs = hashlib.sha1(np.ascontiguousarray(words, dtype=np.int32).tobytes())
-    hs.update(repr((FWD_MIN_WG, ADJ_MIN_WG, ADJ_PAIRED_REMAP)).encode())
-    h = hs.hexdigest()[:20]
+    h = hashlib.sha1(np.ascontiguousarray(words, dtype=np.int32).tobytes()).hexdigest()[:20]
     return f"tqsv_j{kind}_{h}"
 
 
