@@ -97,9 +97,11 @@ int tqsv_stage_adjoint(tqsv_stream_t s, void* psi, void* lam, int64_t bsz, int n
  *      for gfx950, and launched with the launch shape and argument meaning of
  *      tqsv_stage_forward / tqsv_stage_adjoint above (same reference
  *      counterparts).  Needs 2^T >= 64 threads per group (m >= 10). ------- */
-/* source -> code object (no GPU needed).  code may be NULL to query *size;
- * log (optional) receives the compiler log. */
-int tqsv_jit_compile(const char* src, void* code, int64_t cap, int64_t* size, char* log, int64_t log_cap);
+/* source -> code object (no GPU needed).  opts (optional): extra compiler
+ * options separated by spaces.  code may be NULL to query *size; log
+ * (optional) receives the compiler log. */
+int tqsv_jit_compile(const char* src, const char* opts, void* code, int64_t cap, int64_t* size, char* log,
+                     int64_t log_cap);
 /* load a code object on the current device; funcs[i] = kernel names[i]. */
 int tqsv_jit_load(const void* code, int64_t size, int n, const char* const* names, void** module, void** funcs);
 int tqsv_jit_unload(void* module);

@@ -57,7 +57,7 @@ SIGNATURES = {
     "tqsv_mg_to_grad": (c_int, [c_void_p, c_float_p, c_int64, c_int64, c_void_p, c_int, c_void_p]),
     "tqsv_sample_workspace_bytes": (c_int64, [c_int64, c_int]),
     "tqsv_sample": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int64, c_uint64, c_void_p, c_void_p, c_int64]),
-    "tqsv_jit_compile": (c_int, [ctypes.c_char_p, c_void_p, c_int64, c_i64_p, c_void_p, c_int64]),
+    "tqsv_jit_compile": (c_int, [ctypes.c_char_p, ctypes.c_char_p, c_void_p, c_int64, c_i64_p, c_void_p, c_int64]),
     "tqsv_jit_load": (c_int, [c_void_p, c_int64, c_int, POINTER(ctypes.c_char_p), POINTER(c_void_p),
                               POINTER(c_void_p)]),
     "tqsv_jit_unload": (c_int, [c_void_p]),

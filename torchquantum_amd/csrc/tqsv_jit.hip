@@ -37,16 +37,34 @@ const char* const kOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xcl
 
 }  // namespace
 
-// Compile HIP source to a gfx950 code object.  code/cap: output buffer (may
-// be null to query the size); *size receives the code object's size; log
+// Compile HIP source to a gfx950 code object.  opts (optional): extra
+// compiler options, separated by spaces (e.g. an LLVM scheduling strategy);
+// the library's own flags always apply.  code/cap: output buffer (may be
+// null to query the size); *size receives the code object's size; log
 // (optional) receives the compiler log, truncated to log_cap - 1 characters.
-TQSV_API int tqsv_jit_compile(const char* src, void* code, int64_t cap, int64_t* size, char* log, int64_t log_cap) {
+TQSV_API int tqsv_jit_compile(const char* src, const char* opts, void* code, int64_t cap, int64_t* size, char* log,
+                              int64_t log_cap) {
   if (!src || !size) { set_error("tqsv_jit_compile: null argument"); return E_ARG; }
   if (log && log_cap > 0) log[0] = 0;
+  std::vector<std::string> extra;
+  if (opts) {
+    std::string cur;
+    for (const char* c = opts; ; ++c) {
+      if (*c == ' ' || *c == 0) {
+        if (!cur.empty()) extra.push_back(cur);
+        cur.clear();
+        if (*c == 0) break;
+      } else {
+        cur += *c;
+      }
+    }
+  }
+  std::vector<const char*> all(kOpts, kOpts + sizeof(kOpts) / sizeof(kOpts[0]));
+  for (const std::string& e : extra) all.push_back(e.c_str());
   hiprtcProgram prog;
   hiprtcResult r = hiprtcCreateProgram(&prog, src, "tqsv_stage_jit.hip", 0, nullptr, nullptr);
   if (r != HIPRTC_SUCCESS) { set_error("tqsv_jit_compile: hiprtcCreateProgram: %s", hiprtcGetErrorString(r)); return E_LAUNCH; }
-  r = hiprtcCompileProgram(prog, (int)(sizeof(kOpts) / sizeof(kOpts[0])), kOpts);
+  r = hiprtcCompileProgram(prog, (int)all.size(), all.data());
   size_t ls = 0;
   hiprtcGetProgramLogSize(prog, &ls);
   std::string text(ls, '\0');
