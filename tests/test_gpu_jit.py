@@ -33,7 +33,8 @@ def _run(n, bsz, ops, **kw):
 
 
 @gpu
-@pytest.mark.parametrize("n,stage_bits,width", [(10, 0, 1), (12, 0, 1), (14, 10, 1), (13, 0, 3), (12, 0, 4)])
+@pytest.mark.parametrize("n,stage_bits,width", [(10, 0, 1), (12, 0, 1), (14, 10, 1), (13, 0, 3), (12, 0, 4),
+                                                (14, 13, 1), (13, 11, 2)])
 def test_jit_random_circuits(n, stage_bits, width):
     """Random circuits over every gate kind (controls on thread and register
     bits, diagonal phases, dense 2/3/4-bit blocks, per-row parameters)."""
@@ -51,15 +52,15 @@ def test_jit_random_circuits(n, stage_bits, width):
 
 
 @gpu
-@pytest.mark.parametrize("adjoint_mg", [True, False])
-def test_jit_adjoint_gradients(adjoint_mg):
+@pytest.mark.parametrize("adjoint_mg,stage_bits", [(True, 0), (False, 0), (True, 13), (True, 11)])
+def test_jit_adjoint_gradients(adjoint_mg, stage_bits):
     """Trainable gates of every parameter kind (1q rotations, controlled
     rotations, 2-wire rotations, diagonal phases), per-row parameters."""
     import torchquantum_amd as tq
     from torchquantum_amd import engine as E
     E._plan_cache.clear()
-    rng = np.random.default_rng(5 + int(adjoint_mg))
-    n, bsz = 10, 2
+    rng = np.random.default_rng(5 + int(adjoint_mg) + stage_bits)
+    n, bsz = (10, 2) if stage_bits == 0 else (stage_bits + 1, 2)
     names = ["rx", "ry", "rz", "u3", "rot", "cnot", "cz", "crx", "cry", "crz", "rzz", "rxx", "ryy", "rzx", "cu3",
              "u1", "phaseshift", "cu1", "h", "swap", "toffoli", "multirz", "xxplusyy", "singleexcitation", "cu",
              "u2", "r", "crot"]
@@ -68,7 +69,7 @@ def test_jit_adjoint_gradients(adjoint_mg):
     grads = {}
     for jit in ("on", "off"):
         q = tq.QuantumDevice(n_wires=n, bsz=bsz, device=DEV)
-        q.engine_options = tq.EngineOptions(jit=jit, adjoint_mg=adjoint_mg)
+        q.engine_options = tq.EngineOptions(jit=jit, adjoint_mg=adjoint_mg, stage_bits=stage_bits)
         ts = apply_ops_tq(q, ops, DEV, requires_grad=True)
         loss = tq.expval_pauli_sum(q, obs, coef).sum()
         loss.backward()
