@@ -13,6 +13,7 @@ from . import gates
 from . import functional
 from .module import QuantumModule, QuantumModuleList, QuantumModuleDict, static_support
 from .device import QuantumDevice, set_engine_options, get_engine_options
+from .noisedevice import NoiseDevice
 from .operator import *  # noqa: F401,F403
 from .operator import op_name_dict, fixed_ops, parameterized_ops, Operator, Operation, Observable
 from .measurement import (expval, MeasureAll, expval_joint_analytical, expval_joint_analytical_batch,

@@ -77,6 +77,9 @@ def run_tape(tape: Sequence[E.TapeOp], state_in: torch.Tensor, n: int, bsz: int,
     param_idx = [i for i, op in enumerate(tape) if op.params is not None and not op.is_matrix_param]
     params = [tape[i].params for i in param_idx]
     need = torch.is_grad_enabled() and (state_in.requires_grad or any(p.requires_grad for p in params))
+    if need and any(op.nonunitary for op in tape):
+        raise NotImplementedError("gradients through noise channels are not supported (the adjoint sweep "
+                                  "un-computes states with U^dagger, which a channel does not have)")
     if need:
         payload = _Payload(c, list(tape), init_zero, param_idx)
         return CircuitFunction.apply(payload, state_in, *params)

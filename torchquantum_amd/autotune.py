@@ -31,7 +31,7 @@ def circuit_key(tape: Sequence[E.TapeOp], n: int, bsz: int) -> str:
     h.update(f"{n}:{bsz}".encode())
     for op in tape:
         shape = None if op.params is None else tuple(op.params.shape)
-        h.update(f"{op.name}{op.wires}{op.inverse}{shape};".encode())
+        h.update(f"{op.name}{op.wires}{op.inverse}{op.conj}{shape};".encode())
     return h.hexdigest()
 
 

@@ -183,7 +183,13 @@ class QuantumDevice(nn.Module):
 
 def _bind_methods():
     from .functional import func_name_dict, func_name_dict_collect
+    from .noisedevice import NoiseDevice
     for name, fn in func_name_dict.items():
         setattr(QuantumDevice, name, fn)
     for name, fn in func_name_dict_collect.items():
         setattr(QuantumDevice, name, fn)
+    # the density device takes the same gate methods (noisedevices.py:101-105)
+    for d in (func_name_dict, func_name_dict_collect):
+        for name, fn in d.items():
+            if name not in NoiseDevice.__dict__:
+                setattr(NoiseDevice, name, fn)

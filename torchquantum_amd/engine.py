@@ -111,6 +111,8 @@ class TapeOp:
     params: Optional[torch.Tensor]  # [rows, P] real, or complex matrix [d,d] / [B,d,d]
     inverse: bool = False
     version: int = 0
+    conj: bool = False              # apply the complex conjugate of the matrix (density column index)
+    nonunitary: bool = False        # a channel's superoperator: no adjoint (gradient) sweep through it
 
     @property
     def is_matrix_param(self) -> bool:
@@ -376,24 +378,27 @@ class _GVLayout:
         for key, idx in groups.items():
             has_p = key[2]
             inv_rows = None
+            cj_rows = None
             if has_p:
                 off = 0
-                inv = []
+                inv, cj = [], []
                 for i in idx:
                     r = tape[i].params.shape[0]
                     self.grow[i], self.rows[i] = off, r
                     self.dwant[i] = tape[i].wants_grad()
                     inv.extend([tape[i].inverse] * r)
+                    cj.extend([tape[i].conj] * r)
                     off += r
                 inv_rows = np.array(inv, dtype=bool)
+                cj_rows = np.array(cj, dtype=bool)
                 P = tape[idx[0]].params.shape[-1]
             else:
                 P = 0
-                for i in idx:
-                    self.grow[i] = 1 if tape[i].inverse else 0
+                for i in idx:   # rows: U, U^dagger, conj(U), U^T
+                    self.grow[i] = (1 if tape[i].inverse else 0) + (2 if tape[i].conj else 0)
             for i in idx:
                 self.gkey[i] = key
-            self.groups.append((key, idx, P, inv_rows, bool(self.dwant[idx].any()) if has_p else False))
+            self.groups.append((key, idx, P, inv_rows, bool(self.dwant[idx].any()) if has_p else False, cj_rows))
 
 
 class _GroupViews:
@@ -445,9 +450,11 @@ class GateValues:
             m = m.reshape((-1,) + m.shape[-2:]) if m.ndim == 3 else m[None]
             if op.inverse:
                 m = np.conj(np.swapaxes(m, -1, -2))
+            if op.conj:
+                m = np.conj(m)
             self.UG[lay.gkey[i]] = m
             self.DG[lay.gkey[i]] = None
-        for key, idx, P, inv, any_grad in lay.groups:
+        for key, idx, P, inv, any_grad, cj in lay.groups:
             name, k, has_p = key
             builder = G.width_builder(name, k)
             if has_p:
@@ -465,10 +472,18 @@ class GateValues:
                     if D is not None:
                         D = D.copy()
                         D[inv] = np.conj(np.swapaxes(D[inv], -1, -2))
+                if cj.any():     # d conj(U) / d theta = conj(dU / d theta) for real theta
+                    U = U.copy()
+                    U[cj] = np.conj(U[cj])
+                    if D is not None:
+                        D = D.copy()
+                        D[cj] = np.conj(D[cj])
                 self.UG[key], self.DG[key] = U, D
             else:
                 u = builder(torch.zeros(1, 1, dtype=torch.float64)).numpy()
-                self.UG[key] = np.concatenate([u, np.conj(np.swapaxes(u, -1, -2))], axis=0)   # row 0: U, 1: U^dag
+                ud = np.conj(np.swapaxes(u, -1, -2))
+                # rows: 0 U, 1 U^dagger, 2 conj(U), 3 conj(U^dagger) = U^T
+                self.UG[key] = np.concatenate([u, ud, np.conj(u), np.conj(ud)], axis=0)
                 self.DG[key] = None
 
 
@@ -1307,7 +1322,7 @@ def structure_key(tape: Sequence[TapeOp], n: int, bsz: int, opts: EngineOptions,
             ps = ("m", tuple(op.params.shape))
         else:
             ps = (tuple(op.params.shape), bool(op.params.requires_grad))
-        items.append((op.name, op.wires, op.inverse, ps))
+        items.append((op.name, op.wires, op.inverse, op.conj, ps))
     return (n, bsz, opts.key(), str(device), tuple(items))
 
 

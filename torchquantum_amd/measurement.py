@@ -28,6 +28,7 @@ from . import operator as op
 from .autograd import ExpvalZAll, expval_pauli_strings
 from .macro import F_DTYPE
 from .module import QuantumModule
+from .noisedevice import NoiseDevice, density_expval_strings
 
 __all__ = ["expval", "MeasureAll", "expval_joint_analytical", "expval_joint_analytical_batch",
            "expval_pauli_sum", "MeasureMultiPauliSum", "gen_bitstrings", "sample_indices", "measure",
@@ -35,8 +36,18 @@ __all__ = ["expval", "MeasureAll", "expval_joint_analytical", "expval_joint_anal
 
 
 def _z_all(qdev) -> torch.Tensor:
+    if isinstance(qdev, NoiseDevice):     # Tr(rho Z_w)
+        n = qdev.n_wires
+        return density_expval_strings(qdev, ["I" * w + "Z" + "I" * (n - 1 - w) for w in range(n)])
     sv = qdev.get_states_1d()
     return ExpvalZAll.apply(sv, qdev.n_wires)
+
+
+def _norm(qdev) -> torch.Tensor:
+    """|psi|^2 per row, or tr(rho) for a density device."""
+    if isinstance(qdev, NoiseDevice):
+        return density_expval_strings(qdev, ["I" * qdev.n_wires])[:, 0]
+    return expval_pauli_strings(qdev.get_states_1d(), qdev.n_wires, ["I" * qdev.n_wires])[:, 0]
 
 
 def expval(qdev, wires: Union[int, List[int]], observables):
@@ -61,8 +72,7 @@ def expval(qdev, wires: Union[int, List[int]], observables):
     out = zc * coef[0]
     if bool((half_sum != 0).any()):
         # marginals sum to the squared norm, which need not be 1 (set_states, cu2, reset...)
-        norm = expval_pauli_strings(qdev.get_states_1d(), qdev.n_wires, ["I" * qdev.n_wires])[:, 0]
-        out = out + norm[:, None] * coef[1]
+        out = out + _norm(qdev)[:, None] * coef[1]
     return out
 
 
@@ -90,12 +100,16 @@ def expval_joint_analytical(qdev, observable: str) -> torch.Tensor:
     """<psi|P|psi> for one Pauli string (char k <-> wire k) -> [bsz]."""
     observable = observable.upper()
     assert len(observable) == qdev.n_wires
+    if isinstance(qdev, NoiseDevice):
+        return density_expval_strings(qdev, [observable])[:, 0]
     sv = qdev.get_states_1d()
     return expval_pauli_strings(sv, qdev.n_wires, [observable])[:, 0]
 
 
 def expval_joint_analytical_batch(qdev, observables: Sequence[str]) -> torch.Tensor:
     """Many Pauli strings in one node -> [bsz, len(observables)]."""
+    if isinstance(qdev, NoiseDevice):
+        return density_expval_strings(qdev, [o.upper() for o in observables])
     sv = qdev.get_states_1d()
     return expval_pauli_strings(sv, qdev.n_wires, [o.upper() for o in observables])
 
@@ -140,7 +154,12 @@ def sample_indices(qdev, n_shots: int = 1024, seed: Optional[int] = None) -> tor
     |psi|^2 -> int64 [bsz, n_shots] of in-row indices (wire 0 = most
     significant bit).  ``seed`` None draws one from torch's default generator,
     so ``torch.manual_seed`` makes the shots reproducible."""
-    sv = qdev.get_states_1d().detach()
+    if isinstance(qdev, NoiseDevice):
+        # outcome probabilities rho_ii drawn through amplitudes sqrt(rho_ii)
+        p = qdev.get_probs_1d().detach().real.clamp_min(0)
+        sv = torch.sqrt(p).to(torch.complex64)
+    else:
+        sv = qdev.get_states_1d().detach()
     if not sv.is_cuda:
         raise NotImplementedError("torchquantum_amd samples on the GPU only (the state is on the CPU)")
     sv = sv.to(torch.complex64).contiguous()
@@ -181,6 +200,9 @@ def _rotate_to_basis(qdev, observable: str):
     """A copy of ``qdev`` with each wire's diagonalizing gates applied
     (measurements.py:203-219)."""
     from .device import QuantumDevice
+    if isinstance(qdev, NoiseDevice):
+        raise NotImplementedError("sampled Pauli expectations of a density device: use "
+                                  "expval_joint_analytical (exact Tr(rho P))")
     pauli = {"X": op.PauliX, "Y": op.PauliY, "Z": op.PauliZ, "I": op.I}
     clone = QuantumDevice(n_wires=qdev.n_wires, bsz=qdev.bsz, device=qdev.get_states_1d().device)
     clone.engine_options = qdev.engine_options
