@@ -349,6 +349,8 @@ def main():
                     help="run only the dominant kernel's timing loops (for rocprofv3 summaries)")
     ap.add_argument("--config5", action="store_true", help="run the 30-qubit workload only")
     ap.add_argument("--only-dense", action="store_true", help="run only the dense k = 5, 6 block timings")
+    ap.add_argument("--only-fused", action="store_true",
+                    help="run only the fused forward of the 20-qubit config-2 circuit (--steps, --warmup)")
     ap.add_argument("--only-gradient", action="store_true",
                     help="run only the fused adjoint gradient (--adj-qubits, --adj-bsz, --adj-steps)")
     ap.add_argument("--c5-qubits", type=int, default=30)
@@ -380,6 +382,13 @@ def main():
         an, ab = args.adj_qubits, args.adj_bsz
         dt, loss, _ = time_gradient(an, ab, 10, "fused", args.adj_steps, 1, tq)
         print(json.dumps({"fused_gradient_s": dt, "workload": f"{an} qubits, bsz {ab}", "loss": loss}), flush=True)
+        return
+    if args.only_fused:
+        n, bsz = 20, 16
+        ops = config2_ops(seed=0, n=n, layers=20)
+        dt, _ = time_forward(n, bsz, ops, "fused", args.steps, args.warmup, tq)
+        print(json.dumps({"fused_forward_ms": dt * 1e3, "workload": f"{n} qubits, bsz {bsz}, config-2 circuit",
+                          "amp_updates_per_s": len(ops) * bsz * (1 << n) / dt}), flush=True)
         return
     if args.config5:
         print(json.dumps({"config5": config5(args, tq)}), flush=True)
