@@ -506,7 +506,9 @@ class _HostMirror:
 
     @staticmethod
     def _key(p: torch.Tensor):
-        base = p._base if p._base is not None else p
+        base = p._base
+        if base is None:          # the common case: the parameter tensor itself
+            return p, None, (p._version, p.data_ptr())
         view = (p.storage_offset(), tuple(p.shape), tuple(p.stride()), p.dtype)
         return base, view, (base._version, base.data_ptr())
 

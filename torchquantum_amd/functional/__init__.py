@@ -121,9 +121,30 @@ def _record(name, cname, s, q_device, wires, params, inverse, comp_method):
 def _make_gate(name: str):
     cname = G.canonical(name)
     spec = G.spec(cname)
+    nw, npar = spec.num_wires, spec.num_params
+    plain = cname not in _MATRIX_GATES and cname not in ("multicnot", "multixcnot") and nw is not None
 
     def gate(q_device, wires, params=None, n_wires=None, static=False, parent_graph=None, inverse=False,
              comp_method="bmm"):
+        # fast path, same checks as _record: a fixed-width gate on int wires
+        # with no parameter or a real [rows, P] tensor, no op history
+        if plain and comp_method == "bmm" and not getattr(q_device, "record_op", False):
+            if type(wires) is int:
+                ws = (wires,)
+            elif type(wires) in (list, tuple) and all(type(w) is int for w in wires):
+                ws = tuple(wires)
+            else:
+                ws = None
+            if ws is not None and len(ws) == nw and (nw == 1 or len(set(ws)) == nw):
+                n = q_device.n_wires
+                if all(0 <= w < n for w in ws):
+                    if npar == 0:
+                        q_device._push(TapeOp(cname, ws, None, bool(inverse), 0))
+                        return
+                    if (type(params) is torch.Tensor or isinstance(params, torch.Tensor)) and params.dim() == 2 \
+                            and params.shape[-1] == npar and not params.is_complex():
+                        q_device._push(TapeOp(cname, ws, params, bool(inverse), params._version))
+                        return
         _record(name, cname, spec, q_device, wires, params, inverse, comp_method)
 
     gate.__name__ = name
