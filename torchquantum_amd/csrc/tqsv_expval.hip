@@ -60,22 +60,31 @@ __global__ __launch_bounds__(EV_THREADS) void k_ez_part(const float4* __restrict
   const uint64_t chunk = blockIdx.x & (nchunks - 1);
   const uint64_t first = (row << n) + (chunk << lg_chunk);  // amplitude index
   const uint64_t npairs = (1ull << lg_chunk) >> 1;
+  // Thread t reads float4 i = t + 256 k, i.e. amplitudes j = chunk_base +
+  // 2 t + 512 k (+1): bit 0 of j is the float4 half, bits 1..8 are bits of t
+  // (fixed per thread: those sums follow from the thread's total at the end)
+  // and bits >= 9 are bits of the wave-uniform chunk_base + 512 k.  The chunk
+  // is at least 2^11 amplitudes, so the split holds.
   float S[EV_MAXQ];
 #pragma unroll
   for (int q = 0; q < EV_MAXQ; ++q) S[q] = 0.f;
-  float tot = 0.f;
-  for (uint64_t i = threadIdx.x; i < npairs; i += blockDim.x) {
+  float tot = 0.f, s0 = 0.f;
+  for (uint64_t i = threadIdx.x, k = 0; i < npairs; i += blockDim.x, ++k) {
     const float4 v = st[(first >> 1) + i];
     const float p0 = v.x * v.x + v.y * v.y;
     const float p1 = v.z * v.z + v.w * v.w;
-    const uint32_t j = (uint32_t)(((chunk << lg_chunk) + 2 * i));  // in-row index of p0 (n <= 32)
+    const uint32_t jhi = (uint32_t)((chunk << lg_chunk) + 512ull * k);   // wave-uniform
     const float pp = p0 + p1;
     tot += pp;
-    S[0] += p1;
+    s0 += p1;
 #pragma unroll
-    for (int q = 1; q < EV_MAXQ; ++q)
-      if (q < n) S[q] += ((j >> q) & 1u) ? pp : 0.f;
+    for (int q = 9; q < EV_MAXQ; ++q)
+      if (q < n && ((jhi >> q) & 1u)) S[q] += pp;
   }
+  S[0] = s0;
+#pragma unroll
+  for (int q = 1; q <= 8; ++q)
+    if (q < n) S[q] = ((threadIdx.x >> (q - 1)) & 1u) ? tot : 0.f;
   float* out = part + (uint64_t)blockIdx.x * (n + 1);
 #pragma unroll
   for (int q = 0; q < EV_MAXQ; ++q) {
