@@ -110,9 +110,13 @@ int tqsv_jit_unload(void* module);
 int tqsv_jit_stage_forward(tqsv_stream_t s, const void* func, void* state_out, const void* state_in, int64_t bsz,
                            int n, const int* prog_host, const void* mats, int init_zero, int grid_cap);
 /* func: adjoint kernel compiled from prog_host's program (parameters: psi,
- * lam, mats, grad, grad_stride, n_groups, groups_per_wg). */
-int tqsv_jit_stage_adjoint(tqsv_stream_t s, const void* func, void* psi, void* lam, int64_t bsz, int n,
-                           const int* prog_host, const void* mats, float* grad, int64_t grad_stride, int grid_cap);
+ * lam, psi_in, lam_in, mats, grad, grad_stride, n_groups, groups_per_wg).
+ * psi_in / lam_in: where psi and lam are read (NULL = in place), so the first
+ * stage of a sweep can read the forward output and the incoming gradient
+ * without copying them. */
+int tqsv_jit_stage_adjoint(tqsv_stream_t s, const void* func, void* psi, void* lam, const void* psi_in,
+                           const void* lam_in, int64_t bsz, int n, const int* prog_host, const void* mats,
+                           float* grad, int64_t grad_stride, int grid_cap);
 
 /* ---- per-gate adjoint steps (the unfused gradient path); mat holds U then
  *      P derivative matrices dU/dtheta_p (P <= 4); grad slots gslot..gslot+P-1.

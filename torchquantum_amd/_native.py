@@ -63,8 +63,8 @@ SIGNATURES = {
     "tqsv_jit_unload": (c_int, [c_void_p]),
     "tqsv_jit_stage_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int_p, c_void_p,
                                        c_int, c_int]),
-    "tqsv_jit_stage_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int_p, c_void_p,
-                                       c_float_p, c_int64, c_int]),
+    "tqsv_jit_stage_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int,
+                                       c_int_p, c_void_p, c_float_p, c_int64, c_int]),
     "tqsv_gemm": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
                           c_float, c_void_p, c_int, c_void_p, c_int]),
     "tqsv_relu_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),

@@ -45,9 +45,8 @@ class CircuitFunction(torch.autograd.Function):
         (out,) = ctx.saved_tensors
         payload = ctx.payload
         c = payload.c
-        psi = out.clone()
-        lam = gout.to(torch.complex64).reshape(out.shape).contiguous().clone()
-        grad = E.adjoint(c, psi, lam, ctx.pool)
+        lam_in = gout.to(torch.complex64).reshape(out.shape).contiguous()
+        grad, lam = E.adjoint(c, out, lam_in, ctx.pool)
         gsum = None            # batch-row sums for parameters shared by all rows, one reduction
         grads: List = []
         for j, ti in enumerate(payload.param_idx):

@@ -156,11 +156,12 @@ TQSV_API int tqsv_jit_stage_forward(hipStream_t s, const void* func, void* state
 }
 
 // One stage of the adjoint sweep with a compiled stage kernel.  Arguments as
-// tqsv_stage_adjoint (kernel parameters: psi, lam, mats, grad, grad_stride,
-// n_groups, groups_per_wg).
-TQSV_API int tqsv_jit_stage_adjoint(hipStream_t s, const void* func, void* psi, void* lam, int64_t bsz, int n,
-                                    const int* prog_host, const void* mats, float* grad, int64_t grad_stride,
-                                    int grid_cap) {
+// tqsv_stage_adjoint, plus psi_in / lam_in: where psi and lam are read (NULL =
+// in place); the swept states are written to psi / lam.  Kernel parameters:
+// psi, lam, psi_in, lam_in, mats, grad, grad_stride, n_groups, groups_per_wg.
+TQSV_API int tqsv_jit_stage_adjoint(hipStream_t s, const void* func, void* psi, void* lam, const void* psi_in,
+                                    const void* lam_in, int64_t bsz, int n, const int* prog_host, const void* mats,
+                                    float* grad, int64_t grad_stride, int grid_cap) {
   if (!func || !psi || !lam || !prog_host || !mats) {
     set_error("tqsv_jit_stage_adjoint: null pointer argument");
     return E_ARG;
@@ -176,10 +177,12 @@ TQSV_API int tqsv_jit_stage_adjoint(hipStream_t s, const void* func, void* psi, 
   if (rc) return rc;
   void* p = psi;
   void* l = lam;
+  const void* pin = psi_in ? psi_in : psi;
+  const void* lin = lam_in ? lam_in : lam;
   const void* m = mats;
   float* g = grad;
   long long gs = grad_stride, ng = n_groups, gp = gpw;
-  void* args[] = {&p, &l, &m, &g, &gs, &ng, &gp};
+  void* args[] = {&p, &l, &pin, &lin, &m, &g, &gs, &ng, &gp};
   hipError_t e = hipModuleLaunchKernel((hipFunction_t)func, grid, 1, 1, 1u << T, 1, 1, 0, s, args, nullptr);
   if (e != hipSuccess) { set_error("tqsv_jit_stage_adjoint: %s", hipGetErrorString(e)); return E_LAUNCH; }
   return OK;

@@ -1432,21 +1432,33 @@ def forward(c: Compiled, state_in: torch.Tensor, pool: torch.Tensor, init_zero: 
     return out
 
 
-def adjoint(c: Compiled, psi: torch.Tensor, lam: torch.Tensor, pool: torch.Tensor) -> torch.Tensor:
-    """Reverse sweep in place on psi (forward output) and lam (dL/dpsi).
-    Returns grads [bsz, n_slots] float32."""
+def adjoint(c: Compiled, psi_src: torch.Tensor, lam_src: torch.Tensor, pool: torch.Tensor):
+    """Reverse sweep from psi_src (forward output) and lam_src (dL/dpsi),
+    which are left unchanged.  Returns (grads [bsz, n_slots] float32, lam
+    after the sweep = dL/d(input state)).  When the sweep starts with a
+    compiled stage, that stage reads psi_src / lam_src and writes fresh
+    buffers; otherwise they are copied first."""
     n, bsz = c.n, c.bsz
     grad = torch.zeros((bsz, c.grad_width), dtype=torch.float32, device=c.device)
-    stream = N.stream_of(psi)
+    stream = N.stream_of(psi_src)
     gstride = grad.shape[1]
     jit = c.jit_kernels("a") if c.prog_host is not None else None
-    for kind, payload in reversed(c.steps):
+    steps = list(reversed(c.steps))
+    first_src = None
+    if steps and steps[0][0] == "stage" and jit and jit.get(steps[0][1][0]) is not None:
+        psi, lam = torch.empty_like(psi_src), torch.empty_like(lam_src)
+        first_src = (psi_src.data_ptr(), lam_src.data_ptr())
+    else:
+        psi, lam = psi_src.clone(), lam_src.clone()
+    for kind, payload in steps:
         if kind == "stage":
             base, stg = payload
             hp = c.prog_host[base:]
             f = jit.get(base) if jit else None
             if f is not None:
-                N.call("tqsv_jit_stage_adjoint", stream, f, psi.data_ptr(), lam.data_ptr(), bsz, n,
+                pin, lin = first_src if first_src is not None else (None, None)
+                first_src = None
+                N.call("tqsv_jit_stage_adjoint", stream, f, psi.data_ptr(), lam.data_ptr(), pin, lin, bsz, n,
                        hp.ctypes.data_as(N.c_int_p), pool.data_ptr(), N.fptr(grad), gstride, c.opts.grid_cap)
             else:
                 N.call("tqsv_stage_adjoint", stream, psi.data_ptr(), lam.data_ptr(), bsz, n,
@@ -1488,4 +1500,4 @@ def adjoint(c: Compiled, psi: torch.Tensor, lam: torch.Tensor, pool: torch.Tenso
     for i, col, sl in c.scatter:
         idx = torch.as_tensor(sl, dtype=torch.long, device=grad.device)
         grad.index_add_(1, idx, grad[:, col:col + len(sl)].clone())
-    return grad[:, :max(c.n_slots, 1)]
+    return grad[:, :max(c.n_slots, 1)], lam

@@ -698,8 +698,12 @@ def gen_adjoint(words: np.ndarray, name: str) -> str:
     src = []
     if ngp:
         src.append(f"__constant__ int {name}_gs[{ngp}] = {{{', '.join(str(int(x)) for x in sp.gslots)}}};")
+    # psi / lam: where the swept states are written; psi_in / lam_in: where
+    # they are read (the same buffers in place, or the forward output and
+    # the incoming gradient for the first stage of the sweep)
     src.append(f'extern "C" __global__ void __launch_bounds__({nt}) {name}(float2* __restrict__ psi, '
-               "float2* __restrict__ lam, const float2* __restrict__ mats, float* __restrict__ grad, "
+               "float2* __restrict__ lam, const float2* psi_in, const float2* lam_in, "
+               "const float2* __restrict__ mats, float* __restrict__ grad, "
                "long long grad_stride, long long n_groups, long long gpw) {")
     if has_remap:
         src.append(f"  __shared__ float2 lds[{1 << sp.m}];")
@@ -732,9 +736,9 @@ def gen_adjoint(words: np.ndarray, name: str) -> str:
                 "      cur_row = (long long)row;",
                 "    }"]
     for j in range(NS):
-        src.append(f"    const float2 p{j} = {_amp(sp, 'psi', j, True)};")
+        src.append(f"    const float2 p{j} = {_amp(sp, 'psi_in', j, True)};")
     for j in range(NS):
-        src.append(f"    const float2 l{j} = {_amp(sp, 'lam', j, True)};")
+        src.append(f"    const float2 l{j} = {_amp(sp, 'lam_in', j, True)};")
     src += g.lines
     for j in range(NS):
         src.append(f"    {_amp(sp, 'psi', j, False)} = {p[j]};")
