@@ -40,10 +40,15 @@ def _opts_dict(o: E.EngineOptions) -> dict:
 
 
 def time_plan(tape, n, bsz, opts: E.EngineOptions, device="cuda", reps: int = 5) -> float:
-    """Median ms of one forward of the compiled plan (state from |0..0>)."""
+    """Median ms of one forward of the compiled plan (state from |0..0>),
+    with the kernels the plan runs in steady state: when its stages compile
+    at run time, the timing waits for them."""
     c = E.Compiled(tape, n, bsz, opts, torch.device(device))
     pool = c.pool(tape, with_grad=False, with_adj=False)
-    E.forward(c, None, pool, init_zero=True)          # warm-up
+    E.forward(c, None, pool, init_zero=True)          # warm-up (starts any stage compile)
+    if c._jit_jobs:
+        E.wait_for_compiled_stages()
+        E.forward(c, None, pool, init_zero=True)
     ts = []
     for _ in range(reps):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
