@@ -208,3 +208,38 @@ def test_density_channel_gradient_raises_and_sampling():
     frac = (idx >= 4).mean(axis=1)
     assert np.all(np.abs(frac - p1) < 5 * np.sqrt(p1 * (1 - p1) / 40000))
     assert np.all(idx % 4 == 0)      # wires 1, 2 stay |0>
+
+
+@gpu
+def test_density_compiled_stages():
+    """A 6-wire density (12-wire register, one compiled stage width) with
+    channels: compiled and interpreting stage kernels agree (1e-6) and match
+    the oracle (1e-5)."""
+    import torchquantum_amd as tq
+    from torchquantum_amd import engine as E, noisedevice as ND
+    E._plan_cache.clear()
+    rng = np.random.default_rng(6)
+    n, bsz = 6, 2
+    ops = random_ops(n, 30, bsz, rng, names=GATES)
+    channels = {4: ("depolarizing", 2, 0.1), 17: ("amplitude_damping", 5, 0.25)}
+    out = {}
+    for jit in ("on", "off"):
+        d = tq.NoiseDevice(n, bsz=bsz, device=DEV)
+        d.engine_options = tq.EngineOptions(jit=jit)
+        for i, op in enumerate(ops):
+            apply_ops_tq(d, [op], DEV)
+            if i in channels:
+                name, w, val = channels[i]
+                getattr(d, name)(w, val)
+        out[jit] = d.densities.reshape(bsz, 2 ** n, 2 ** n).cpu().numpy()
+    assert any(c._jit.get("f") for c in E._plan_cache.values() if c.opts.jit == "on")
+    kr = {"depolarizing": ND.depolarizing_kraus, "amplitude_damping": ND.amplitude_damping_kraus}
+    seq = []
+    for i, o in enumerate(oracle_ops(ops, bsz)):
+        seq.append(o)
+        if i in channels:
+            name, w, val = channels[i]
+            seq.append({"kraus": kr[name](val), "wires": [w]})
+    ref = OD.run_density_ops(seq, n, bsz)
+    assert np.abs(out["on"] - out["off"]).max() < 1e-6
+    assert np.abs(out["on"] - ref).max() < 1e-5
