@@ -351,6 +351,8 @@ def main():
     ap.add_argument("--only-dense", action="store_true", help="run only the dense k = 5, 6 block timings")
     ap.add_argument("--only-fused", action="store_true",
                     help="run only the fused forward of the 20-qubit config-2 circuit (--steps, --warmup)")
+    ap.add_argument("--only-pergate", action="store_true",
+                    help="run only the per-gate forward of the 20-qubit config-2 circuit (--steps, --warmup)")
     ap.add_argument("--only-gradient", action="store_true",
                     help="run only the fused adjoint gradient (--adj-qubits, --adj-bsz, --adj-steps)")
     ap.add_argument("--c5-qubits", type=int, default=30)
@@ -383,11 +385,12 @@ def main():
         dt, loss, _ = time_gradient(an, ab, 10, "fused", args.adj_steps, 1, tq)
         print(json.dumps({"fused_gradient_s": dt, "workload": f"{an} qubits, bsz {ab}", "loss": loss}), flush=True)
         return
-    if args.only_fused:
+    if args.only_fused or args.only_pergate:
         n, bsz = 20, 16
+        mode = "fused" if args.only_fused else "pergate"
         ops = config2_ops(seed=0, n=n, layers=20)
-        dt, _ = time_forward(n, bsz, ops, "fused", args.steps, args.warmup, tq)
-        print(json.dumps({"fused_forward_ms": dt * 1e3, "workload": f"{n} qubits, bsz {bsz}, config-2 circuit",
+        dt, _ = time_forward(n, bsz, ops, mode, args.steps, args.warmup, tq)
+        print(json.dumps({f"{mode}_forward_ms": dt * 1e3, "workload": f"{n} qubits, bsz {bsz}, config-2 circuit",
                           "amp_updates_per_s": len(ops) * bsz * (1 << n) / dt}), flush=True)
         return
     if args.config5:

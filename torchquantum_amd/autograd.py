@@ -73,14 +73,13 @@ def run_tape(tape: Sequence[E.TapeOp], state_in: torch.Tensor, n: int, bsz: int,
     device = state_in.device
     N.require_gpu(state_in, "the quantum state")
     c = E.compile_tape(tape, n, bsz, opts, device)
-    param_idx = [i for i, op in enumerate(tape) if op.params is not None and not op.is_matrix_param]
-    params = [tape[i].params for i in param_idx]
-    need = torch.is_grad_enabled() and (state_in.requires_grad or any(p.requires_grad for p in params))
+    need = torch.is_grad_enabled() and (state_in.requires_grad or c.any_param_grad)
     if need and any(op.nonunitary for op in tape):
         raise NotImplementedError("gradients through noise channels are not supported (the adjoint sweep "
                                   "un-computes states with U^dagger, which a channel does not have)")
     if need:
-        payload = _Payload(c, list(tape), init_zero, param_idx)
+        params = [tape[i].params for i in c.param_idx]
+        payload = _Payload(c, list(tape), init_zero, c.param_idx)
         return CircuitFunction.apply(payload, state_in, *params)
     with torch.no_grad():
         pool = c.pool(tape, with_grad=False, with_adj=False)
@@ -90,6 +89,12 @@ def run_tape(tape: Sequence[E.TapeOp], state_in: torch.Tensor, n: int, bsz: int,
 # ----------------------------------------------------------------------------
 # expectations
 # ----------------------------------------------------------------------------
+def _rows(state: torch.Tensor, bsz: int) -> torch.Tensor:
+    """[bsz, 2^n] contiguous and 16-byte aligned (the kernels read float4)."""
+    st = state.reshape(bsz, -1).contiguous()
+    return st if st.data_ptr() % 16 == 0 else st.clone()
+
+
 def _workspace(state: torch.Tensor, bsz: int, n: int, nterms: int) -> torch.Tensor:
     nbytes = int(N.lib().tqsv_expval_workspace_bytes(bsz, n, nterms))
     if nbytes < 0:
@@ -110,7 +115,7 @@ class ExpvalZAll(torch.autograd.Function):
     @staticmethod
     def forward(ctx, state: torch.Tensor, n: int):
         bsz = state.shape[0]
-        st = state.reshape(bsz, -1).contiguous()
+        st = _rows(state, bsz)
         out = torch.empty((bsz, n), dtype=torch.float32, device=state.device)
         ws = _workspace(st, bsz, n, n)
         N.call("tqsv_expval_z_all", N.stream_of(st), st.data_ptr(), bsz, n, N.fptr(out), ws.data_ptr(),
@@ -155,7 +160,7 @@ class ExpvalPauli(torch.autograd.Function):
     @staticmethod
     def forward(ctx, state: torch.Tensor, n: int, xs: tuple, zs: tuple, nys: tuple):
         bsz = state.shape[0]
-        st = state.reshape(bsz, -1).contiguous()
+        st = _rows(state, bsz)
         T = len(xs)
         out = torch.empty((bsz, T), dtype=torch.float32, device=state.device)
         ws = _workspace(st, bsz, n, T)

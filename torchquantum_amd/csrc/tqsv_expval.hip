@@ -26,6 +26,9 @@ constexpr int EV_THREADS = 256;
 constexpr int EV_MAXQ = 32;      // wires per <Z_w> launch (n <= 32)
 constexpr int EV_MAXT = 16;      // terms per diagonal-term launch
 
+// grid of 256-thread blocks for one wavefront per output
+static unsigned sum_grid(int64_t outputs) { return (unsigned)((outputs * 64 + 255) / 256); }
+
 // chunks per row so that the grid has enough blocks; chunk = power of two amps
 static int ev_chunk_log2(int64_t bsz, int n) {
   // at least 2 amps per thread per chunk, at most the whole row
@@ -50,6 +53,16 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
   return s;  // valid in thread 0
 }
 
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Loads in flight per thread in the chunk walks below: one 16-byte load per
+// trip leaves too few bytes in flight to cover HBM latency.
+constexpr int EV_UNROLL = 4;
+static_assert(EV_UNROLL == 4 && EV_THREADS == 256, "k_ez_part maps amplitude bits 9, 10 to the unroll slot");
+
 // <Z_w> partials: part[(row * nchunks + chunk) * (n + 1) + q], q < n: sum of p
 // over amplitudes with bit q set; q = n: total probability.
 __global__ __launch_bounds__(EV_THREADS) void k_ez_part(const float4* __restrict__ st, int n, int lg_chunk,
@@ -61,26 +74,46 @@ __global__ __launch_bounds__(EV_THREADS) void k_ez_part(const float4* __restrict
   const uint64_t first = (row << n) + (chunk << lg_chunk);  // amplitude index
   const uint64_t npairs = (1ull << lg_chunk) >> 1;
   // Thread t reads float4 i = t + 256 k, i.e. amplitudes j = chunk_base +
-  // 2 t + 512 k (+1): bit 0 of j is the float4 half, bits 1..8 are bits of t
-  // (fixed per thread: those sums follow from the thread's total at the end)
-  // and bits >= 9 are bits of the wave-uniform chunk_base + 512 k.  The chunk
-  // is at least 2^11 amplitudes, so the split holds.
+  // 2 t + 512 k (+1), all parts on disjoint bits: bit 0 of j is the float4
+  // half; bits 1..8 are bits of t (fixed per thread: those sums follow from
+  // the thread's total at the end); bits 9, 10 are bits of the unroll slot u
+  // (k = trip * 4 + u); bits 11 .. lg_chunk - 1 are bits of the trip
+  // (wave-uniform, one add per trip); bits >= lg_chunk are bits of the
+  // chunk (constant per block: taken from the total at the end).  A chunk
+  // shorter than 512 amplitudes is a whole row (n <= 11), where k = 0 and
+  // threads past its end read nothing.
   float S[EV_MAXQ];
 #pragma unroll
   for (int q = 0; q < EV_MAXQ; ++q) S[q] = 0.f;
   float tot = 0.f, s0 = 0.f;
-  for (uint64_t i = threadIdx.x, k = 0; i < npairs; i += blockDim.x, ++k) {
-    const float4 v = st[(first >> 1) + i];
-    const float p0 = v.x * v.x + v.y * v.y;
-    const float p1 = v.z * v.z + v.w * v.w;
-    const uint32_t jhi = (uint32_t)((chunk << lg_chunk) + 512ull * k);   // wave-uniform
-    const float pp = p0 + p1;
-    tot += pp;
-    s0 += p1;
+  for (uint64_t b = threadIdx.x, k = 0; b < npairs; b += EV_UNROLL * EV_THREADS, k += EV_UNROLL) {
+    float4 v[EV_UNROLL];
 #pragma unroll
-    for (int q = 9; q < EV_MAXQ; ++q)
-      if (q < n && ((jhi >> q) & 1u)) S[q] += pp;
+    for (int u = 0; u < EV_UNROLL; ++u) {
+      const uint64_t i = b + (uint64_t)u * EV_THREADS;
+      v[u] = i < npairs ? st[(first >> 1) + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    float ts = 0.f;
+#pragma unroll
+    for (int u = 0; u < EV_UNROLL; ++u) {
+      const float p0 = v[u].x * v[u].x + v[u].y * v[u].y;
+      const float p1 = v[u].z * v[u].z + v[u].w * v[u].w;
+      const float pp = p0 + p1;
+      ts += pp;
+      s0 += p1;
+      if (u & 1) S[9] += pp;
+      if (u & 2) S[10] += pp;
+    }
+    tot += ts;
+    const uint32_t kk = (uint32_t)k;   // wave-uniform, a multiple of 4
+#pragma unroll
+    for (int q = 11; q < EV_MAXQ; ++q)
+      if (q < lg_chunk && ((kk >> (q - 9)) & 1u)) S[q] += ts;
   }
+  const uint64_t cb = chunk << lg_chunk;
+#pragma unroll
+  for (int q = 9; q < EV_MAXQ; ++q)
+    if (q >= lg_chunk) S[q] = ((cb >> q) & 1u) ? tot : 0.f;
   S[0] = s0;
 #pragma unroll
   for (int q = 1; q <= 8; ++q)
@@ -97,20 +130,24 @@ __global__ __launch_bounds__(EV_THREADS) void k_ez_part(const float4* __restrict
   if (threadIdx.x == 0) out[n] = s;
 }
 
-// out[row * n + w] = total - 2 * S[n - 1 - w]
-__global__ void k_ez_final(const float* __restrict__ part, int n, int nchunks, int64_t bsz, float* __restrict__ out) {
-  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// out[row * n + w] = total - 2 * S[n - 1 - w]; one wavefront per output, its
+// lanes striding over the row's chunk partials.
+__global__ __launch_bounds__(256) void k_ez_final(const float* __restrict__ part, int n, int nchunks, int64_t bsz,
+                                                  float* __restrict__ out) {
+  const int64_t idx = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;   // wave-uniform
+  const int lane = threadIdx.x & 63;
   if (idx >= bsz * n) return;
   const int64_t row = idx / n;
-  const int w = (int)(idx % n);
-  const int q = n - 1 - w;
+  const int q = n - 1 - (int)(idx % n);
   double s = 0.0, t = 0.0;
-  for (int c = 0; c < nchunks; ++c) {
+  for (int c = lane; c < nchunks; c += 64) {
     const float* p = part + (row * nchunks + c) * (n + 1);
     s += p[q];
     t += p[n];
   }
-  out[idx] = (float)(t - 2.0 * s);
+  s = wave_sum_d(s);
+  t = wave_sum_d(t);
+  if (lane == 0) out[idx] = (float)(t - 2.0 * s);
 }
 
 // Diagonal Pauli strings (x = 0): sum_j p_j (-1)^{popcount(j & z_t)}, up to
@@ -120,30 +157,50 @@ struct ZMasks {
   uint32_t z[EV_MAXT];
 };
 
-__global__ __launch_bounds__(EV_THREADS) void k_ediag_part(const float2* __restrict__ st, int n, int lg_chunk,
+// Amplitude j = chunk_base + 512 k + 2 t + h (disjoint bits, as in
+// k_ez_part), so parity(j & z) = parity(h & z) ^ parity(2 t & z) ^
+// parity((chunk_base + 512 k) & z): the h part picks p0 + p1 or p0 - p1 per
+// term, the uniform part is a sign per trip, the thread part one sign at the
+// end -- one multiply-add per term per float4.
+__global__ __launch_bounds__(EV_THREADS) void k_ediag_part(const float4* __restrict__ st, int n, int lg_chunk,
                                                            ZMasks zm, float* __restrict__ part) {
   __shared__ float red[EV_THREADS / 64];
   const int nchunks = 1 << (n - lg_chunk);
   const uint64_t row = blockIdx.x >> (n - lg_chunk);
   const uint64_t chunk = blockIdx.x & (nchunks - 1);
   const uint64_t first = (row << n) + (chunk << lg_chunk);
-  const uint64_t cnt = 1ull << lg_chunk;
+  const uint64_t npairs = (1ull << lg_chunk) >> 1;
   float acc[EV_MAXT];
 #pragma unroll
   for (int t = 0; t < EV_MAXT; ++t) acc[t] = 0.f;
-  for (uint64_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-    const float2 v = st[first + i];
-    const float p = v.x * v.x + v.y * v.y;
-    const uint32_t j = (uint32_t)((chunk << lg_chunk) + i);
+  for (uint64_t b = threadIdx.x, k = 0; b < npairs; b += EV_UNROLL * EV_THREADS, k += EV_UNROLL) {
+    float4 v[EV_UNROLL];
 #pragma unroll
-    for (int t = 0; t < EV_MAXT; ++t)
-      if (t < zm.nt) acc[t] += (__popc(j & zm.z[t]) & 1) ? -p : p;
+    for (int u = 0; u < EV_UNROLL; ++u) {
+      const uint64_t i = b + (uint64_t)u * EV_THREADS;
+      v[u] = i < npairs ? st[(first >> 1) + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < EV_UNROLL; ++u) {
+      const float p0 = v[u].x * v[u].x + v[u].y * v[u].y;
+      const float p1 = v[u].z * v[u].z + v[u].w * v[u].w;
+      const float ps = p0 + p1, pd = p0 - p1;
+      const uint32_t jhi = (uint32_t)((chunk << lg_chunk) + 512ull * (k + u));   // wave-uniform
+#pragma unroll
+      for (int t = 0; t < EV_MAXT; ++t) {
+        if (t < zm.nt) {
+          const float sg = (__popc(jhi & zm.z[t]) & 1) ? -1.f : 1.f;
+          acc[t] = fmaf(sg, (zm.z[t] & 1u) ? pd : ps, acc[t]);
+        }
+      }
+    }
   }
   float* out = part + (uint64_t)blockIdx.x * zm.nt;
 #pragma unroll
   for (int t = 0; t < EV_MAXT; ++t) {
     if (t < zm.nt) {
-      const float s = block_sum(acc[t], red);
+      const float tsg = (__popc((2u * threadIdx.x) & zm.z[t]) & 1) ? -1.f : 1.f;
+      const float s = block_sum(tsg * acc[t], red);
       if (threadIdx.x == 0) out[t] = s;
     }
   }
@@ -187,18 +244,22 @@ __global__ __launch_bounds__(EV_THREADS) void k_ex_part(const float2* __restrict
 // Sum per-chunk partials of one term: partial of (row, chunk) sits at
 // part[(row * nchunks + chunk) * cstride] (complex: two floats re, im).
 // out[row * ostride + ooff] = scale_re * Re(sum) - scale_im * Im(sum).
-__global__ void k_sum_chunks(const float* __restrict__ part, int nchunks, int cstride, int is_complex,
-                             int64_t bsz, float scale_re, float scale_im, float* __restrict__ out,
-                             int64_t ostride, int ooff) {
-  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// One wavefront per row, its lanes striding over the chunks.
+__global__ __launch_bounds__(256) void k_sum_chunks(const float* __restrict__ part, int nchunks, int cstride,
+                                                    int is_complex, int64_t bsz, float scale_re, float scale_im,
+                                                    float* __restrict__ out, int64_t ostride, int ooff) {
+  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;   // wave-uniform
+  const int lane = threadIdx.x & 63;
   if (row >= bsz) return;
   double sr = 0.0, si = 0.0;
-  for (int c = 0; c < nchunks; ++c) {
+  for (int c = lane; c < nchunks; c += 64) {
     const float* p = part + (row * nchunks + c) * cstride;
     sr += p[0];
     if (is_complex) si += p[1];
   }
-  out[row * ostride + ooff] = (float)(scale_re * sr - scale_im * si);
+  sr = wave_sum_d(sr);
+  si = wave_sum_d(si);
+  if (lane == 0) out[row * ostride + ooff] = (float)(scale_re * sr - scale_im * si);
 }
 
 // out_j (+)= sum_t coef[row, t] * (-1)^{popcount((j^x_t) & z_t)} * psi_{j ^ x_t}
@@ -529,6 +590,7 @@ TQSV_API int tqsv_expval_z_all(hipStream_t s, const void* state, int64_t bsz, in
                                int64_t ws_bytes) {
   if (!state || !out || !workspace) { set_error("tqsv_expval_z_all: null pointer"); return E_ARG; }
   if (bsz < 1 || n < 1 || n > EV_MAXQ) { set_error("tqsv_expval_z_all: need 1 <= n <= %d", EV_MAXQ); return E_ARG; }
+  if ((uintptr_t)state & 15) { set_error("tqsv_expval_z_all: state must be 16-byte aligned"); return E_ARG; }
   const int lg = ev_chunk_log2(bsz, n);
   const int64_t blocks = bsz << (n - lg);
   if (ws_bytes < blocks * (n + 1) * (int64_t)sizeof(float)) { set_error("tqsv_expval_z_all: workspace too small"); return E_ARG; }
@@ -536,7 +598,7 @@ TQSV_API int tqsv_expval_z_all(hipStream_t s, const void* state, int64_t bsz, in
   int rc = check_launch("tqsv_expval_z_all/part");
   if (rc) return rc;
   const int64_t outs = bsz * n;
-  k_ez_final<<<(unsigned)((outs + 255) / 256), 256, 0, s>>>((const float*)workspace, n, 1 << (n - lg), bsz, out);
+  k_ez_final<<<sum_grid(outs), 256, 0, s>>>((const float*)workspace, n, 1 << (n - lg), bsz, out);
   return check_launch("tqsv_expval_z_all/final");
 }
 
@@ -550,6 +612,7 @@ TQSV_API int tqsv_expval_pauli(hipStream_t s, const void* state, int64_t bsz, in
     return E_ARG;
   }
   if (bsz < 1 || n < 1 || n > 32 || nterms < 0) { set_error("tqsv_expval_pauli: need 1 <= n <= 32"); return E_ARG; }
+  if ((uintptr_t)state & 15) { set_error("tqsv_expval_pauli: state must be 16-byte aligned"); return E_ARG; }
   const int lg = ev_chunk_log2(bsz, n);
   const int64_t nchunks = 1ll << (n - lg);
   const int64_t blocks = bsz * nchunks;
@@ -563,13 +626,13 @@ TQSV_API int tqsv_expval_pauli(hipStream_t s, const void* state, int64_t bsz, in
       ZMasks zm;
       zm.nt = cnt;
       for (int q = 0; q < cnt; ++q) zm.z[q] = (uint32_t)zmask[idx[q]];
-      k_ediag_part<<<(unsigned)blocks, EV_THREADS, 0, s>>>((const float2*)state, n, lg, zm, (float*)workspace);
+      k_ediag_part<<<(unsigned)blocks, EV_THREADS, 0, s>>>((const float4*)state, n, lg, zm, (float*)workspace);
       int rc = check_launch("tqsv_expval_pauli/diag");
       if (rc) return rc;
       // contiguous output indices? scatter one term at a time
       for (int q = 0; q < cnt; ++q) {
         // sum chunk partials of term q: treat part as [row, chunk, cnt]
-        k_sum_chunks<<<(unsigned)((bsz + 255) / 256), 256, 0, s>>>(
+        k_sum_chunks<<<sum_grid(bsz), 256, 0, s>>>(
             (const float*)workspace + q, (int)nchunks, cnt, 0, bsz, 1.f, 0.f, out, out_stride, idx[q]);
         if ((rc = check_launch("tqsv_expval_pauli/sum"))) return rc;
       }
@@ -601,7 +664,7 @@ TQSV_API int tqsv_expval_pauli(hipStream_t s, const void* state, int64_t bsz, in
         // partial = Re S (nY even) or Im S (nY odd); <P> = Re(i^{nY} S)
         const int ny = __builtin_popcount((uint32_t)(xmask[t] & zmask[t]));
         const float sc = (ny & 3) == 0 || (ny & 3) == 3 ? 1.f : -1.f;
-        k_sum_chunks<<<(unsigned)((bsz + 255) / 256), 256, 0, s>>>((const float*)workspace + 2 * q, (int)wpr,
+        k_sum_chunks<<<sum_grid(bsz), 256, 0, s>>>((const float*)workspace + 2 * q, (int)wpr,
                                                                    2 * g.nt, 0, bsz, sc, 0.f, out, out_stride, t);
         if ((rc = check_launch("tqsv_expval_pauli/groupsum"))) return rc;
         done[t] = 1;
@@ -625,7 +688,7 @@ TQSV_API int tqsv_expval_pauli(hipStream_t s, const void* state, int64_t bsz, in
     int rc = check_launch("tqsv_expval_pauli/x");
     if (rc) return rc;
     // Re(i^ny * S) = sre*Re(S) - sim*Im(S)
-    k_sum_chunks<<<(unsigned)((bsz + 255) / 256), 256, 0, s>>>((const float*)workspace, (int)nch, 2, 1, bsz, sre,
+    k_sum_chunks<<<sum_grid(bsz), 256, 0, s>>>((const float*)workspace, (int)nch, 2, 1, bsz, sre,
                                                                sim, out, out_stride, t);
     if ((rc = check_launch("tqsv_expval_pauli/xsum"))) return rc;
   }

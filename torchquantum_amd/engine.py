@@ -104,7 +104,7 @@ def wait_for_compiled_stages(timeout: Optional[float] = None) -> bool:
 # ----------------------------------------------------------------------------
 # tape
 # ----------------------------------------------------------------------------
-@dataclass
+@dataclass(slots=True)
 class TapeOp:
     name: str                       # canonical gate name
     wires: Tuple[int, ...]
@@ -599,7 +599,7 @@ def _host_params(tape: Sequence[TapeOp]) -> List[Optional[np.ndarray]]:
         p = op.params
         if p is None:
             continue
-        if p.device.type == "cpu":
+        if not p.is_cuda:
             if p.is_complex():
                 out[i] = p.detach().numpy().astype(np.complex128)
             else:
@@ -1021,6 +1021,10 @@ class Compiled:
             self.param_offsets.append(off)
             off += op.n_params()
         self.n_slots = off
+        # tape ops with real parameters, and whether any of them is trainable
+        # (both fixed by the structure key, which holds shapes and requires_grad)
+        self.param_idx = [i for i, op in enumerate(tape) if op.params is not None and not op.is_matrix_param]
+        self.any_param_grad = any(tape[i].params.requires_grad for i in self.param_idx)
         kops = lower(tape, n, bsz, self.param_offsets)
         if opts.mode == "fused" and opts.fusion_width >= 2:
             kops = fuse_blocks(kops, tape, n, opts.fusion_width)
@@ -1317,14 +1321,16 @@ _plan_cache: Dict[tuple, Compiled] = {}
 
 def structure_key(tape: Sequence[TapeOp], n: int, bsz: int, opts: EngineOptions, device) -> tuple:
     items = []
+    add = items.append
     for op in tape:
-        if op.params is None:
+        p = op.params
+        if p is None:
             ps = None
-        elif op.is_matrix_param:
-            ps = ("m", tuple(op.params.shape))
+        elif p.is_complex():
+            ps = ("m", p.shape)                 # torch.Size: a tuple (hash and equality)
         else:
-            ps = (tuple(op.params.shape), bool(op.params.requires_grad))
-        items.append((op.name, op.wires, op.inverse, op.conj, ps))
+            ps = (p.shape, p.requires_grad)
+        add((op.name, op.wires, op.inverse, op.conj, ps))
     return (n, bsz, opts.key(), str(device), tuple(items))
 
 

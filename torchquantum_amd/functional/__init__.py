@@ -129,15 +129,24 @@ def _make_gate(name: str):
         # fast path, same checks as _record: a fixed-width gate on int wires
         # with no parameter or a real [rows, P] tensor, no op history
         if plain and comp_method == "bmm" and not getattr(q_device, "record_op", False):
-            if type(wires) is int:
+            tw = type(wires)
+            ws = None
+            if tw is int:
                 ws = (wires,)
-            elif type(wires) in (list, tuple) and all(type(w) is int for w in wires):
+            elif tw is list or tw is tuple:
                 ws = tuple(wires)
-            else:
-                ws = None
+                for w in ws:
+                    if type(w) is not int:
+                        ws = None
+                        break
             if ws is not None and len(ws) == nw and (nw == 1 or len(set(ws)) == nw):
                 n = q_device.n_wires
-                if all(0 <= w < n for w in ws):
+                ok = True
+                for w in ws:
+                    if w < 0 or w >= n:
+                        ok = False
+                        break
+                if ok:
                     if npar == 0:
                         q_device._push(TapeOp(cname, ws, None, bool(inverse), 0))
                         return
