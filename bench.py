@@ -224,7 +224,8 @@ def time_batch(n_circ, steps, tq):
             "levels": len(cb.levels), "launches_per_batch": cb.n_launches,
             "ms_graph_replay": t_graph * 1e3, "ms_eager_batch": t_eager * 1e3, "ms_per_circuit_loop": t_loop * 1e3,
             "amp_updates_per_s_graph": upd / t_graph, "speedup_vs_per_circuit_loop": t_loop / t_graph,
-            "build_s": t_build, "z0_max_abs_diff_vs_per_circuit": float((out - ref).abs().max())}
+            "build_s": t_build, "z0_max_abs_diff_vs_per_circuit": float((out - ref).abs().max())}, \
+        (out.reshape(-1).cpu().numpy(), circs)
 
 
 def copy_calibration(nbytes=1 << 31):
@@ -334,6 +335,104 @@ def cpu_baseline(ops_sample, n, bsz, n_gates_total_per_step):
     return upd / dt, threads, dt
 
 
+def oracle_errors(tq, batch_out=None):
+    """Max-abs error of the GPU results against the complex128 oracle, per
+    workload, at sizes the oracle finishes in seconds (part of the CPU leg:
+    the oracle is the checker here, never the thing timed).  Configs 1 and 2
+    run at full size (their rows share parameters and start in |0..0>, so one
+    oracle row stands for all); the gradient ansatz and the 30-qubit circuit
+    run through the same builders at fewer qubits; the uneven batch is checked
+    on its circuits of at most 12 qubits."""
+    from oracle import statevector as O
+    import torchquantum_amd.functional as tqf
+    from circuits import config1_ops, config2_ops, config3_ops, config3_observables, config5_ops
+    res = {}
+
+    def run_gpu(n, bsz, ops, opts):
+        q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
+        q.engine_options = opts
+        build_config2(q, ops, tqf)
+        z = tq.MeasureAll(tq.PauliZ)(q)
+        return q.get_states_1d().cpu().numpy(), z.cpu().numpy()
+
+    forward_cases = (("config1: 10 q, bsz 32 (full size)", 10, 32, config1_ops(seed=0)),
+                     ("config2: 20 q, bsz 16 (full size)", 20, 16, config2_ops(seed=0, n=20, layers=20)),
+                     ("config5 circuit at 16 q, bsz 4", 16, 4, config5_ops(seed=0, n=16, layers=20)))
+    for name, n, bsz, ops in forward_cases:
+        log(f"[bench] oracle check {name}")
+        ref = O.run_ops(ops, n, 1)
+        zref = O.expval_z_all(ref, n)
+        e = {}
+        for label, opts in (("pergate", tq.EngineOptions(mode="pergate")),
+                            ("fused", tq.EngineOptions(mode="fused", jit="on")),
+                            ("fused_interpreting", tq.EngineOptions(mode="fused", jit="off"))):
+            st, z = run_gpu(n, bsz, ops, opts)
+            e[label] = {"amp_max_abs": float(np.abs(st - ref).max()), "expval_max_abs": float(np.abs(z - zref).max())}
+        res[name] = e
+
+    # gradient ansatz (10 x [RX,RY,RZ all wires, CNOT ring], per-row RY input
+    # layer, loss = mean_b sum_t c_t <P_t>) at 10 qubits bsz 4: loss and a
+    # seeded 32-parameter subset of the gradient against central differences
+    # of the oracle (h = 1e-3)
+    n, bsz = 10, 4
+    log(f"[bench] oracle check gradient ansatz at {n} q")
+    ops = config3_ops(seed=0, n=n, layers=10)
+    obs, coef = config3_observables(n)
+    phi = np.random.default_rng(1).random(bsz) * np.pi
+    theta0 = np.array([float(np.asarray(op["params"]).reshape(-1)[0]) for op in ops if op["params"] is not None])
+
+    def oracle_loss(theta):
+        seq = [dict(name="ry", wires=[w], params=phi.reshape(bsz, 1), inverse=False) for w in range(n)]
+        k = 0
+        for op in ops:
+            if op["params"] is None:
+                seq.append(op)
+            else:
+                seq.append(dict(op, params=np.array([[theta[k]]])))
+                k += 1
+        st = O.run_ops(seq, n, bsz)
+        return float(np.mean(sum(c * O.expval_pauli(st, o) for o, c in zip(obs, coef))))
+
+    idx = np.random.default_rng(2).choice(len(theta0), 32, replace=False)
+    g_ref = O.central_fd(oracle_loss, theta0, 1e-3, idx)[idx]
+    loss_ref = oracle_loss(theta0)
+    e = {}
+    for label, opts in (("pergate", tq.EngineOptions(mode="pergate")),
+                        ("fused", tq.EngineOptions(mode="fused", jit="on")),
+                        ("fused_interpreting", tq.EngineOptions(mode="fused", jit="off"))):
+        params = [torch.tensor([[t]], dtype=torch.float32, device="cuda", requires_grad=True) for t in theta0]
+        q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
+        q.engine_options = opts
+        ph = torch.tensor(phi, dtype=torch.float32, device="cuda").reshape(bsz, 1)
+        for w in range(n):
+            tqf.ry(q, w, params=ph)
+        k = 0
+        for op in ops:
+            if op["params"] is None:
+                tqf.func_name_dict[op["name"]](q, op["wires"])
+            else:
+                tqf.func_name_dict[op["name"]](q, op["wires"], params=params[k])
+                k += 1
+        loss = tq.expval_pauli_sum(q, obs, coef).mean()
+        loss.backward()
+        g = torch.cat([p.grad.reshape(-1) for p in params]).cpu().numpy().astype(np.float64)
+        e[label] = {"loss_abs": abs(float(loss.detach()) - loss_ref), "grad_max_abs_vs_fd": float(np.abs(g[idx] - g_ref).max())}
+    res[f"config3 ansatz at {n} q, bsz {bsz} (32-param FD subset)"] = e
+
+    if batch_out is not None:
+        z0, circs = batch_out
+        log("[bench] oracle check uneven batch (circuits of <= 12 qubits)")
+        err, cnt = 0.0, 0
+        for c, (nq, cops) in enumerate(circs):
+            if nq > 12:
+                continue
+            ref = O.expval_z_all(O.run_ops(cops, nq, 1), nq)[0, 0]
+            err = max(err, abs(float(z0[c]) - float(ref)))
+            cnt += 1
+        res["config4 batch"] = {"circuits_checked": cnt, "z0_max_abs": err}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -439,9 +538,10 @@ def main():
             "loss": loss_f,
         }
 
+    batch_out = None
     if not args.skip_batch:
         log("[bench] uneven batch (256 circuits)")
-        extra["batch"] = time_batch(256, 5, tq)
+        extra["batch"], batch_out = time_batch(256, 5, tq)
         log(f"[bench] batch graph {extra['batch']['ms_graph_replay']:.2f} ms")
 
     # the reference's own CPU-runnable case: 10 qubits, bsz 32, 8 x [U3 all, CNOT ring], <Z_i>
@@ -467,6 +567,7 @@ def main():
         cpu["small_circuit"] = {"ms_per_circuit_batch": cdt1 * 1e3, "amp_updates_per_s": rate1,
                                 "sample": "the whole 10-qubit bsz-32 circuit (160 gates), same CPU path"}
         extra["small_circuit"]["fused_speedup_vs_cpu"] = cdt1 / t1_f
+        extra["oracle_error"] = oracle_errors(tq, batch_out)
 
     line = {
         "metric": "amplitude updates/s & adjoint grads/s vs plain per-gate kernel, % HBM peak",
