@@ -56,7 +56,9 @@ class EngineOptions:
 
     mode:        "fused" (default) or "pergate".
     stage_bits:  m, local bits per stage (R_BITS + log2 threads per group); 0 = auto.
-    low_bits:    lowest bit positions always kept local (coalescing).
+    low_bits:    lowest bit positions always kept local (coalescing).  With 4,
+                 a wavefront's loads cover whole 128-byte lines; fewer leave
+                 64-byte runs, more constrain the partition into more stages.
     merge_1q:    multiply runs of single-wire gates into one 2x2 (fused mode).
     fusion_width: k; >= 2 pre-multiplies runs of gates acting within k wires
                  into one dense 2^k x 2^k block (k <= 6; trainable blocks k <= 2).
@@ -76,7 +78,7 @@ class EngineOptions:
     """
     mode: str = "fused"
     stage_bits: int = 0
-    low_bits: int = 3
+    low_bits: int = 4
     merge_1q: bool = True
     fusion_width: int = 1
     grid_cap: int = 0
