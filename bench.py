@@ -215,6 +215,9 @@ def time_batch(n_circ, steps, tq):
         return torch.stack(res)
     ref = per_circuit()
     torch.cuda.synchronize()
+    tq.wait_for_compiled_stages()   # large circuits' compiled stages: ready, and no compile left running
+    per_circuit()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     ref = per_circuit()
     torch.cuda.synchronize()
@@ -510,13 +513,25 @@ def main():
     log(f"[bench] fused, program-interpreting stage kernels {dt_fi * 1e3:.2f} ms/step")
     parity = float((ev_pg - ev_f).abs().max())
 
+    # the reference's own CPU-runnable case: 10 qubits, bsz 32, 8 x [U3 all, CNOT ring], <Z_i>
+    # (host-bound: timed before the large-state sections)
+    extra = {}
+    from circuits import config1_ops
+    ops1 = config1_ops(seed=0)
+    upd1 = len(ops1) * 32 * (1 << 10)
+    log("[bench] 10-qubit bsz-32 circuit (160 gates)")
+    t1_pg, _ = time_forward(10, 32, ops1, "pergate", 20, 3, tq)
+    t1_f, _ = time_forward(10, 32, ops1, "fused", 20, 3, tq)
+    extra["small_circuit"] = {"workload": "10 qubits, bsz 32, 8 x [U3 all wires, CNOT ring], <Z_i> (160 gates)",
+                              "pergate_ms_per_circuit_batch": t1_pg * 1e3, "fused_ms_per_circuit_batch": t1_f * 1e3,
+                              "fused_amp_updates_per_s": upd1 / t1_f}
+
     log("[bench] kernel roofline")
     ms, nbytes = kernel_roofline(n, bsz, iters=60)
     achieved = nbytes / (ms * 1e-3) / 1e9
     ms_big, nbytes_big = kernel_roofline(26, 16, iters=26)
     achieved_big = nbytes_big / (ms_big * 1e-3) / 1e9
 
-    extra = {}
     log("[bench] dense k = 5, 6 blocks")
     extra["dense_block"] = dense_block_bench()
     if not args.skip_adjoint:
@@ -544,16 +559,6 @@ def main():
         extra["batch"], batch_out = time_batch(256, 5, tq)
         log(f"[bench] batch graph {extra['batch']['ms_graph_replay']:.2f} ms")
 
-    # the reference's own CPU-runnable case: 10 qubits, bsz 32, 8 x [U3 all, CNOT ring], <Z_i>
-    from circuits import config1_ops
-    ops1 = config1_ops(seed=0)
-    upd1 = len(ops1) * 32 * (1 << 10)
-    log("[bench] 10-qubit bsz-32 circuit (160 gates)")
-    t1_pg, _ = time_forward(10, 32, ops1, "pergate", 20, 3, tq)
-    t1_f, _ = time_forward(10, 32, ops1, "fused", 20, 3, tq)
-    extra["small_circuit"] = {"workload": "10 qubits, bsz 32, 8 x [U3 all wires, CNOT ring], <Z_i> (160 gates)",
-                              "pergate_ms_per_circuit_batch": t1_pg * 1e3, "fused_ms_per_circuit_batch": t1_f * 1e3,
-                              "fused_amp_updates_per_s": upd1 / t1_f}
 
     cpu = None
     if not args.skip_cpu:
