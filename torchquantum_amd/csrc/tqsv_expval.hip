@@ -376,36 +376,41 @@ __global__ __launch_bounds__(PG_THREADS) void k_pauli_group_apply(float2* __rest
     for (int t = 0; t < g.nt; ++t) {
       float2 c = cf[g.col[t]];
       if (__popc(goff & g.zg[t]) & 1) c = make_float2(-c.x, -c.y);
+      // source amplitude k = l ^ x of local index l = tid | r << 8 splits
+      // into disjoint fields: k = (tid ^ x_lo) | (r ^ x_hi) << 8, so its
+      // phase sign is a per-thread part (folded into the coefficient once
+      // per term) times a wave-uniform part per r
       const uint32_t xl = g.xl[t], zl = g.zl[t];
-      // source amplitude k = l ^ x with its phase sign folded into the coefficient
-#define TQSV_PG_SRC(r)                                                   \
-  const uint32_t k = ((uint32_t)tid + (uint32_t)(r) * PG_THREADS) ^ xl; \
-  const float2 v = buf[k];                                              \
-  const float sg = (__popc(k & zl) & 1) ? -1.f : 1.f;
+      const uint32_t tk = (uint32_t)tid ^ (xl & (PG_THREADS - 1));
+      const uint32_t xh = xl >> 8, zh = zl >> 8;
+      if (__popc(tk & zl & (PG_THREADS - 1)) & 1) c = make_float2(-c.x, -c.y);
       if (c.y == 0.f) {          // real coefficient (wave-uniform branch)
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-          TQSV_PG_SRC(r)
-          const float cr = sg * c.x;
+          const uint32_t rr = (uint32_t)r ^ xh;                   // wave-uniform
+          const float2 v = buf[tk + rr * PG_THREADS];
+          const float cr = (__popc(rr & zh) & 1) ? -c.x : c.x;
           acc[r].x = fmaf(cr, v.x, acc[r].x);
           acc[r].y = fmaf(cr, v.y, acc[r].y);
         }
       } else if (c.x == 0.f) {   // imaginary coefficient
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-          TQSV_PG_SRC(r)
-          const float ci = sg * c.y;
+          const uint32_t rr = (uint32_t)r ^ xh;
+          const float2 v = buf[tk + rr * PG_THREADS];
+          const float ci = (__popc(rr & zh) & 1) ? -c.y : c.y;
           acc[r].x = fmaf(-ci, v.y, acc[r].x);
           acc[r].y = fmaf(ci, v.x, acc[r].y);
         }
       } else {
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-          TQSV_PG_SRC(r)
-          acc[r] = cfma(make_float2(sg * c.x, sg * c.y), v, acc[r]);
+          const uint32_t rr = (uint32_t)r ^ xh;
+          const float2 v = buf[tk + rr * PG_THREADS];
+          const float2 cc = (__popc(rr & zh) & 1) ? make_float2(-c.x, -c.y) : c;
+          acc[r] = cfma(cc, v, acc[r]);
         }
       }
-#undef TQSV_PG_SRC
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) rout[tsc | rsc[r]] = acc[r];
@@ -447,15 +452,20 @@ __global__ __launch_bounds__(PG_THREADS) void k_pauli_group_expval(const float2*
     for (int t = 0; t < g.nt; ++t) {
       const uint32_t xl = g.xl[t], zl = g.zl[t];
       const bool odd = __popc(xl & zl) & 1;                  // nY odd: Im S
+      // l = tid | r << 8 and l ^ x = (tid ^ x_lo) | (r ^ x_hi) << 8 (disjoint
+      // fields): the sign of l splits into a per-thread part, applied once
+      // per term, and a wave-uniform part per r
+      const uint32_t tk = (uint32_t)tid ^ (xl & (PG_THREADS - 1));
+      const uint32_t xh = xl >> 8, zh = zl >> 8;
       float s = 0.f;
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
-        const uint32_t l = (uint32_t)tid + (uint32_t)r * PG_THREADS;
-        const float2 b = buf[l ^ xl];
+        const float2 b = buf[tk + ((uint32_t)r ^ xh) * PG_THREADS];
         const float d = odd ? fmaf(b.x, a[r].y, -b.y * a[r].x) : fmaf(b.x, a[r].x, b.y * a[r].y);
-        s += (__popc(l & zl) & 1) ? -d : d;
+        s = fmaf((__popc((uint32_t)r & zh) & 1) ? -1.f : 1.f, d, s);
       }
-      lacc[t][tid] += (__popc(goff & g.zg[t]) & 1) ? -s : s;
+      const bool neg = ((__popc(goff & g.zg[t]) ^ __popc((uint32_t)tid & zl & (PG_THREADS - 1))) & 1) != 0;
+      lacc[t][tid] += neg ? -s : s;
     }
   }
   float* o = part + (uint64_t)blockIdx.x * 2 * g.nt;
