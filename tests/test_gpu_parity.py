@@ -47,6 +47,30 @@ def test_each_gate(name, mode):
 
 
 @gpu
+@pytest.mark.parametrize("name", ["cz", "cu1", "crz", "ccz", "cs"])
+def test_low_bit_controlled_diagonals(name):
+    """Controlled diagonals whose controls include amplitude bit 1 or 0 (the
+    whole-line diagonal path) and bit pairs above it, shared and per-row
+    parameters, per-gate kernels, from a random state; 1e-5."""
+    from torchquantum_amd import gates as G
+    rng = np.random.default_rng(7)
+    n, bsz = 6, 3
+    k = G.spec(name).num_wires
+    init = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+    init /= np.linalg.norm(init, axis=1, keepdims=True)
+    ops = []
+    for top in range(n - k + 1):
+        for rows in (1, bsz):
+            wires = list(range(top, top + k))[::-1] if rows == 1 else list(range(top, top + k))
+            npar = G.spec(name).num_params
+            p = rng.uniform(-np.pi, np.pi, size=(rows, npar)) if npar else None
+            ops.append(dict(name=name, wires=wires, params=p, inverse=bool(rows > 1)))
+    got = _run(n, bsz, ops, "pergate", init)
+    ref = O.run_ops(oracle_ops(ops, bsz), n, bsz, state=init)
+    assert np.abs(got - ref).max() < 1e-5, name
+
+
+@gpu
 @pytest.mark.parametrize("mode,stage_bits", [("pergate", 0), ("fused", 0), ("fused", 4), ("fused", 6),
                                              ("fused", 9), ("fused", 13)])
 @pytest.mark.parametrize("n", [4, 7, 11, 14])

@@ -317,6 +317,29 @@ __global__ __launch_bounds__(256) void k_diag4(float4* __restrict__ st, uint64_t
   }
 }
 
+// Every float4 of the state (two adjacent amplitudes), each amplitude
+// changed only where (index & cmask) == cval.  For controls that include
+// float4 bit 0 (amplitude bit 1) the changed float4s are isolated 16-byte
+// pieces: moving whole lines in and out costs less than the partial writes
+// of k_diag4 (CZ on amplitude bits (1, 0) or (2, 1)).
+template <bool BATCHED, bool NT>
+__global__ __launch_bounds__(256) void k_diag4_lines(float4* __restrict__ st, uint64_t n_items, uint64_t cmask,
+                                                    uint64_t cval, uint64_t pmask, const float2* __restrict__ ph,
+                                                    int64_t bstride, int lg_items_row, int reps) {
+  uint64_t it = tile_first(reps);
+  for (int r = 0; r < reps; ++r, it += 256) {
+    if (it >= n_items) return;
+    const float2* p = BATCHED ? ph + (it >> lg_items_row) * bstride : ph;
+    const float2 f0 = p[0], f1 = p[1];
+    const uint64_t i = 2 * it;   // row bits sit above bit n, outside cmask and pmask
+    const float4 v = ld4<NT>(st + it);
+    float2 a = make_float2(v.x, v.y), b = make_float2(v.z, v.w);
+    if ((i & cmask) == cval) a = cmul((__popcll(i & pmask) & 1) ? f1 : f0, a);
+    if (((i | 1ull) & cmask) == cval) b = cmul((__popcll((i | 1ull) & pmask) & 1) ? f1 : f0, b);
+    st4<NT>(st + it, make_float4(a.x, a.y, b.x, b.y));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // swap of bits a, b (optionally controlled): exchanges |..0..1..> <-> |..1..0..>
 // ---------------------------------------------------------------------------
@@ -694,6 +717,15 @@ TQSV_API int tqsv_apply_diag(hipStream_t s, void* state, int64_t bsz, int n, uin
     // float4 items; a control on bit 0 is applied inside the item
     const uint64_t cm4 = (cmask & ~1ull) >> 1;
     const int c0 = (cmask & 1ull) ? (int)(cval & 1ull) : -1;
+    if (cm4 & 1ull) {
+      // the touched float4s are isolated 16-byte pieces: whole-line pass
+      const int lg = n - 1;
+      const uint64_t n_items = (uint64_t)bsz << lg;
+      const Tiles t = tiles_for(n_items);
+      LAUNCH_BN(k_diag4_lines, b, stream_nt(bsz, n), t, s, (float4*)state, n_items, cmask, cval, pmask, p,
+                ph_bstride, lg, t.reps);
+      return check_launch("tqsv_apply_diag");
+    }
     if (!build_insert(ins, cm4, n - 1)) { set_error("too many control bits (max 12)"); return E_LIMIT; }
     const int lg = n - 1 - __builtin_popcountll(cm4);
     const uint64_t n_items = (uint64_t)bsz << lg;
