@@ -569,6 +569,12 @@ class _PinnedRing:
             if buf is None or buf.numel() < n:
                 cap = max(n, 2 * (buf.numel() if buf is not None else 0), 4096)
                 buf = torch.empty(cap, dtype=torch.complex64, pin_memory=True)
+                # the first use fills every empty slot too, so that later
+                # flushes make no pinned allocation (slots that are still
+                # empty have no copy in flight)
+                for j in range(self.SLOTS):
+                    if j != i and self.slots[j] is None:
+                        self.slots[j] = (torch.empty(cap, dtype=torch.complex64, pin_memory=True), None)
             buf[:n].numpy()[:] = flat
             out = torch.empty(n, dtype=torch.complex64, device=device)
             out.copy_(buf[:n], non_blocking=True)
