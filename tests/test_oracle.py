@@ -173,6 +173,63 @@ def test_decompositions():
     assert np.allclose(dg, np.diag(np.diag(dg))) and np.allclose(np.abs(np.diag(dg)), 1)
 
 
+def _circuit_matrix(seq, n):
+    """Product of a gate sequence [(name, wires, params)], first gate applied first,
+    each embedded by the brute-force Kronecker builder."""
+    m = np.eye(2 ** n, dtype=complex)
+    for name, wires, p in seq:
+        m = O.full_unitary(O.gate_matrix(name, p), wires, n) @ m
+    return m
+
+
+def test_relative_phase_toffolis_exact():
+    """RCCX and RC3X with their exact phases, from the published Clifford+T
+    circuits (Margolus' 3-qubit gate; the 4-qubit relative-phase Toffoli of
+    Maslov, Phys. Rev. A 93, 022311 (2016), as written in Qiskit's RCCXGate /
+    RC3XGate definitions), target = last wire."""
+    pi = math.pi
+    t, td = ("u1", [pi / 4]), ("u1", [-pi / 4])
+    seq3 = [("hadamard", [2], None), (t[0], [2], t[1]), ("cnot", [1, 2], None), (td[0], [2], td[1]),
+            ("cnot", [0, 2], None), (t[0], [2], t[1]), ("cnot", [1, 2], None), (td[0], [2], td[1]),
+            ("hadamard", [2], None)]
+    assert np.allclose(_circuit_matrix(seq3, 3), O.gate_matrix("rccx"), atol=1e-12)
+    seq4 = [("hadamard", [3], None), (t[0], [3], t[1]), ("cnot", [2, 3], None), (td[0], [3], td[1]),
+            ("hadamard", [3], None), ("cnot", [0, 3], None), (t[0], [3], t[1]), ("cnot", [1, 3], None),
+            (td[0], [3], td[1]), ("cnot", [0, 3], None), (t[0], [3], t[1]), ("cnot", [1, 3], None),
+            (td[0], [3], td[1]), ("hadamard", [3], None), (t[0], [3], t[1]), ("cnot", [2, 3], None),
+            (td[0], [3], td[1]), ("hadamard", [3], None)]
+    assert np.allclose(_circuit_matrix(seq4, 4), O.gate_matrix("rc3x"), atol=1e-12)
+    # a phase flipped on one entry must fail
+    bad = O.gate_matrix("rccx").copy()
+    bad[6, 7] *= -1
+    assert not np.allclose(_circuit_matrix(seq3, 3), bad)
+
+
+def test_shadamard_is_ry_quarter_pi():
+    """SHadamard = RY(pi/4) = exp(-i pi/8 Y) (hadamard.py:16-22)."""
+    sh = O.gate_matrix("shadamard")
+    assert np.allclose(sh, expm(-0.5j * (math.pi / 4) * Y))
+    assert np.allclose(sh, O.gate_matrix("ry", [math.pi / 4]))
+    # and it is the basis change that maps Z to H: sh^dagger Z sh = H (up to sign convention)
+    assert np.allclose(sh @ Z @ sh.conj().T, O.gate_matrix("hadamard"))
+
+
+@pytest.mark.parametrize("t,beta", [(0.37, 0.9), (-1.9, -2.4), (3.1, 1.3), (0.8, math.pi / 2)])
+def test_xx_pm_yy_with_phase(t, beta):
+    """XX+YY(theta, beta) = exp(-i theta/4 D (XX+YY) D^dagger) with D = diag(1, e^{-i beta})
+    on wire 0, and XX-YY(theta, beta) = exp(-i theta/4 D' (XX-YY) D'^dagger) with
+    D' = diag(1, e^{i beta}) on wire 0: the beta-rotated generators (xx_plus_yy.py,
+    xx_min_yy.py).  A sign flip of beta must fail."""
+    kr = np.kron
+    gp, gm = kr(X, X) + kr(Y, Y), kr(X, X) - kr(Y, Y)
+    d = kr(np.diag([1, np.exp(-1j * beta)]), I2)
+    dm = kr(np.diag([1, np.exp(1j * beta)]), I2)
+    assert np.allclose(O.gate_matrix("xxplusyy", [t, beta]), expm(-0.25j * t * (d @ gp @ d.conj().T)))
+    assert np.allclose(O.gate_matrix("xxminyy", [t, beta]), expm(-0.25j * t * (dm @ gm @ dm.conj().T)))
+    assert not np.allclose(O.gate_matrix("xxplusyy", [t, -beta]), expm(-0.25j * t * (d @ gp @ d.conj().T)))
+    assert not np.allclose(O.gate_matrix("xxminyy", [t, -beta]), expm(-0.25j * t * (dm @ gm @ dm.conj().T)))
+
+
 def test_qft_is_inverse_dft():
     for k in (1, 2, 3, 4):
         d = 2 ** k
@@ -266,3 +323,51 @@ def test_parameter_shift_identity():
     ps = np.array([(f(v + math.pi / 2 * np.eye(4)[i]) - f(v - math.pi / 2 * np.eye(4)[i])) / 2 for i in range(4)])
     fd = O.central_fd(f, v, h=1e-5)
     assert np.allclose(ps, fd, atol=1e-8)
+
+
+# ------------------------------------------------------------------ restated reference path
+def test_refpath_autograd_matches_oracle():
+    """oracle/refpath.py's differentiable restatement of the reference's
+    per-gate torch path: states and Pauli expectations equal the oracle's
+    (complex128), its autograd gradient equals central FD of the oracle, and
+    the complex64 run (the reference's dtype) is within float32 rounding."""
+    import torch
+    from oracle import refpath as R
+    rng = np.random.default_rng(13)
+    n, bsz = 4, 2
+    names = ["rx", "ry", "rz", "u3", "cnot", "hadamard", "cz"]
+    ops, base = [], []
+    for _ in range(24):
+        nm = names[rng.integers(len(names))]
+        k = 2 if nm in ("cnot", "cz") else 1
+        w = [int(x) for x in rng.choice(n, k, replace=False)]
+        npar = {"rx": 1, "ry": 1, "rz": 1, "u3": 3}.get(nm, 0)
+        rows = bsz if (npar and rng.random() < 0.3) else 1
+        p = rng.uniform(-3, 3, size=(rows, npar)) if npar else None
+        ops.append(dict(name=nm, wires=w, inverse=bool(rng.random() < 0.2)))
+        base.append(p)
+    obs = ["ZXIY", "XXZI", "IYYZ"]
+
+    def oracle_f(flat):
+        k, seq = 0, []
+        for op, p in zip(ops, base):
+            if p is not None:
+                q = flat[k:k + p.size].reshape(p.shape)
+                k += p.size
+                seq.append(dict(op, params=q if q.shape[0] > 1 else q[0]))
+            else:
+                seq.append(dict(op, params=None))
+        st = O.run_ops(seq, n, bsz)
+        return st, float(sum(O.expval_pauli(st, o).sum() for o in obs))
+
+    flat0 = np.concatenate([p.reshape(-1) for p in base if p is not None])
+    st_ref, _ = oracle_f(flat0)
+    for cdtype, tol in ((torch.complex128, 1e-12), (torch.complex64, 1e-5)):
+        tp = [None if p is None else torch.tensor(p, requires_grad=True) for p in base]
+        st = R.run_autograd(ops, tp, n, bsz, cdtype)
+        assert np.abs(st.reshape(bsz, -1).detach().numpy() - st_ref).max() < tol
+        loss = sum(R.expval_pauli(st, o).sum() for o in obs)
+        loss.backward()
+        g = np.concatenate([t.grad.numpy().reshape(-1) for t in tp if t is not None])
+        fd = O.central_fd(lambda v: oracle_f(v)[1], flat0.copy(), h=1e-5)
+        assert np.abs(g - fd).max() < (1e-7 if cdtype == torch.complex128 else 1e-4)
