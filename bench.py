@@ -1,19 +1,25 @@
 """Benchmark: amplitude updates/s & adjoint grads/s vs the plain per-gate kernel,
 and % of HBM peak, on one MI355X.
 
-Workload of the headline line (``value``): 20 qubits, bsz 16, complex64;
-20 layers of [RY(theta ~ U[0, 2pi)) on every wire, CZ on even then odd pairs];
-<Z_i> per wire; per-gate HIP kernels, forward only.  One step = one forward
-pass of that circuit over the batch plus the <Z> read-out.
+Headline (``value``): adjoint gradient evaluations/s of the fused engine on
+24 qubits, bsz 64, complex64: 10 x [RX, RY, RZ on every wire (angles
+N(0, 0.1)), CNOT ring], a per-row RY input layer, loss = mean_b [sum_i
+<Z_i Z_i+1> + 0.5 sum_i <X_i>], d loss / d theta for all 720 parameters.
+One step = one forward pass + the loss + the adjoint backward.
 
-Extra fields in the same JSON line:
-  * fused:   the same circuit through the fused-stage engine;
-  * adjoint: 24 qubits, bsz 64, 10 x [RX, RY, RZ all wires (N(0, 0.1)), CNOT
-    ring], loss = sum <Z_i Z_i+1> + 0.5 sum <X_i>; gradient evaluations/s for
-    the fused+adjoint engine and for the plain per-gate adjoint path;
-  * roofline of the dominant kernel (1-qubit per-gate kernel, HIP events);
-  * cpu_baseline: the reference's per-gate torch bmm path (restated) on the
-    host cores, on a bounded sample of the headline circuit.
+Fields of the same JSON line:
+  * roofline: the dominant kernel of that step, the compiled adjoint stage
+    (HIP events around every launch inside the timed loop); the compiled
+    forward stage and the dense 1-qubit gate on the same HBM-resident state
+    are reported beside it;
+  * pergate_gradient: the same gradient through the plain per-gate kernels
+    (same adjoint algorithm, no fusion) -- the ">= 5x" denominator;
+  * config2 / small_circuit / dense_block / batch / config5: the other
+    workloads, each with its own rate and, where it applies, roofline;
+  * cpu_baseline: the reference's per-gate torch bmm path (restated) with
+    torch autograd on the host cores, on a bounded sample of the headline
+    workload, scaled to gradient evaluations/s;
+  * oracle_error: errors against the complex128 oracle for every workload.
 """
 from __future__ import annotations
 
@@ -70,47 +76,108 @@ def time_forward(n, bsz, ops, mode, steps, warmup, tq, jit="auto"):
     return dt, out
 
 
-def time_gradient(n, bsz, layers, mode, steps, warmup, tq, jit="auto"):
-    from circuits import config3_ops, config3_observables
+def _ansatz_loss_step(n, bsz, layers, mode, tq, jit, params, phi, ops, obs, coef):
     import torchquantum_amd.functional as tqf
+    for p in params:
+        p.grad = None
+    q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
+    q.engine_options = tq.EngineOptions(mode=mode, jit=jit)
+    for w in range(n):
+        tqf.ry(q, w, params=phi)
+    k = 0
+    for op in ops:
+        if op["params"] is None:
+            tqf.func_name_dict[op["name"]](q, op["wires"])
+        else:
+            tqf.func_name_dict[op["name"]](q, op["wires"], params=params[k])
+            k += 1
+    loss = tq.expval_pauli_sum(q, obs, coef).mean()
+    if params[0].requires_grad:
+        loss.backward()
+    return loss
+
+
+def ansatz_inputs(n, bsz, layers, theta=None):
+    """Circuit, observables, per-row input angles and parameter tensors of the
+    gradient workload (angles N(0, 0.1) from seed 0, input RY angles U[0, pi)
+    from seed 1)."""
+    from circuits import config3_ops, config3_observables
     ops = config3_ops(seed=0, n=n, layers=layers)
-    params = [torch.tensor(op["params"], dtype=torch.float32, device="cuda", requires_grad=True) for op in ops
-              if op["params"] is not None]
+    if theta is None:
+        theta = [float(np.asarray(op["params"]).reshape(-1)[0]) for op in ops if op["params"] is not None]
+    params = [torch.tensor([[t]], dtype=torch.float32, device="cuda", requires_grad=True) for t in theta]
     obs, coef = config3_observables(n)
-    # per-row input layer makes the batch rows distinct: RY(phi_b) on every wire
     g = torch.Generator().manual_seed(1)
     phi = (torch.rand(bsz, 1, generator=g) * np.pi).to("cuda")
+    return ops, obs, coef, phi, params
+
+
+def time_gradient(n, bsz, layers, mode, steps, warmup, tq, jit="auto", timer=None):
+    """Seconds per gradient evaluation (mean of ``steps`` after ``warmup``);
+    with ``timer`` a list, HIP events around every stage launch of the timed
+    steps are collected into it."""
+    from torchquantum_amd import engine as E
+    ops, obs, coef, phi, params = ansatz_inputs(n, bsz, layers)
 
     def step():
-        for p in params:
-            p.grad = None
-        q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
-        q.engine_options = tq.EngineOptions(mode=mode, jit=jit)
-        for w in range(n):
-            tqf.ry(q, w, params=phi)
-        k = 0
-        for op in ops:
-            if op["params"] is None:
-                tqf.func_name_dict[op["name"]](q, op["wires"])
-            else:
-                tqf.func_name_dict[op["name"]](q, op["wires"], params=params[k])
-                k += 1
-        loss = tq.expval_pauli_sum(q, obs, coef).mean()
-        loss.backward()
-        return loss
+        return _ansatz_loss_step(n, bsz, layers, mode, tq, jit, params, phi, ops, obs, coef)
 
-    for _ in range(warmup):
+    for i in range(warmup):
         step()
+        log(f"[gradient {n}q {mode}] warm-up step {i + 1}/{warmup}")
     if warmup and tq.wait_for_compiled_stages():
         step()
+        log(f"[gradient {n}q {mode}] compiled stage kernels ready")
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        loss = step()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
+    if timer is not None:
+        E.set_stage_timer(timer)
+    try:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            loss = step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+    finally:
+        E.set_stage_timer(None)
     grads = torch.cat([p.grad.reshape(-1) for p in params])
     return dt, float(loss.detach()), grads
+
+
+def stage_launch_stats(records, steps, direction, amps, bytes_per_amp):
+    """Average duration (HIP events) of the compiled stage launches of one
+    direction, and the algorithmic rate: bytes_per_amp x amps per launch."""
+    ms = [e0.elapsed_time(e1) for d, comp, e0, e1 in records if d == direction and comp]
+    if not ms:
+        return None
+    avg = float(np.mean(ms))
+    nbytes = bytes_per_amp * amps
+    return {"avg_launch_ms": avg, "launches_per_step": len(ms) / steps, "ms_per_step": sum(ms) / steps,
+            "bytes_per_launch": nbytes, "achieved_GBs": nbytes / (avg * 1e-3) / 1e9}
+
+
+def parameter_shift_check(n, bsz, layers, tq, n_params=8, seed=7):
+    """Full-size check of the fused adjoint gradient: the parameter-shift rule
+    (f(t + pi/2) - f(t - pi/2)) / 2, exact for RX/RY/RZ
+    (util/utils.py:1017-1075), from GPU forward passes only, on seeded
+    parameters of the workload."""
+    ops, obs, coef, phi, params = ansatz_inputs(n, bsz, layers)
+    _ansatz_loss_step(n, bsz, layers, "fused", tq, "auto", params, phi, ops, obs, coef)
+    g = torch.cat([p.grad.reshape(-1) for p in params]).cpu().numpy().astype(np.float64)
+    theta = [float(p.detach()) for p in params]
+    idx = np.random.default_rng(seed).choice(len(theta), n_params, replace=False)
+    errs = []
+    with torch.no_grad():
+        for i in idx:
+            vals = []
+            for sh in (np.pi / 2, -np.pi / 2):
+                th = list(theta)
+                th[i] += sh
+                _, _, _, _, ps = ansatz_inputs(n, bsz, layers, theta=th)
+                for p in ps:
+                    p.requires_grad_(False)
+                vals.append(float(_ansatz_loss_step(n, bsz, layers, "fused", tq, "auto", ps, phi, ops, obs, coef)))
+            errs.append(abs(g[i] - (vals[0] - vals[1]) / 2))
+    return {"params_checked": [int(i) for i in idx], "grad_max_abs_vs_parameter_shift": float(max(errs))}
 
 
 def kernel_roofline(n, bsz, iters=50):
@@ -222,13 +289,65 @@ def time_batch(n_circ, steps, tq):
     ref = per_circuit()
     torch.cuda.synchronize()
     t_loop = time.perf_counter() - t0
+    alg = batch_pass_bytes(cb)
     return {"workload": f"{n_circ} circuits, n ~ U{{8..22}}, depth ~ U{{4..40}} layers of n random gates "
                         "from {H,RX,RY,RZ,U3,CNOT,CZ,SWAP}, <Z_0> each, bsz 1",
+            "roofline": {"bound": "hbm", "algorithmic_bytes_per_batch": alg,
+                         "achieved": alg / t_graph / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": alg / t_graph / 1e9 / HBM_PEAK_GBS,
+                         "stages_total": int(sum(len(o) for o in cb.stage_offsets)),
+                         "pergate_equivalent_bytes": pergate_bytes(circs),
+                         "note": "16 B x 2^n per executed stage of each circuit + the <Z_0> read, per graph "
+                                 "replay; states of <= 64 MiB are cache-resident, so this is not an HBM figure"},
             "levels": len(cb.levels), "launches_per_batch": cb.n_launches,
             "ms_graph_replay": t_graph * 1e3, "ms_eager_batch": t_eager * 1e3, "ms_per_circuit_loop": t_loop * 1e3,
             "amp_updates_per_s_graph": upd / t_graph, "speedup_vs_per_circuit_loop": t_loop / t_graph,
             "build_s": t_build, "z0_max_abs_diff_vs_per_circuit": float((out - ref).abs().max())}, \
         (out.reshape(-1).cpu().numpy(), circs)
+
+
+def encoder_training(tq, n, bsz, blocks, steps, warmup):
+    """One QML training step = encoder (per-row RY on every wire from a
+    [bsz, n] input, matrices built on the GPU) + RXYZCXLayer0 ansatz +
+    <Z_i> read-out + a linear head + cross entropy, backward, and an Adam
+    update of the ansatz and head inside the timed loop (the training loop of
+    examples/mnist/mnist.py:82-135 at a larger size).  New synthetic inputs
+    every step."""
+    enc = tq.GeneralEncoder([{"input_idx": [i], "func": "ry", "wires": [i]} for i in range(n)])
+    ansatz = tq.RXYZCXLayer0({"n_wires": n, "n_blocks": blocks}).to("cuda")
+    head = torch.nn.Linear(n, 4).to("cuda")
+    opt = torch.optim.Adam(list(ansatz.parameters()) + list(head.parameters()), lr=1e-2)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    xs = [torch.rand(bsz, n, device="cuda", generator=g) * np.pi for _ in range(4)]
+    ys = [torch.randint(0, 4, (bsz,), device="cuda", generator=g) for _ in range(4)]
+
+    def step(i):
+        q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
+        enc(q, xs[i % 4])
+        ansatz(q)
+        logits = head(tq.MeasureAll(tq.PauliZ)(q))
+        loss = torch.nn.functional.cross_entropy(logits, ys[i % 4])
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        return loss
+
+    for i in range(warmup):
+        step(i)
+    tq.wait_for_compiled_stages()
+    step(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        loss = step(i)
+    t_host = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    n_par = sum(p.numel() for p in ansatz.parameters())
+    return {"workload": f"{n} qubits, bsz {bsz}: RY encoder from a [bsz, {n}] input + RXYZCXLayer0 x {blocks} "
+                        f"({n_par} params) + <Z_i> + linear head, cross entropy, Adam step; new inputs each step",
+            "ms_per_step": dt * 1e3, "steps_per_s": 1.0 / dt, "host_issue_ms_per_step": t_host / steps * 1e3,
+            "loss_last": float(loss.detach())}
 
 
 def copy_calibration(nbytes=1 << 31):
@@ -244,10 +363,11 @@ def copy_calibration(nbytes=1 << 31):
     return nbytes, e0.elapsed_time(e1)
 
 
-def load_traffic():
-    """HBM bytes per launch of the dominant kernel from the committed counter
-    summary (two rocprofv3 --pmc runs, scaled by the copy calibration)."""
-    path = os.path.join(ROOT, "profiles", "k_1q_traffic.json")
+def load_traffic(name):
+    """HBM bytes per launch of a kernel from a committed counter summary
+    (profiles/<name>.json: two rocprofv3 --pmc runs, FETCH_SIZE and
+    WRITE_SIZE, scaled by what they read for a device copy of known size)."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -327,15 +447,84 @@ def config5(args, tq):
     return res
 
 
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(ops_sample, n, bsz, n_gates_total_per_step):
     from oracle import refpath
-    threads = os.cpu_count() or 1
+    threads = cpu_threads()
     torch.set_num_threads(threads)
     t0 = time.perf_counter()
     refpath.run(ops_sample, n, bsz, device="cpu")
     dt = time.perf_counter() - t0
     upd = len(ops_sample) * bsz * (1 << n)
     return upd / dt, threads, dt
+
+
+def cpu_baseline_gradient(n, bsz_full, layers, n_gates):
+    """The reference's per-gate torch path under torch autograd (what the
+    reference computes for a gradient: gate_wrapper.py:96-140, complex64,
+    every bmm input saved for backward) on the host cores: the first
+    ``n_gates`` gates of the gradient workload at bsz 1, forward + backward,
+    loss restricted to the ZZ ring (one diagonal pass).  Scaled to gradient
+    evaluations/s of the whole workload: 1 / (t_sample x gates / n_gates x bsz)."""
+    from oracle import refpath as R
+    from circuits import config3_ops
+    threads = cpu_threads()
+    torch.set_num_threads(threads)
+    ops = config3_ops(seed=0, n=n, layers=layers)
+    total = len(ops) + n                      # + the per-row input layer
+    sample = ops[:n_gates]
+    params = [None if op["params"] is None else
+              torch.tensor(np.asarray(op["params"], dtype=np.float32).reshape(1, -1), requires_grad=True)
+              for op in sample]
+    j = torch.arange(1 << n)
+    sign = torch.zeros(1 << n)
+    for i in range(n):
+        b0 = (j >> (n - 1 - i)) & 1
+        b1 = (j >> (n - 1 - (i + 1) % n)) & 1
+        sign += (1 - 2 * (b0 ^ b1)).float()
+    del j
+    t0 = time.perf_counter()
+    st = R.run_autograd(sample, params, n, 1, torch.complex64)
+    loss = ((st.reshape(1, -1).abs() ** 2) * sign).sum()
+    loss.backward()
+    dt = time.perf_counter() - t0
+    return 1.0 / (dt * (total / n_gates) * bsz_full), threads, dt, total
+
+
+F_PASS = {"cnot": 0.5, "swap": 0.5, "cz": 0.25, "toffoli": 0.25, "cswap": 0.25}
+
+
+def pergate_bytes(circuits):
+    """Sum over circuits and gates of 16 B x 2^n x f (f = 1 dense, 1/2 CNOT/SWAP,
+    1/4 CZ/Toffoli/CSWAP), plus the <Z_0> read-out (8 B x 2^n) per circuit:
+    the traffic of a per-gate execution."""
+    tot = 0.0
+    for n, ops in circuits:
+        a = float(1 << n)
+        tot += sum(16.0 * a * F_PASS.get(op["name"], 1.0) for op in ops) + 8.0 * a
+    return tot
+
+
+def batch_pass_bytes(cb):
+    """Algorithmic bytes of the batch as executed: every stage of circuit c is
+    one read + write of its 2^n_c amplitudes (16 B each), plus the <Z_0>
+    read-out (8 B per amplitude)."""
+    return float(sum((16.0 * len(offs) + 8.0) * (1 << n) for n, offs in zip(cb.ns, cb.stage_offsets)))
 
 
 def oracle_errors(tq, batch_out=None):
@@ -439,27 +628,32 @@ def oracle_errors(tq, batch_out=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=10, help="timed gradient evaluations of the headline workload")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--adj-qubits", type=int, default=24)
     ap.add_argument("--adj-bsz", type=int, default=64)
-    ap.add_argument("--adj-steps", type=int, default=2)
-    ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--pergate-steps", type=int, default=1, help="timed per-gate gradient evaluations")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-config5", action="store_true")
+    ap.add_argument("--skip-extras", action="store_true",
+                    help="headline gradient (fused, per-gate) and its roofline only")
     ap.add_argument("--only-roofline", action="store_true",
-                    help="run only the dominant kernel's timing loops (for rocprofv3 summaries)")
+                    help="run only the dense 1-qubit kernel timing loops (for rocprofv3 summaries)")
     ap.add_argument("--config5", action="store_true", help="run the 30-qubit workload only")
     ap.add_argument("--only-dense", action="store_true", help="run only the dense k = 5, 6 block timings")
     ap.add_argument("--only-fused", action="store_true",
-                    help="run only the fused forward of the 20-qubit config-2 circuit (--steps, --warmup)")
+                    help="run only the fused forward of the 20-qubit RY/CZ circuit (--steps, --warmup)")
     ap.add_argument("--only-pergate", action="store_true",
-                    help="run only the per-gate forward of the 20-qubit config-2 circuit (--steps, --warmup)")
+                    help="run only the per-gate forward of the 20-qubit RY/CZ circuit (--steps, --warmup)")
     ap.add_argument("--only-gradient", action="store_true",
-                    help="run only the fused adjoint gradient (--adj-qubits, --adj-bsz, --adj-steps)")
+                    help="run only the fused adjoint gradient (--adj-qubits, --adj-bsz, --steps)")
+    ap.add_argument("--only-batch", action="store_true", help="run only the uneven batch")
+    ap.add_argument("--only-encoder", action="store_true", help="run only the encoder training steps")
     ap.add_argument("--c5-qubits", type=int, default=30)
     ap.add_argument("--c5-bsz", type=int, default=4)
     ap.add_argument("--cpu-layers", type=int, default=2)
+    ap.add_argument("--cpu-grad-gates", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding over several "
@@ -474,126 +668,167 @@ def main():
     if args.only_roofline:
         cb, cms = copy_calibration()
         ms, nbytes = kernel_roofline(20, 16, iters=60)
-        ms_big, nbytes_big = kernel_roofline(26, 16, iters=26)
+        ms_big, nbytes_big = kernel_roofline(24, 64, iters=24)
         print(json.dumps({"copy_bytes": cb, "copy_ms": cms, "k_1q_20q_bsz16_ms": ms,
-                          "k_1q_20q_bsz16_GBs": nbytes / ms / 1e6, "k_1q_26q_bsz16_ms": ms_big,
-                          "k_1q_26q_bsz16_GBs": nbytes_big / ms_big / 1e6}), flush=True)
+                          "k_1q_20q_bsz16_GBs": nbytes / ms / 1e6, "k_1q_24q_bsz64_ms": ms_big,
+                          "k_1q_24q_bsz64_GBs": nbytes_big / ms_big / 1e6}), flush=True)
         return
     if args.only_dense:
         print(json.dumps({"dense_block": dense_block_bench()}), flush=True)
         return
+    if args.only_encoder:
+        print(json.dumps({"encoder_training": {"small": encoder_training(tq, 10, 256, 4, 20, 3),
+                                               "large": encoder_training(tq, 22, 64, 4, 3, 2)}}), flush=True)
+        return
+    if args.only_batch:
+        res, _ = time_batch(256, 5, tq)
+        print(json.dumps({"batch": res}), flush=True)
+        return
     if args.only_gradient:
         an, ab = args.adj_qubits, args.adj_bsz
-        dt, loss, _ = time_gradient(an, ab, 10, "fused", args.adj_steps, 1, tq)
-        print(json.dumps({"fused_gradient_s": dt, "workload": f"{an} qubits, bsz {ab}", "loss": loss}), flush=True)
+        rec = []
+        dt, loss, _ = time_gradient(an, ab, 10, "fused", args.steps, 1, tq, timer=rec)
+        amps = float(ab) * (1 << an)
+        print(json.dumps({"fused_gradient_s": dt, "workload": f"{an} qubits, bsz {ab}", "loss": loss,
+                          "adjoint_stage": stage_launch_stats(rec, args.steps, "a", amps, 32.0),
+                          "forward_stage": stage_launch_stats(rec, args.steps, "f", amps, 16.0)}), flush=True)
         return
     if args.only_fused or args.only_pergate:
         n, bsz = 20, 16
         mode = "fused" if args.only_fused else "pergate"
         ops = config2_ops(seed=0, n=n, layers=20)
         dt, _ = time_forward(n, bsz, ops, mode, args.steps, args.warmup, tq)
-        print(json.dumps({f"{mode}_forward_ms": dt * 1e3, "workload": f"{n} qubits, bsz {bsz}, config-2 circuit",
+        print(json.dumps({f"{mode}_forward_ms": dt * 1e3, "workload": f"{n} qubits, bsz {bsz}, RY/CZ circuit",
                           "amp_updates_per_s": len(ops) * bsz * (1 << n) / dt}), flush=True)
         return
     if args.config5:
         print(json.dumps({"config5": config5(args, tq)}), flush=True)
         return
 
-    n, bsz = 20, 16
-    ops = config2_ops(seed=0, n=n, layers=20)
-    upd_per_step = len(ops) * bsz * (1 << n)
+    # ---------------------------------------------------------------- headline: adjoint gradient
+    an, ab = args.adj_qubits, args.adj_bsz
+    amps = float(ab) * (1 << an)
+    log(f"[bench] fused adjoint gradient, {an} qubits bsz {ab} (warm-up compiles the stage kernels)")
+    rec = []
+    dt_af, loss_f, g_f = time_gradient(an, ab, 10, "fused", args.steps, args.warmup, tq, timer=rec)
+    log(f"[bench] fused {dt_af:.4f} s/gradient")
+    adj = stage_launch_stats(rec, args.steps, "a", amps, 32.0)
+    fwd = stage_launch_stats(rec, args.steps, "f", amps, 16.0)
+    del rec
+    torch.cuda.empty_cache()
+    log("[bench] per-gate adjoint gradient")
+    dt_ap, loss_p, g_p = time_gradient(an, ab, 10, "pergate", args.pergate_steps, 1, tq)
+    log(f"[bench] per-gate {dt_ap:.3f} s/gradient")
+    log("[bench] fused gradient with the program-interpreting stage kernels")
+    dt_ai, _, _ = time_gradient(an, ab, 10, "fused", 1, 1, tq, jit="off")
+    torch.cuda.empty_cache()
+    log("[bench] dense 1-qubit gate roofline (same state size, and a cache-resident one)")
+    ms_d, nbytes_d = kernel_roofline(an, ab, iters=2 * an)
+    ms_c, nbytes_c = kernel_roofline(20, 16, iters=60)
+    torch.cuda.empty_cache()
+    n_params = len(g_f)
+    traffic = load_traffic("adjoint_stage_24q_traffic") or {}
+    extra = {
+        "pergate_gradient": {"grads_per_s": 1.0 / dt_ap, "s_per_gradient": dt_ap,
+                             "fused_speedup_vs_pergate": dt_ap / dt_af,
+                             "grad_max_abs_diff_fused_vs_pergate": float((g_f - g_p).abs().max())},
+        "fused_interpreting_stage_kernels": {"grads_per_s": 1.0 / dt_ai, "s_per_gradient": dt_ai},
+        "forward_stage": None if fwd is None else {**fwd, "frac": fwd["achieved_GBs"] / HBM_PEAK_GBS,
+                                                   "kernel": "compiled forward stage tqsv_jf_* (16 B/amplitude)"},
+        "dense_gate": {"kernel": "dense 1-qubit gate k_1q_t0/k_1q_tile/k_1q_pair, RY cycling over every target "
+                                 f"bit, {an} qubits bsz {ab} ({nbytes_d / 2**30:.0f} GiB, HBM-resident)",
+                       "avg_launch_ms": ms_d, "achieved": nbytes_d / (ms_d * 1e-3) / 1e9,
+                       "frac": nbytes_d / (ms_d * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "cache_resident_20q_bsz16": {
+                           "avg_launch_ms": ms_c, "achieved": nbytes_c / (ms_c * 1e-3) / 1e9,
+                           "note": "256 MiB state held by the 256 MB Infinity Cache: not an HBM figure"}},
+    }
 
-    log("[bench] config2 per-gate forward")
-    dt_pg, ev_pg = time_forward(n, bsz, ops, "pergate", args.steps, args.warmup, tq)
-    log(f"[bench] per-gate {dt_pg * 1e3:.2f} ms/step")
-    log("[bench] config2 fused forward")
-    dt_f, ev_f = time_forward(n, bsz, ops, "fused", args.steps, args.warmup, tq)
-    log(f"[bench] fused {dt_f * 1e3:.2f} ms/step")
-    dt_fi, _ = time_forward(n, bsz, ops, "fused", args.steps, args.warmup, tq, jit="off")
-    log(f"[bench] fused, program-interpreting stage kernels {dt_fi * 1e3:.2f} ms/step")
-    parity = float((ev_pg - ev_f).abs().max())
-
-    # the reference's own CPU-runnable case: 10 qubits, bsz 32, 8 x [U3 all, CNOT ring], <Z_i>
-    # (host-bound: timed before the large-state sections)
-    extra = {}
-    from circuits import config1_ops
-    ops1 = config1_ops(seed=0)
-    upd1 = len(ops1) * 32 * (1 << 10)
-    log("[bench] 10-qubit bsz-32 circuit (160 gates)")
-    t1_pg, _ = time_forward(10, 32, ops1, "pergate", 20, 3, tq)
-    t1_f, _ = time_forward(10, 32, ops1, "fused", 20, 3, tq)
-    extra["small_circuit"] = {"workload": "10 qubits, bsz 32, 8 x [U3 all wires, CNOT ring], <Z_i> (160 gates)",
-                              "pergate_ms_per_circuit_batch": t1_pg * 1e3, "fused_ms_per_circuit_batch": t1_f * 1e3,
-                              "fused_amp_updates_per_s": upd1 / t1_f}
-
-    log("[bench] kernel roofline")
-    ms, nbytes = kernel_roofline(n, bsz, iters=60)
-    achieved = nbytes / (ms * 1e-3) / 1e9
-    ms_big, nbytes_big = kernel_roofline(26, 16, iters=26)
-    achieved_big = nbytes_big / (ms_big * 1e-3) / 1e9
-
-    log("[bench] dense k = 5, 6 blocks")
-    extra["dense_block"] = dense_block_bench()
-    if not args.skip_adjoint:
-        an, ab = args.adj_qubits, args.adj_bsz
-        log(f"[bench] adjoint gradient fused n={an} bsz={ab}")
-        dt_af, loss_f, g_f = time_gradient(an, ab, 10, "fused", args.adj_steps, 1, tq)
-        log(f"[bench] adjoint fused {dt_af:.3f} s/grad")
-        dt_ai, _, _ = time_gradient(an, ab, 10, "fused", 1, 1, tq, jit="off")
-        log(f"[bench] adjoint fused, program-interpreting stage kernels {dt_ai:.3f} s/grad")
-        log(f"[bench] adjoint per-gate n={an} bsz={ab}")
-        dt_ap, loss_p, g_p = time_gradient(an, ab, 10, "pergate", args.adj_steps, 1, tq)
-        log(f"[bench] adjoint per-gate {dt_ap:.3f} s/grad")
-        extra["adjoint"] = {
-            "workload": f"{an} qubits, bsz {ab}, 10 x [RX,RY,RZ all wires, CNOT ring], 720 params",
-            "fused_grads_per_s": 1.0 / dt_af, "pergate_grads_per_s": 1.0 / dt_ap,
-            "fused_interpreting_stage_kernels_grads_per_s": 1.0 / dt_ai,
-            "speedup_vs_pergate": dt_ap / dt_af,
-            "grad_max_abs_diff_fused_vs_pergate": float((g_f - g_p).abs().max()),
-            "loss": loss_f,
-        }
+    if not args.skip_extras:
+        n, bsz = 20, 16
+        ops = config2_ops(seed=0, n=n, layers=20)
+        upd = len(ops) * bsz * (1 << n)
+        log("[bench] 20-qubit RY/CZ circuit: per-gate and fused forward")
+        dt_pg, ev_pg = time_forward(n, bsz, ops, "pergate", 10, 3, tq)
+        dt_f, ev_f = time_forward(n, bsz, ops, "fused", 10, 3, tq)
+        dt_fi, _ = time_forward(n, bsz, ops, "fused", 10, 3, tq, jit="off")
+        extra["config2"] = {"workload": "20 qubits, bsz 16, 20 x [RY all wires, CZ even+odd pairs], <Z_i> (780 gates)",
+                            "pergate_amp_updates_per_s": upd / dt_pg, "pergate_ms_per_step": dt_pg * 1e3,
+                            "fused_amp_updates_per_s": upd / dt_f, "fused_ms_per_step": dt_f * 1e3,
+                            "fused_speedup_vs_pergate": dt_pg / dt_f,
+                            "fused_interpreting_ms_per_step": dt_fi * 1e3,
+                            "expval_max_abs_diff_fused_vs_pergate": float((ev_pg - ev_f).abs().max())}
+        from circuits import config1_ops
+        ops1 = config1_ops(seed=0)
+        upd1 = len(ops1) * 32 * (1 << 10)
+        log("[bench] 10-qubit bsz-32 circuit (160 gates)")
+        t1_pg, _ = time_forward(10, 32, ops1, "pergate", 20, 3, tq)
+        t1_f, _ = time_forward(10, 32, ops1, "fused", 20, 3, tq)
+        extra["small_circuit"] = {"workload": "10 qubits, bsz 32, 8 x [U3 all wires, CNOT ring], <Z_i> (160 gates)",
+                                  "pergate_ms_per_circuit_batch": t1_pg * 1e3,
+                                  "fused_ms_per_circuit_batch": t1_f * 1e3, "fused_amp_updates_per_s": upd1 / t1_f}
+        log("[bench] dense k = 5, 6 blocks")
+        extra["dense_block"] = dense_block_bench()
+        log("[bench] encoder + ansatz training steps")
+        extra["encoder_training"] = {"small": encoder_training(tq, 10, 256, 4, 20, 3),
+                                     "large": encoder_training(tq, 22, 64, 4, 3, 2)}
+        torch.cuda.empty_cache()
 
     batch_out = None
-    if not args.skip_batch:
+    if not args.skip_batch and not args.skip_extras:
         log("[bench] uneven batch (256 circuits)")
         extra["batch"], batch_out = time_batch(256, 5, tq)
         log(f"[bench] batch graph {extra['batch']['ms_graph_replay']:.2f} ms")
+        torch.cuda.empty_cache()
 
+    if not args.skip_config5 and not args.skip_extras:
+        extra["config5"] = config5(args, tq)
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.skip_cpu:
-        sample = ops[: 39 * args.cpu_layers]
-        log(f"[bench] cpu baseline ({len(sample)} gates)")
-        rate, threads, cdt = cpu_baseline(sample, n, bsz, len(ops))
-        cpu = {"value": rate, "unit": "amplitude updates/s", "cores": threads, "kind": "port",
-               "sample": f"first {len(sample)} gates of the 780-gate circuit, 20 qubits bsz 16, torch CPU "
-                         f"bmm path restated from gate_wrapper.py:96-140 ({cdt:.1f} s)"}
-        rate1, _, cdt1 = cpu_baseline(ops1, 10, 32, len(ops1))
-        cpu["small_circuit"] = {"ms_per_circuit_batch": cdt1 * 1e3, "amp_updates_per_s": rate1,
-                                "sample": "the whole 10-qubit bsz-32 circuit (160 gates), same CPU path"}
-        extra["small_circuit"]["fused_speedup_vs_cpu"] = cdt1 / t1_f
+        log(f"[bench] cpu baseline: reference path + autograd, {args.cpu_grad_gates} gates at {an} q bsz 1")
+        rate, threads, cdt, total = cpu_baseline_gradient(an, ab, 10, args.cpu_grad_gates)
+        cpu = {"value": rate, "unit": "gradient evaluations/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(),
+               "sample": f"first {args.cpu_grad_gates} of the {total} gates at {an} qubits bsz 1, forward + "
+                         f"torch autograd backward through the restated reference bmm path "
+                         f"(gate_wrapper.py:96-140, complex64), loss = ZZ ring; {cdt:.1f} s, scaled by "
+                         f"{total}/{args.cpu_grad_gates} gates x bsz {ab}"}
+        if not args.skip_extras:
+            sample = config2_ops(seed=0, n=20, layers=20)[: 39 * args.cpu_layers]
+            r2, _, c2 = cpu_baseline(sample, 20, 16, 780)
+            cpu["config2_forward"] = {"amp_updates_per_s": r2, "sample": f"first {len(sample)} gates of the "
+                                      f"780-gate circuit, 20 qubits bsz 16, forward ({c2:.1f} s)"}
+            from circuits import config1_ops
+            r1, _, c1 = cpu_baseline(config1_ops(seed=0), 10, 32, 160)
+            cpu["small_circuit"] = {"ms_per_circuit_batch": c1 * 1e3, "amp_updates_per_s": r1}
+        log("[bench] oracle errors")
         extra["oracle_error"] = oracle_errors(tq, batch_out)
+        log("[bench] full-size parameter-shift check")
+        extra["oracle_error"][f"gradient at {an} q bsz {ab}: fused adjoint vs parameter shift"] = \
+            parameter_shift_check(an, ab, 10, tq)
 
     line = {
         "metric": "amplitude updates/s & adjoint grads/s vs plain per-gate kernel, % HBM peak",
-        "value": upd_per_step / dt_pg,
-        "unit": "amplitude updates/s",
+        "value": 1.0 / dt_af,
+        "unit": "gradient evaluations/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": dt_pg * 1e3,
+        "ms_per_step": dt_af * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "complex64 (fp32)", "data": "synthetic (seeded angles)",
-        "config": {"workload": "20 qubits, bsz 16, 20 x [RY all wires, CZ even+odd pairs], <Z_i>, per-gate "
-                               "kernels, forward", "n_qubits": n, "bsz": bsz, "gates": len(ops)},
-        "fused": {"amp_updates_per_s": upd_per_step / dt_f, "ms_per_step": dt_f * 1e3,
-                  "speedup_vs_pergate": dt_pg / dt_f, "expval_max_abs_diff_vs_pergate": parity,
-                  "ms_per_step_interpreting_stage_kernels": dt_fi * 1e3},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (load_traffic() or {}).get("k_1q_20q_bsz16_bytes_per_launch"),
-                     "kernel": "dense 1-qubit gate k_1q_t0/k_1q_tile/k_1q_pair (RY over every target bit, "
-                               "20 qubits bsz 16: 256 MiB/launch)", "avg_launch_ms": ms,
-                     "at_26q_bsz16": {"achieved": achieved_big, "frac": achieved_big / HBM_PEAK_GBS,
-                                      "avg_launch_ms": ms_big}},
+        "config": {"workload": f"{an} qubits, bsz {ab}, 10 x [RX, RY, RZ all wires, CNOT ring] + per-row RY input "
+                               f"layer, loss = mean_b [sum <Z_i Z_i+1> + 0.5 sum <X_i>], adjoint gradient of "
+                               f"{n_params} parameters, fused stages", "n_qubits": an, "bsz": ab,
+                   "params": n_params, "loss": loss_f},
+        "roofline": None if adj is None else {
+            "bound": "hbm", "achieved": adj["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": adj["achieved_GBs"] / HBM_PEAK_GBS,
+            "traffic": traffic.get("bytes_per_launch"),
+            "kernel": "compiled adjoint stage tqsv_ja_* (reads psi and lam, writes both: 32 B per amplitude, "
+                      f"{adj['bytes_per_launch'] / 1e9:.2f} GB per launch)",
+            "avg_launch_ms": adj["avg_launch_ms"], "launches_per_step": adj["launches_per_step"],
+            "share_of_step": adj["ms_per_step"] / (dt_af * 1e3)},
         "cpu_baseline": cpu,
         **extra,
     }

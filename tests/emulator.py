@@ -78,7 +78,7 @@ def stage_apply(words, st, mats, n, init_zero=False, adjoint=False, lam=None, gr
                     assert int(h[E.H_FLAGS]) & E.FEAT_MATK, "MATK op in a stage without the MATK feature"
                 tok = (tid & w[4]) == w[5]
                 rm, rv = w[6] & 255, (w[6] >> 8) & 255
-                P = ((8 if (w[0] & E.F_MG) else (w[0] >> 4) & 15)) if w[7] >= 0 else 0
+                P = ((4 if (w[0] & E.F_MG) else (w[0] >> 4) & 15)) if w[7] >= 0 else 0
                 if not adjoint:
                     a = _fwd(typ, a, M, w, tok, rm, rv, tid)
                 else:
@@ -179,22 +179,23 @@ def _bwd(typ, p, l, M, w, tok, rm, rv, tid, P):
     p, l = p.copy(), l.copy()
     g = np.zeros(8)
     if typ == E.OP_MAT1 and (w[0] & E.F_MG):
+        # four sums S = Im <lam|s|x> (s = I, X, Y, Z) over x = psi before the un-apply
         K = w[3]
         U = np.array(M[:4]).reshape(2, 2)
         Ud = U.conj().T
-        mg = np.zeros((2, 2), dtype=complex)
+        paulis = [np.eye(2), np.array([[0, 1], [1, 0]]), np.array([[0, -1j], [1j, 0]]), np.diag([1, -1])]
         for j, j1 in _pairs(K):
             ok = tok & ((j & rm) == rv)
             x = np.stack([p[:, j], p[:, j1]], -1)
             lm = np.stack([l[:, j], l[:, j1]], -1)
+            for q, sg in enumerate(paulis):
+                g[q] += np.imag(np.sum(np.conj(lm[ok]) * (x[ok] @ sg.T)))
             y = x @ Ud.T
-            mg += np.einsum("ta,tb->ab", lm[ok], np.conj(y[ok]))
             ln = lm @ Ud.T
             p[:, j] = np.where(ok, y[:, 0], x[:, 0])
             p[:, j1] = np.where(ok, y[:, 1], x[:, 1])
             l[:, j] = np.where(ok, ln[:, 0], lm[:, 0])
             l[:, j1] = np.where(ok, ln[:, 1], lm[:, 1])
-        g[:8] = np.stack([mg.reshape(-1).real, mg.reshape(-1).imag], -1).reshape(-1)
         return p, l, g
     if typ == E.OP_MAT1:
         K = w[3]
@@ -360,14 +361,17 @@ def adjoint_compiled(c: E.Compiled, pool: np.ndarray, psi, lam):
                 off, bs = c.adj_off[payload]
                 solo_apply(k, psi, pool[off:], n, bs)
                 solo_apply(k, lam, pool[off:], n, bs)
-    # Mg -> parameter gradients (semantics of tqsv_mg_to_grad)
+    # Hermitian-generator sums -> parameter gradients (semantics of tqsv_mg_to_grad)
     for it in c.mg_items:
         for row in range(bsz):
-            mg = grad[row, it["mg_slot"]:it["mg_slot"] + 8]
-            Mg = mg[0::2] + 1j * mg[1::2]
-            D = pool[it["d_off"] + row * it["bstride"]:]
+            sI, sX, sY, sZ = grad[row, it["mg_slot"]:it["mg_slot"] + 4]
+            U = pool[it["u_off"] + row * it["bstride"]:][:4].reshape(2, 2)
             for q in range(it["P"]):
-                grad[row, it["slot"][q]] += np.real(np.sum(D[4 * q:4 * q + 4] * np.conj(Mg)))
+                D = pool[it["u_off"] + row * it["bstride"] + 4 * (q + 1):][:4].reshape(2, 2)
+                H = 1j * D @ U.conj().T
+                a, d = np.real(H[0, 0] + H[1, 1]) / 2, np.real(H[0, 0] - H[1, 1]) / 2
+                b, cy = np.real(H[0, 1] + H[1, 0]) / 2, np.imag(H[1, 0] - H[0, 1]) / 2
+                grad[row, it["slot"][q]] += a * sI + b * sX + cy * sY + d * sZ
     # scatter columns of per-gate ops with non-consecutive slots
     for i, col, sl in c.scatter:
         for q, slot in enumerate(sl):

@@ -20,6 +20,8 @@ from .measurement import (expval, MeasureAll, expval_joint_analytical, expval_jo
                           expval_pauli_sum, MeasureMultiPauliSum, gen_bitstrings, measure, sample_indices,
                           find_observable_groups, expval_joint_sampling, expval_joint_sampling_grouping)
 from .layer import (Op1QAllLayer, Op2QAllLayer, EntanglePairwise, LayerTemplate0, RXYZCXLayer0, RandomLayer)
+from .encoding import (Encoder, GeneralEncoder, PhaseEncoder, MultiPhaseEncoder, StateEncoder, AmplitudeEncoder,
+                       MagnitudeEncoder, encoder_op_list_name_dict)
 from .engine import EngineOptions, wait_for_compiled_stages
 from . import device as _device_mod
 

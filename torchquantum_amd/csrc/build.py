@@ -22,7 +22,7 @@ SOURCES = [
     "tqsv_stage.hip",
     "tqsv_expval.hip",
     "tqsv_adjoint.hip",
-    "tqsv_batch.hip",
+    "tqsv_devgates.hip",
     "tqsv_mlp.hip",
     "tqsv_sample.hip",
     "tqsv_jit.hip",
@@ -52,7 +52,10 @@ def hipcc() -> str:
 def build(verbose: bool = False, jobs: int = 6) -> Path:
     """Compile every source to an object, then link the shared library."""
     cc = hipcc()
-    srcs = [HERE / s for s in SOURCES if (HERE / s).exists()]
+    srcs = [HERE / s for s in SOURCES]
+    missing = [str(p) for p in srcs if not p.exists()]
+    if missing:
+        raise RuntimeError(f"libtqsv.so sources missing: {missing}")
     tmp = Path(tempfile.mkdtemp(prefix="tqsv_build_"))
     try:
         procs = []

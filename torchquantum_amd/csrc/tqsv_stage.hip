@@ -171,15 +171,22 @@ __device__ __forceinline__ void b_mat1(float2 (&p)[NS], float2 (&l)[NS], const f
   }
 }
 
-// psi <- U^dag psi ; Mg += lam conj(psi_prev)^T ; lam <- U^dag lam.  The
-// parameter gradients follow as Re sum_ab D_p[a][b] conj(Mg[a][b]) in a tiny
-// pass afterwards (16 FMA per pair instead of 20 per parameter).
-// NC: the op has no controls (every pair is updated: no selects)
+// Single-wire op with trainable parameters, Hermitian-generator form.
+// For a unitary U(theta), H_p = i (dU/dtheta_p) U^dag is Hermitian, so with
+// x = psi before the un-apply (the op's output) and lam the incoming
+// gradient, dL/dtheta_p = Re <lam| dU_p |U^dag x> = Im <lam| H_p |x>.  Writing
+// H_p = a I + b X + c Y + d Z (real a..d), every parameter's gradient is a
+// combination of the four sums S = Im <lam|s|x>, s = I, X, Y, Z, which this
+// op accumulates (12 FMA per amplitude pair); tqsv_mg_to_grad folds them
+// with the coefficients of H_p afterwards.  Then psi <- U^dag psi and
+// lam <- U^dag lam.  NC: the op has no controls (every pair is updated).
 template <int K, bool NC>
 __device__ __forceinline__ void b_mat1_mg(float2 (&p)[NS], float2 (&l)[NS], const float2* __restrict__ M, bool tok,
                                           int rm, int rv, float (&g)[8]) {
   const float2 u0 = M[0], u1 = M[1], u2 = M[2], u3 = M[3];
-  float2 m00 = make_float2(0.f, 0.f), m01 = m00, m10 = m00, m11 = m00;
+  // a = sum Im conj(l0) x0, b = sum Im conj(l1) x1, c = sum conj(l0) x1, d = sum conj(l1) x0
+  float a = 0.f, b = 0.f;
+  float2 c = make_float2(0.f, 0.f), d = c;
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
     if (j & (1 << K)) continue;
@@ -187,15 +194,15 @@ __device__ __forceinline__ void b_mat1_mg(float2 (&p)[NS], float2 (&l)[NS], cons
     if (!NC && (j & rm) != rv) continue;  // register controls: a wave-uniform skip
     const bool ok = NC || tok;
     const float2 x0 = p[j], x1 = p[j1];
+    const float2 l0 = l[j], l1 = l[j1];
+    const float2 z = make_float2(0.f, 0.f);
+    const float2 g0 = ok ? l0 : z, g1 = ok ? l1 : z;
+    a = fmaf(g0.x, x0.y, fmaf(-g0.y, x0.x, a));
+    b = fmaf(g1.x, x1.y, fmaf(-g1.y, x1.x, b));
+    c = cfmac(g0, x1, c);
+    d = cfmac(g1, x0, d);
     const float2 y0 = cfmac(u2, x1, cmulc(u0, x0));
     const float2 y1 = cfmac(u3, x1, cmulc(u1, x0));
-    const float2 l0 = l[j], l1 = l[j1];
-    // lam_a * conj(y_b) = conj(conj(lam_a) y_b): accumulate conj(y_b) lam_a via cfmac(y, lam)
-    const float2 z = make_float2(0.f, 0.f);
-    m00 = cfmac(ok ? y0 : z, l0, m00);
-    m01 = cfmac(ok ? y1 : z, l0, m01);
-    m10 = cfmac(ok ? y0 : z, l1, m10);
-    m11 = cfmac(ok ? y1 : z, l1, m11);
     const float2 n0 = cfmac(u2, l1, cmulc(u0, l0));
     const float2 n1 = cfmac(u3, l1, cmulc(u1, l0));
     if (NC) {
@@ -210,8 +217,11 @@ __device__ __forceinline__ void b_mat1_mg(float2 (&p)[NS], float2 (&l)[NS], cons
       l[j1] = csel(ok, n1, l1);
     }
   }
-  g[0] += m00.x; g[1] += m00.y; g[2] += m01.x; g[3] += m01.y;
-  g[4] += m10.x; g[5] += m10.y; g[6] += m11.x; g[7] += m11.y;
+  // S_I = Im<l|x>, S_X = Im<l|X|x>, S_Y = Im<l|Y|x> = Re conj(l1) x0 - Re conj(l0) x1, S_Z = Im<l|Z|x>
+  g[0] += a + b;
+  g[1] += c.y + d.y;
+  g[2] += d.x - c.x;
+  g[3] += a - b;
 }
 
 template <int KH, int KL>
@@ -309,7 +319,7 @@ __device__ __forceinline__ void b_diag(float2 (&p)[NS], float2 (&l)[NS], const f
 // Op flag bits (word 0): the compiler sets them from the gate structure.
 enum OpFlag {
   F_BATCHED = 1 << 8, F_NOCTRL = 1 << 9, F_REAL = 1 << 10, F_XPERM = 1 << 11, F_SIGN = 1 << 12,
-  F_MG = 1 << 13   // adjoint: accumulate the 2x2 matrix gradient sum(lam conj(psi)^T) (8 floats)
+  F_MG = 1 << 13   // adjoint: accumulate the four Hermitian-generator sums Im<lam|s|psi>, s = I, X, Y, Z
 };
 
 // no controls: every pair is updated, no selects
@@ -699,7 +709,7 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
       float gq[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       bwd_op<FEAT>(p, l, w, M, tid, gq);
       if (w[7] >= 0) {
-        const int P = (w[0] & F_MG) ? 8 : ((w[0] >> 4) & 15);
+        const int P = (w[0] & F_MG) ? 4 : ((w[0] >> 4) & 15);
         if (wwidth == 64) {
           // halving exchange: every lane ends with one value's wave total
           if (P > 4) {
@@ -872,12 +882,14 @@ __global__ void k_batch_zsum(const ZCirc* __restrict__ circ, int n_circ, const f
   out[c] = (float)s;
 }
 
-// Mg -> parameter gradients: grad[row, slot_p] += Re sum_ab D_p[a][b] conj(Mg[a][b])
+// Hermitian-generator sums -> parameter gradients: for each parameter p of an
+// op accumulated with F_MG, H_p = i D_p U^dag = a I + b X + c Y + d Z and
+// grad[row, slot_p] += a S_I + b S_X + c S_Y + d S_Z.
 struct MgItem {
-  int32_t mg_slot;     // first of 8 Mg floats in the grad row
+  int32_t mg_slot;     // first of the 4 sums (S_I, S_X, S_Y, S_Z) in the grad row
   int32_t P;
   int32_t slot[4];     // parameter slots
-  int64_t d_off;       // pool offset (complex) of D_0 for row 0
+  int64_t u_off;       // pool offset (complex) of U for row 0; D_p follows at u_off + 4 (p + 1)
   int64_t bstride;     // pool stride between rows (0 = shared)
 };
 
@@ -888,14 +900,21 @@ __global__ void k_mg_to_grad(float* __restrict__ grad, int64_t gstride, int64_t 
   const int64_t row = t / n_items;
   const MgItem it = items[t % n_items];
   float* gr = grad + row * gstride;
-  const float* mg = gr + it.mg_slot;
-  const float2 M[4] = {make_float2(mg[0], mg[1]), make_float2(mg[2], mg[3]), make_float2(mg[4], mg[5]),
-                       make_float2(mg[6], mg[7])};
-  const float2* D = mats + it.d_off + row * it.bstride;
+  const float sI = gr[it.mg_slot], sX = gr[it.mg_slot + 1], sY = gr[it.mg_slot + 2], sZ = gr[it.mg_slot + 3];
+  const float2* U = mats + it.u_off + row * it.bstride;
   for (int q = 0; q < it.P && q < 4; ++q) {
-    float s = 0.f;
-    for (int e = 0; e < 4; ++e) s += redot(M[e], D[4 * q + e]);   // Re(D conj(Mg)) = Re(conj(Mg) D)
-    gr[it.slot[q]] += s;
+    const float2* D = U + 4 * (q + 1);
+    // G = D U^dag: G[r][c] = sum_k D[r][k] conj(U[c][k]);  H = i G
+    float2 G[4];
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) G[2 * r + c] = cfmac(U[2 * c + 1], D[2 * r + 1], cmulc(U[2 * c], D[2 * r]));
+    // H = i G: H[r][c] = (-G.y, G.x).  a = Re(H00 + H11)/2, d = Re(H00 - H11)/2,
+    // b = Re(H01 + H10)/2, c = Im(H10 - H01)/2
+    const float h00 = -G[0].y, h11 = -G[3].y;
+    const float ca = 0.5f * (h00 + h11), cd = 0.5f * (h00 - h11);
+    const float cb = 0.5f * (-G[1].y - G[2].y);
+    const float cy = 0.5f * (G[2].x - G[1].x);
+    gr[it.slot[q]] += ca * sI + cb * sX + cy * sY + cd * sZ;
   }
 }
 

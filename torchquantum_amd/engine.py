@@ -43,7 +43,14 @@ MAX_P_PER_OP = 4
 H_NOPS, H_M, H_T, H_NNL, H_OFF_OPS, H_NGP, H_OFF_GSLOT, H_FLAGS, H_POS = range(9)
 OP_MAT1, OP_MAT2, OP_DIAG, OP_MATK, OP_REMAP = 1, 2, 3, 4, 5
 F_BATCHED, F_NOCTRL, F_REAL, F_XPERM, F_SIGN, F_MG = 1 << 8, 1 << 9, 1 << 10, 1 << 11, 1 << 12, 1 << 13
-MG_ITEM = np.dtype([("mg_slot", "<i4"), ("P", "<i4"), ("slot", "<i4", (4,)), ("d_off", "<i8"), ("bstride", "<i8")])
+MG_ITEM = np.dtype([("mg_slot", "<i4"), ("P", "<i4"), ("slot", "<i4", (4,)), ("u_off", "<i8"), ("bstride", "<i8")])
+MG_SUMS = 4     # Hermitian-generator sums S_I, S_X, S_Y, S_Z per F_MG op
+# gates whose per-row matrices the GPU builds from device parameters (tqsv_devgates.hip)
+DEV_GATE_CODE = {"rx": 1, "ry": 2, "rz": 3, "phaseshift": 4, "u1": 4, "u2": 5, "u3": 6}
+DGF_INVERSE, DGF_CONJ, DGF_DERIV, DGF_DIAG = 1, 2, 4, 8
+DEV_GATE = np.dtype([("params", "<u8"), ("rstride", "<i8"), ("cstride", "<i8"), ("dst", "<i8"), ("bstride", "<i8"),
+                     ("adj", "<i8"), ("abstride", "<i8"), ("code", "<i4"), ("flags", "<i4"), ("rows", "<i4"),
+                     ("diag_ab", "<i4")])
 FEAT_MAT2, FEAT_MATK = 1, 2
 
 
@@ -65,8 +72,16 @@ class EngineOptions:
                  Blocks of <= 4 wires run inside stages; 5- and 6-wire blocks
                  run as their own launches on the matrix cores.
     grid_cap:    workgroups per stage launch (0 = auto).
-    adjoint_mg:  adjoint sweep accumulates 2x2 matrix gradients for ops with
-                 >= 2 trainable parameters (else one inner product per parameter).
+    adjoint_mg:  adjoint sweep accumulates, for trainable single-wire ops in
+                 stages, the four sums Im<lam|s|psi> (s = I, X, Y, Z) that
+                 carry every parameter's gradient through the Hermitian
+                 generator i dU/dtheta U^dagger (else one inner product per
+                 parameter).
+    device_gates: per-row (batched) parameters of RX, RY, RZ, PhaseShift/U1,
+                 U2 and U3 that live on the GPU (encoder inputs) have their
+                 matrices built by a kernel straight into the matrix pool: no
+                 device-to-host copy of the inputs and no host work per row.
+                 Such gates are not merged or fused with neighbours.
     jit:         "auto" | "on" | "off": run stages as kernels compiled at run
                  time from their programs (straight-line code, hipRTC) instead
                  of the program-interpreting kernels.  "auto" compiles for
@@ -84,14 +99,43 @@ class EngineOptions:
     grid_cap: int = 0
     adjoint_mg: bool = True
     jit: str = "auto"
+    device_gates: bool = True
 
     def key(self):
         return (self.mode, self.stage_bits, self.low_bits, self.merge_1q, self.fusion_width, self.grid_cap,
-                self.adjoint_mg, self.jit)
+                self.adjoint_mg, self.jit, self.device_gates)
 
 
 DEFAULT_OPTIONS = EngineOptions()
 JIT_MIN_AMPS = 1 << 22
+
+# Optional per-launch timing of stage kernels: when set to a list, every
+# stage launch appends (direction "f"/"a", compiled?, start, end), two HIP
+# events recorded on the launch stream around it.
+_stage_timer: Optional[list] = None
+
+
+def set_stage_timer(records: Optional[list]) -> None:
+    """Record HIP events around every stage launch into ``records`` (None
+    switches it off).  Used to time the stage kernels inside a timed loop."""
+    global _stage_timer
+    _stage_timer = records
+
+
+def _timed(direction: str, compiled: bool):
+    """Start event (or None) for a stage launch when the stage timer is on."""
+    if _stage_timer is None:
+        return None
+    e0 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    return (direction, compiled, e0)
+
+
+def _timed_end(tok):
+    if tok is not None and _stage_timer is not None:
+        e1 = torch.cuda.Event(enable_timing=True)
+        e1.record()
+        _stage_timer.append((tok[0], tok[1], tok[2], e1))
 
 
 def wait_for_compiled_stages(timeout: Optional[float] = None) -> bool:
@@ -154,6 +198,7 @@ class KOp:
     real: bool = False              # matrix entries are real for every parameter value
     xperm: bool = False             # the 2x2 is Pauli X (a controlled exchange)
     sign: bool = False              # diagonal phase -1 where the controls match, no parity
+    dev: int = 0                    # > 0: matrix built on the GPU from device parameters (DEV_GATE_CODE)
 
     @property
     def cmask(self) -> int:
@@ -194,7 +239,18 @@ SIGN_GATES = {"pauliz", "cz", "ccz"}
 REAL_1Q_BLOCK = X_GATES | {"ry", "hadamard", "shadamard", "cry", "chadamard", "pauliz", "i"}
 
 
-def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]) -> List[KOp]:
+def device_gate_code(op: TapeOp, bsz: int) -> int:
+    """DEV_GATE_CODE of a tape op whose per-row matrices the GPU builds (0: host)."""
+    p = op.params
+    if p is None or len(op.wires) != 1 or bsz < 2 or p.dim() != 2 or p.shape[0] != bsz:
+        return 0
+    if not p.is_cuda or p.dtype != torch.float32:
+        return 0
+    return DEV_GATE_CODE.get(op.name, 0)
+
+
+def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int],
+          device_gates: bool = False) -> List[KOp]:
     """Tape ops -> kernel ops (no fusion).  Identity gates vanish."""
     out: List[KOp] = []
     for ti, op in enumerate(tape):
@@ -205,6 +261,7 @@ def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]
         if rows not in (1, bsz):
             raise ValueError(f"gate '{op.name}': parameter batch {rows} does not match bsz {bsz}")
         batched = rows > 1
+        dev = device_gate_code(op, bsz) if device_gates else 0
         slots = ()
         if op.wants_grad():
             slots = tuple(param_offsets[ti] + p for p in range(op.n_params()))
@@ -215,7 +272,7 @@ def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]
             out.append(KOp("diag", tuple(pos[i] for i in pidx), tuple(pos[i] for i in cidx),
                            tuple(1 for _ in cidx), (ti,), "diag",
                            G.diag_phase_indices(cidx, pidx, k), slots, batched,
-                           sign=s.name in SIGN_GATES, real=s.name in SIGN_GATES))
+                           sign=s.name in SIGN_GATES, real=s.name in SIGN_GATES, dev=dev))
         elif s.kind == "ctrl1":
             ci, cv, t = G.ctrl_values(s, k)
             out.append(KOp("mat1", (pos[t],), tuple(pos[i] for i in ci), tuple(cv), (ti,),
@@ -230,7 +287,7 @@ def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]
                                           "(qubitunitary*) are not supported")
             kind = "mat1" if k == 1 else ("mat2" if k == 2 else "matk")
             out.append(KOp(kind, tuple(pos), (), (), (ti,), "full", (0, 0), slots, batched,
-                           real=k == 1 and s.name in REAL_1Q_BLOCK, xperm=k == 1 and s.name in X_GATES))
+                           real=k == 1 and s.name in REAL_1Q_BLOCK, xperm=k == 1 and s.name in X_GATES, dev=dev))
     return out
 
 
@@ -244,7 +301,7 @@ def merge_single_wire(kops: List[KOp], tape: Sequence[TapeOp]) -> List[KOp]:
     pending: Dict[int, int] = {}   # bit -> index in out of the open run
 
     def single(k: KOp) -> bool:
-        if k.cbits:
+        if k.cbits or k.dev:
             return False
         if k.kind == "mat1":
             return True
@@ -301,7 +358,7 @@ def fuse_blocks(kops: List[KOp], tape: Sequence[TapeOp], n: int, width: int) -> 
 
     for i, k in enumerate(kops):
         bs = set(k.allbits)
-        if len(k.src) != 1 or (k.kind == "matk" and len(k.bits) > R_BITS) or len(bs) > width:
+        if k.dev or len(k.src) != 1 or (k.kind == "matk" and len(k.bits) > R_BITS) or len(bs) > width:
             place(dict(bits=bs, members=[i], solo=True, P=k.P, closed=False))
             continue
         cands = sorted({last[b] for b in bs if b in last})
@@ -360,8 +417,9 @@ class _GVLayout:
     (gate, width, has-params), each op's group key, first row in its group,
     row count and whether its derivatives are wanted."""
 
-    def __init__(self, tape: Sequence[TapeOp]):
+    def __init__(self, tape: Sequence[TapeOp], skip: frozenset = frozenset()):
         T = len(tape)
+        self.skip = skip        # tape ops whose matrices the GPU builds: no host values
         self.gkey: List[Optional[tuple]] = [None] * T
         self.grow = np.zeros(T, dtype=np.int64)
         self.rows = np.ones(T, dtype=np.int64)
@@ -369,6 +427,8 @@ class _GVLayout:
         self.mat_ops: List[int] = []
         groups: Dict[tuple, List[int]] = {}
         for i, op in enumerate(tape):
+            if i in skip:
+                continue
             if op.is_matrix_param:
                 self.mat_ops.append(i)
                 self.gkey[i] = ("__mat__", i)
@@ -445,7 +505,7 @@ class GateValues:
         self.DG: Dict[tuple, Optional[np.ndarray]] = {}
         self.U = _GroupViews(self, False)
         self.D = _GroupViews(self, True)
-        host = _host_params(tape)
+        host = _host_params(tape, lay.skip)
         for i in lay.mat_ops:
             op = tape[i]
             m = host[i].astype(np.complex128)
@@ -596,10 +656,11 @@ def _upload(host: np.ndarray, device: torch.device) -> torch.Tensor:
     return ring.upload(host, device)
 
 
-def _host_params(tape: Sequence[TapeOp]) -> List[Optional[np.ndarray]]:
-    """Parameter values of every tape op as numpy (float64 or complex128).
-    GPU tensors without a valid host copy are gathered with one
-    concatenation and one device-to-host copy."""
+def _host_params(tape: Sequence[TapeOp], skip: frozenset = frozenset()) -> List[Optional[np.ndarray]]:
+    """Parameter values of every tape op as numpy (float64 or complex128),
+    except the ops in ``skip`` (their matrices are built on the GPU).  GPU
+    tensors without a valid host copy are gathered with one concatenation
+    and one device-to-host copy."""
     out: List[Optional[np.ndarray]] = [None] * len(tape)
     dev: Dict[tuple, List[int]] = {}
     cpu_real: List[int] = []
@@ -607,7 +668,7 @@ def _host_params(tape: Sequence[TapeOp]) -> List[Optional[np.ndarray]]:
     with _MIRROR._lock:    # one acquisition for the whole tape (reentrant)
         for i, op in enumerate(tape):
             p = op.params
-            if p is None:
+            if p is None or i in skip:
                 continue
             if not p.is_cuda:
                 if p.is_complex():
@@ -1040,7 +1101,7 @@ class Compiled:
         # (both fixed by the structure key, which holds shapes and requires_grad)
         self.param_idx = [i for i, op in enumerate(tape) if op.params is not None and not op.is_matrix_param]
         self.any_param_grad = any(tape[i].params.requires_grad for i in self.param_idx)
-        kops = lower(tape, n, bsz, self.param_offsets)
+        kops = lower(tape, n, bsz, self.param_offsets, opts.device_gates)
         if opts.mode == "fused" and opts.fusion_width >= 2:
             kops = fuse_blocks(kops, tape, n, opts.fusion_width)
             for k in kops:
@@ -1068,6 +1129,24 @@ class Compiled:
                 self.adj_off[i] = (pos, per if k.batched else 0)
                 pos += per * rows
         self.pool_len = max(pos, 1)
+        # gates whose matrices the GPU builds (device parameters): one DEV_GATE
+        # record each, appended to the uploaded pool; the parameter pointers
+        # and strides are filled per flush
+        dev = [i for i, k in enumerate(kops) if k.dev]
+        self.dev_kops = dev
+        self.dev_src = [kops[i].src[0] for i in dev]
+        self.dev_tape = frozenset(self.dev_src)
+        self.dev_items = np.zeros(len(dev), dtype=DEV_GATE)
+        for j, i in enumerate(dev):
+            k, op = kops[i], tape[kops[i].src[0]]
+            self.dev_items["dst"][j], self.dev_items["bstride"][j] = self.data_off[i]
+            self.dev_items["adj"][j], self.dev_items["abstride"][j] = self.adj_off.get(i, (-1, 0))
+            self.dev_items["code"][j] = k.dev
+            self.dev_items["flags"][j] = ((DGF_INVERSE if op.inverse else 0) | (DGF_CONJ if op.conj else 0)
+                                          | (DGF_DIAG if k.kind == "diag" else 0))
+            self.dev_items["rows"][j] = bsz
+            self.dev_items["diag_ab"][j] = (k.diag_idx[0] & 1) | ((k.diag_idx[1] & 1) << 1)
+        self.dev_tail = (len(dev) * DEV_GATE.itemsize + 7) // 8        # complex64 elements
         self.steps: List[tuple] = []
         self.perms: Dict[int, np.ndarray] = {}
         self._fill = None
@@ -1079,19 +1158,20 @@ class Compiled:
         if opts.mode == "fused" and n >= R_BITS:
             m = choose_stage_bits(n, opts)
             slot_of = {i: list(k.grad_slots) for i, k in enumerate(kops)}
-            # ops with >= 2 trainable parameters accumulate a 2x2 matrix gradient
-            # (8 floats past the parameter columns) that one small kernel folds
-            # into their parameter slots after the sweep
+            # trainable single-wire ops accumulate the four sums S = Im<lam|s|psi>
+            # (s = I, X, Y, Z; past the parameter columns) instead of one
+            # inner product per parameter; one small kernel folds them with
+            # the coefficients of H_p = i dU/dtheta_p U^dagger after the sweep
             mg_slot_of = {}
             items = []
             for i, k in enumerate(kops):
-                if opts.adjoint_mg and k.kind == "mat1" and k.P >= 2 and not k.xperm:
-                    base = self.n_slots + 8 * len(items)
-                    mg_slot_of[i] = list(range(base, base + 8))
+                if opts.adjoint_mg and k.kind == "mat1" and k.P >= 1 and not k.xperm:
+                    base = self.n_slots + MG_SUMS * len(items)
+                    mg_slot_of[i] = list(range(base, base + MG_SUMS))
                     sl = list(k.grad_slots) + [0] * (4 - k.P)
-                    items.append((base, k.P, sl, self.data_off[i][0] + 4, self.data_off[i][1]))
+                    items.append((base, k.P, sl, self.data_off[i][0], self.data_off[i][1]))
             self.mg_items = np.array(items, dtype=MG_ITEM) if items else np.zeros(0, dtype=MG_ITEM)
-            self.grad_width = max(self.n_slots + 8 * len(items), 1)
+            self.grad_width = max(self.n_slots + MG_SUMS * len(items), 1)
             words = []
             base = 0
             for kind, payload in partition(kops, n, m, opts.low_bits):
@@ -1193,14 +1273,29 @@ class Compiled:
         host = self.pool_host(tape, with_grad, with_adj)
         if torch.device(self.device).type != "cuda":
             return torch.from_numpy(host).to(self.device)
-        return _upload(host, torch.device(self.device))
+        pool = _upload(host, torch.device(self.device))
+        if len(self.dev_items):
+            N.call("tqsv_build_dev_gates", N.stream_of(pool), pool.data_ptr(), pool.data_ptr() + 8 * self.pool_len,
+                   len(self.dev_items), self.bsz)
+        return pool
 
     def pool_host(self, tape: Sequence[TapeOp], with_grad: bool, with_adj: bool) -> np.ndarray:
         """The matrix pool (complex64) for the current parameter values."""
         if self._gv_layout is None:
-            self._gv_layout = _GVLayout(tape)
+            self._gv_layout = _GVLayout(tape, self.dev_tape)
         gv = GateValues(tape, with_grad, self._gv_layout)
-        host = np.zeros(self.pool_len, dtype=np.complex64)
+        host = np.zeros(self.pool_len + self.dev_tail, dtype=np.complex64)
+        if len(self.dev_items):
+            items = self.dev_items.copy()
+            for j, ti in enumerate(self.dev_src):
+                p = tape[ti].params
+                if not p.is_cuda or p.dtype != torch.float32 or p.dim() != 2 or p.shape[0] != self.bsz:
+                    raise RuntimeError("device-built gate: its parameter tensor changed form after planning")
+                items["params"][j] = p.data_ptr()
+                items["rstride"][j], items["cstride"][j] = p.stride(0), p.stride(1)
+                if with_grad and self.kops[self.dev_kops[j]].P:
+                    items["flags"][j] |= DGF_DERIV
+            host[self.pool_len:].view(np.uint8)[:items.nbytes] = items.view(np.uint8)
         if self._fill is None:
             self._fill = self._build_fill(tape)
         singles, chains, others = self._fill
@@ -1269,6 +1364,8 @@ class Compiled:
         for i, op in enumerate(tape):
             gk[i] = ("__mat__", i) if op.is_matrix_param else (op.name, len(op.wires), op.n_params() > 0)
         for i, k in enumerate(self.kops):
+            if k.dev:
+                continue                      # written on the GPU (tqsv_build_dev_gates)
             aoff = self.adj_off.get(i, (None, 0))[0]
             if k.batched or k.kind == "swap" or k.extract == "block" or any(tape[t].is_matrix_param for t in k.src):
                 others.append(i)
@@ -1344,7 +1441,7 @@ def structure_key(tape: Sequence[TapeOp], n: int, bsz: int, opts: EngineOptions,
         elif p.is_complex():
             ps = ("m", p.shape)                 # torch.Size: a tuple (hash and equality)
         else:
-            ps = (p.shape, p.requires_grad)
+            ps = (p.shape, p.requires_grad, p.is_cuda, p.dtype)
         add((op.name, op.wires, op.inverse, op.conj, ps))
     return (n, bsz, opts.key(), str(device), tuple(items))
 
@@ -1427,6 +1524,7 @@ def forward(c: Compiled, state_in: torch.Tensor, pool: torch.Tensor, init_zero: 
             src = out if fresh else cur_in
             hp = c.prog_host[base:]
             f = jit.get(base) if jit else None
+            tok = _timed("f", f is not None)
             if f is not None:
                 N.call("tqsv_jit_stage_forward", stream, f, out.data_ptr(),
                        (src.data_ptr() if src is not None else None), bsz, n, hp.ctypes.data_as(N.c_int_p),
@@ -1435,6 +1533,7 @@ def forward(c: Compiled, state_in: torch.Tensor, pool: torch.Tensor, init_zero: 
                 N.call("tqsv_stage_forward", stream, out.data_ptr(), (src.data_ptr() if src is not None else None),
                        bsz, n, hp.ctypes.data_as(N.c_int_p), c.prog_dev.data_ptr() + 4 * base, pool.data_ptr(),
                        1 if (init_zero and not fresh) else 0, c.opts.grid_cap)
+            _timed_end(tok)
             fresh = True
         else:
             if not fresh:
@@ -1476,6 +1575,7 @@ def adjoint(c: Compiled, psi_src: torch.Tensor, lam_src: torch.Tensor, pool: tor
             base, stg = payload
             hp = c.prog_host[base:]
             f = jit.get(base) if jit else None
+            tok = _timed("a", f is not None)
             if f is not None:
                 pin, lin = first_src if first_src is not None else (None, None)
                 first_src = None
@@ -1485,6 +1585,7 @@ def adjoint(c: Compiled, psi_src: torch.Tensor, lam_src: torch.Tensor, pool: tor
                 N.call("tqsv_stage_adjoint", stream, psi.data_ptr(), lam.data_ptr(), bsz, n,
                        hp.ctypes.data_as(N.c_int_p), c.prog_dev.data_ptr() + 4 * base, pool.data_ptr(),
                        N.fptr(grad), gstride, c.opts.grid_cap)
+            _timed_end(tok)
             continue
         i = payload
         k = c.kops[i]

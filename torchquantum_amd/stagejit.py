@@ -433,32 +433,35 @@ class _Gen:
         return self.lane_gate(tk, g)
 
     def b_mat1_mg(self, w, p: List[str], l: List[str]) -> List[str]:
+        """Hermitian-generator form (see k_mg_to_grad): the four sums
+        S = Im<lam|s|x>, s = I, X, Y, Z, over x = psi before the un-apply,
+        12 FMA per pair, then psi <- U^dag psi, lam <- U^dag lam."""
         K = w[3]
         nc = bool(w[0] & F_NOCTRL)
         rm, rv = (0, 0) if nc else self.reg_ctrl(w)
         M = self.mat_base(w)
         tk = None if nc else self.thread_ok(w)
         u = self.select_identity(tk, [self.var(f"{M}[{i}]") for i in range(4)], 2)
-        m = [None] * 4     # m00, m01, m10, m11
+        a = b = c = d = None
         for j in range(NS):
             if j & (1 << K) or (j & rm) != rv:
                 continue
             j1 = j | (1 << K)
             x0, x1 = p[j], p[j1]
+            l0, l1 = l[j], l[j1]
+            # a += Im conj(l0) x0, b += Im conj(l1) x1, c += conj(l0) x1, d += conj(l1) x0
+            a = self.var(f"fmaf({l0}.x, {x0}.y, fmaf(-{l0}.y, {x0}.x, {a if a else '0.f'}))", "float")
+            b = self.var(f"fmaf({l1}.x, {x1}.y, fmaf(-{l1}.y, {x1}.x, {b if b else '0.f'}))", "float")
+            c = self.var(f"cfmac({l0}, {x1}, {c if c else ZERO})")
+            d = self.var(f"cfmac({l1}, {x0}, {d if d else ZERO})")
             y0 = self.var(f"cfmac({u[2]}, {x1}, cmulc({u[0]}, {x0}))")
             y1 = self.var(f"cfmac({u[3]}, {x1}, cmulc({u[1]}, {x0}))")
-            l0, l1 = l[j], l[j1]
-            for idx, (yy, ll) in enumerate(((y0, l0), (y1, l0), (y0, l1), (y1, l1))):
-                m[idx] = self.var(f"cfmac({yy}, {ll}, {m[idx] if m[idx] else ZERO})")
             n0 = self.var(f"cfmac({u[2]}, {l1}, cmulc({u[0]}, {l0}))")
             n1 = self.var(f"cfmac({u[3]}, {l1}, cmulc({u[1]}, {l0}))")
             p[j], p[j1], l[j], l[j1] = y0, y1, n0, n1
-        g: List[Optional[str]] = []
-        for x in m:
-            if x is None:
-                g += [None, None]
-            else:
-                g += [f"{x}.x", f"{x}.y"]
+        if a is None:
+            return self.lane_gate(tk, [None] * 4)
+        g = [f"{a} + {b}", f"{c}.y + {d}.y", f"{d}.x - {c}.x", f"{a} - {b}"]
         return self.lane_gate(tk, g)
 
     def b_mat2(self, w, p: List[str], l: List[str], P: int) -> List[str]:
@@ -559,7 +562,7 @@ class _Gen:
         l >> 3 (8 values), l >> 4 (4), l >> 5 (2) or of the only value --
         the assignment of the interpreter's halving exchange -- built from
         permlane swaps and DPP, with no LDS traffic and no selects."""
-        Pp = 8 if (w[0] & F_MG) else ((w[0] >> 4) & 15)
+        Pp = 4 if (w[0] & F_MG) else ((w[0] >> 4) & 15)
         if Pp > 4:
             nv, sh, cond = 8, 3, "(lane & 7) == 0"
         elif Pp > 2:
@@ -853,10 +856,15 @@ def get_kernels(programs: Sequence[np.ndarray], kind: str, device_index: int) ->
         items = list(missing.items())
         chunks = [items[i:i + KERNELS_PER_SOURCE] for i in range(0, len(items), KERNELS_PER_SOURCE)]
         codes = []
-        for ch in chunks:
+        verbose = os.environ.get("TQSV_JIT_LOG") == "1"
+        for ci, ch in enumerate(chunks):
             if _shutdown.is_set():
                 raise RuntimeError("stage kernel compile cancelled at interpreter exit")
             codes.append(compile_source(PRELUDE + "\n".join(gen(w, nm) for nm, w in ch), COMPILE_OPTS[kind]))
+            if verbose:
+                import sys
+                print(f"[stagejit] {kind}: {min((ci + 1) * KERNELS_PER_SOURCE, len(items))}/{len(items)} stage "
+                      "kernels compiled", file=sys.stderr, flush=True)
         lib = N.lib()
         with _lock:
             for ch, code in zip(chunks, codes):
