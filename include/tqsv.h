@@ -150,6 +150,15 @@ int64_t tqsv_expval_workspace_bytes(int64_t bsz, int n, int nterms);
  * PauliZ (torchquantum/measurement/measurements.py:282-343). */
 int tqsv_expval_z_all(tqsv_stream_t s, const void* state, int64_t bsz, int n, float* out, void* workspace,
                       int64_t ws_bytes);
+/* <Z_w> for every wire, as tqsv_expval_z_all, allocating nothing and taking
+ * no workspace: the chunk partials go to a fixed device array of this
+ * library (4 MiB per GPU, held until the process ends; shapes needing more
+ * return -3).  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Calls that overlap on
+ * one GPU share the array and race: serialise them.  The stream must belong
+ * to the current device. */
+int tqsv_expval_z_all_noalloc(tqsv_stream_t s, const void* state, int64_t bsz, int n, float* out);
 /* Re<psi|P_t|psi> for Pauli strings given by host masks (x: X/Y positions,
  * z: Z/Y positions): out[row * out_stride + t].  Replaces
  * expval_joint_analytical (measurements.py:232-279). */

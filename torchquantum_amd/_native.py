@@ -45,6 +45,7 @@ SIGNATURES = {
                                   c_void_p, c_int64, c_int, c_float_p, c_int64, c_int]),
     "tqsv_expval_workspace_bytes": (c_int64, [c_int64, c_int, c_int]),
     "tqsv_expval_z_all": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_float_p, c_void_p, c_int64]),
+    "tqsv_expval_z_all_noalloc": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_float_p]),
     "tqsv_expval_pauli": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_u64_p, c_u64_p, c_float_p, c_int64,
                                   c_void_p, c_int64]),
     "tqsv_pauli_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int, c_u64_p, c_u64_p, c_void_p,

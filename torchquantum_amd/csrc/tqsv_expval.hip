@@ -12,6 +12,7 @@
 // waves -> one partial per (row, chunk, term) -> a second tiny kernel sums
 // the chunks in float64.  No float atomics, so results are deterministic.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "tqsv_common.h"
@@ -585,6 +586,42 @@ static unsigned grid_cap(uint64_t work) {
 }
 
 // Bytes of float workspace the expectation entry points need.
+// ---- allocation-free <Z_w> -------------------------------------------------
+// tqsv_expval_z_all_noalloc runs the same two kernels as tqsv_expval_z_all,
+// with the chunk partials in a fixed device array instead of a caller
+// workspace, for callers that cannot allocate device memory.  Each GPU holds
+// its own copy of the array until the process ends.  Every partial that
+// k_ez_final reads is written by k_ez_part in the same call, so nothing needs
+// clearing between calls.  Calls that overlap on one GPU share the array and
+// race; callers serialise them (one stream, or their own ordering).
+constexpr int64_t NOALLOC_SCRATCH_FLOATS = 1ll << 20;   // 4 MiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) float scratch[NOALLOC_SCRATCH_FLOATS];
+static std::mutex g_scratch_mu;
+static float* g_scratch_dev[NOALLOC_MAX_DEVICES];      // device address of `scratch`, per device
+
+// Device address of this GPU's `scratch`, looked up once per device (with
+// that device current); host code uses only this address, never the symbol.
+static int scratch_address(float** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    set_error("tqsv_expval_z_all_noalloc: no current HIP device (or device index >= %d)", NOALLOC_MAX_DEVICES);
+    return E_LAUNCH;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("tqsv_expval_z_all_noalloc: hipGetSymbolAddress failed on device %d: %s", dev, hipGetErrorString(e));
+      return E_LAUNCH;
+    }
+    g_scratch_dev[dev] = (float*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return OK;
+}
+
 TQSV_API int64_t tqsv_expval_workspace_bytes(int64_t bsz, int n, int nterms) {
   if (bsz < 1 || n < 1 || n > 32) return -1;
   const int lg = ev_chunk_log2(bsz, n);
@@ -610,6 +647,31 @@ TQSV_API int tqsv_expval_z_all(hipStream_t s, const void* state, int64_t bsz, in
   const int64_t outs = bsz * n;
   k_ez_final<<<sum_grid(outs), 256, 0, s>>>((const float*)workspace, n, 1 << (n - lg), bsz, out);
   return check_launch("tqsv_expval_z_all/final");
+}
+
+// <Z_w> for every wire, as tqsv_expval_z_all, with no workspace argument and
+// no allocation: the partials go to this GPU's fixed scratch array
+// (NOALLOC_SCRATCH_FLOATS floats; larger shapes are rejected).
+TQSV_API int tqsv_expval_z_all_noalloc(hipStream_t s, const void* state, int64_t bsz, int n, float* out) {
+  if (!state || !out) { set_error("tqsv_expval_z_all_noalloc: null pointer"); return E_ARG; }
+  if (bsz < 1 || n < 1 || n > EV_MAXQ) { set_error("tqsv_expval_z_all_noalloc: need 1 <= n <= %d", EV_MAXQ); return E_ARG; }
+  if ((uintptr_t)state & 15) { set_error("tqsv_expval_z_all_noalloc: state must be 16-byte aligned"); return E_ARG; }
+  const int lg = ev_chunk_log2(bsz, n);
+  const int64_t blocks = bsz << (n - lg);
+  const int64_t need = blocks * (n + 1);
+  if (need > NOALLOC_SCRATCH_FLOATS) {
+    set_error("tqsv_expval_z_all_noalloc: bsz %lld x %d qubits needs %lld floats of scratch, the maximum is %lld "
+              "(use tqsv_expval_z_all with a workspace)", (long long)bsz, n, (long long)need,
+              (long long)NOALLOC_SCRATCH_FLOATS);
+    return E_LIMIT;
+  }
+  float* ws = nullptr;
+  int rc = scratch_address(&ws);
+  if (rc) return rc;
+  k_ez_part<<<(unsigned)blocks, EV_THREADS, 0, s>>>((const float4*)state, n, lg, ws);
+  if ((rc = check_launch("tqsv_expval_z_all_noalloc/part"))) return rc;
+  k_ez_final<<<sum_grid(bsz * n), 256, 0, s>>>(ws, n, 1 << (n - lg), bsz, out);
+  return check_launch("tqsv_expval_z_all_noalloc/final");
 }
 
 // Re <psi|P_t|psi> for nterms Pauli strings given by bit masks (in-row bit
