@@ -132,16 +132,20 @@ int tqsv_adjoint_diag(tqsv_stream_t s, void* psi, void* lam, int64_t bsz, int n,
                       int64_t grad_stride, int gslot);
 
 /* ---- gate matrices from device parameters ------------------------------- */
-/* sizeof the DevGate record (layout checked by the host). */
+/* sizeof(DevOp) | sizeof(DevMember) << 16 (layouts checked by the host). */
 int tqsv_dev_gate_bytes(void);
-/* Write the matrix-pool blocks (U, dU/dtheta_p, U^dagger) of single-wire
- * rotations (RX, RY, RZ, PhaseShift/U1, U2, U3) from per-row parameters held
- * in device memory; items_dev: n_items DevGate records in device memory;
- * max_rows: the largest row count.  Replaces the per-call matrix build of the
- * reference's functional gates on batched parameters (rx.py:19-46,
- * ry.py:50-67, rz.py:150-171, u3.py:52-77 via gate_wrapper.py:373-376), as
- * fed by encoders (encoding/encodings.py:91-102). */
-int tqsv_build_dev_gates(tqsv_stream_t s, void* pool, const void* items_dev, int n_items, int64_t max_rows);
+/* Write the matrix-pool blocks (U, dU/dtheta_p, U^dagger) of single-wire ops
+ * -- one gate or a merged run of single-wire gates (RX, RY, RZ,
+ * PhaseShift/U1, U2, U3, Rot with shared [1, P] or per-row [bsz, P]
+ * parameters in device memory; fixed H, SH, X, Y, SX, SXDG) -- from their
+ * parameters on the GPU.  ops_dev / members_dev: n_ops DevOp records and
+ * their DevMember records in device memory; max_rows: the largest row
+ * count.  Replaces the per-call matrix build of the reference's functional
+ * gates (rx.py:19-46, ry.py:50-67, rz.py:150-171, u3.py:52-77 via
+ * gate_wrapper.py:361-425), for trainable Operator parameters and for
+ * per-row encoder inputs (encoding/encodings.py:91-102). */
+int tqsv_build_dev_gates(tqsv_stream_t s, void* pool, const void* ops_dev, const void* members_dev, int n_ops,
+                         int64_t max_rows);
 
 /* ---- expectations ------------------------------------------------------ */
 /* bytes of float workspace needed by the calls below (-1: unsupported n). */

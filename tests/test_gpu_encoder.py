@@ -47,7 +47,8 @@ def test_device_built_gates_vs_oracle(mode):
         tqf.cnot(q, wires=[int(rng.integers(n // 2)), n - 1 - int(rng.integers(n // 2))])
         ops.append(dict(name="cnot", wires=list(q._tape[-1].wires), params=None, inverse=False))
     c = E.Compiled(q._tape, n, bsz, q.engine_options, "cpu")      # the plan, built on the host only
-    assert len(c.dev_items) == 3 * len(FAMILIES)
+    param_ops = {i for i, op in enumerate(q._tape) if op.params is not None}
+    assert param_ops <= c.dev_tape and c.host_static
     got = q.get_states_1d().detach().cpu().numpy()
     ref = O.run_ops(ops, n, bsz, state=init)
     assert np.abs(got - ref).max() < 1e-5
@@ -193,3 +194,62 @@ def test_phase_encoders_on_strided_inputs():
         ops.append(dict(name="ry", wires=[k], params=xn[:, 5 + k:6 + k], inverse=False))
     ref = O.run_ops(ops, n, bsz)
     assert np.abs(q.get_states_1d().cpu().numpy() - ref).max() < 1e-5
+
+
+@gpu
+def test_device_built_merged_runs_and_gradients():
+    """Merged single-wire runs built on the GPU: fixed gates (H, SX, Y, SH)
+    between shared trainable and per-row parameterised gates (RX, RY, RZ,
+    U2, U3, Rot), inverse flags; the plan must hold device-built runs; states
+    vs the oracle (1e-5) and gradients of every parameter vs central FD of
+    the oracle (2e-4 + 1e-3 rel)."""
+    import torchquantum_amd as tq
+    import torchquantum_amd.functional as tqf
+    from torchquantum_amd import engine as E
+    rng = np.random.default_rng(17)
+    n, bsz = 5, 3
+    seq = [("hadamard", 0), ("rx", 1), ("u3", bsz), ("sx", 0), ("rot", 1), ("ry", 1), ("pauliy", 0),
+           ("rz", bsz), ("u2", 1), ("shadamard", 0)]
+    ops, tens = [], []
+    for w in range(n):
+        for j, (name, rows) in enumerate(seq[w % 3:w % 3 + 6]):
+            npar = FAMILIES.get(name, 3 if name == "rot" else 0)
+            p = rng.uniform(-np.pi, np.pi, size=(rows, npar)) if npar else None
+            ops.append(dict(name=name, wires=[w], params=p, inverse=bool((j + w) % 4 == 1)))
+        ops.append(dict(name="cnot", wires=[w, (w + 1) % n], params=None, inverse=False))
+    q = tq.QuantumDevice(n_wires=n, bsz=bsz, device=DEV)
+    for op in ops:
+        p = op["params"]
+        t = None
+        if p is not None:
+            t = torch.tensor(p, dtype=torch.float32, device=DEV, requires_grad=True)
+            tens.append(t)
+        tqf.func_name_dict[op["name"]](q, wires=op["wires"], params=t, inverse=op["inverse"])
+    c = E.Compiled(q._tape, n, bsz, q._options(), "cpu")
+    runs = [c.kops[i] for i in c.dev_kops if len(c.kops[i].src) > 1]
+    assert runs and c.host_static
+    obs, coef = ["ZXIYZ", "XXIII", "IIZZY"], [1.0, -0.5, 0.7]
+    val = tq.expval_pauli_sum(q, obs, coef)
+    st = q.get_states_1d().detach().cpu().numpy()
+    ref = O.run_ops([dict(o, params=(None if o["params"] is None else
+                                     (o["params"] if o["params"].shape[0] > 1 else o["params"][0]))) for o in ops],
+                    n, bsz)
+    assert np.abs(st - ref).max() < 1e-5
+    val.sum().backward()
+    got = np.concatenate([t.grad.detach().cpu().numpy().reshape(-1) for t in tens])
+    flat = np.concatenate([o["params"].reshape(-1) for o in ops if o["params"] is not None])
+
+    def f(v):
+        k, o2 = 0, []
+        for o in ops:
+            p = o["params"]
+            if p is not None:
+                p = v[k:k + p.size].reshape(p.shape)
+                k += p.size
+                p = p if p.shape[0] > 1 else p[0]
+            o2.append(dict(o, params=p))
+        s = O.run_ops(o2, n, bsz)
+        return float(sum(cf * O.expval_pauli(s, ob).sum() for ob, cf in zip(obs, coef)))
+
+    fd = O.central_fd(f, flat.copy(), h=1e-4)
+    assert np.all(np.abs(got - fd) <= 2e-4 + 1e-3 * np.abs(fd)), np.abs(got - fd).max()

@@ -56,7 +56,7 @@ SIGNATURES = {
     "tqsv_batch_zexpval": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int64, c_float_p, c_float_p]),
     "tqsv_mg_item_bytes": (c_int, []),
     "tqsv_dev_gate_bytes": (c_int, []),
-    "tqsv_build_dev_gates": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int64]),
+    "tqsv_build_dev_gates": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int64]),
     "tqsv_mg_to_grad": (c_int, [c_void_p, c_float_p, c_int64, c_int64, c_void_p, c_int, c_void_p]),
     "tqsv_sample_workspace_bytes": (c_int64, [c_int64, c_int]),
     "tqsv_sample": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int64, c_uint64, c_void_p, c_void_p, c_int64]),

@@ -45,12 +45,17 @@ OP_MAT1, OP_MAT2, OP_DIAG, OP_MATK, OP_REMAP = 1, 2, 3, 4, 5
 F_BATCHED, F_NOCTRL, F_REAL, F_XPERM, F_SIGN, F_MG = 1 << 8, 1 << 9, 1 << 10, 1 << 11, 1 << 12, 1 << 13
 MG_ITEM = np.dtype([("mg_slot", "<i4"), ("P", "<i4"), ("slot", "<i4", (4,)), ("u_off", "<i8"), ("bstride", "<i8")])
 MG_SUMS = 4     # Hermitian-generator sums S_I, S_X, S_Y, S_Z per F_MG op
-# gates whose per-row matrices the GPU builds from device parameters (tqsv_devgates.hip)
-DEV_GATE_CODE = {"rx": 1, "ry": 2, "rz": 3, "phaseshift": 4, "u1": 4, "u2": 5, "u3": 6}
+# single-wire gates whose matrices the GPU builds from device parameters
+# (tqsv_devgates.hip), and the fixed single-wire gates that may sit in a
+# merged run built there
+DEV_GATE_CODE = {"rx": 1, "ry": 2, "rz": 3, "phaseshift": 4, "u1": 4, "u2": 5, "u3": 6, "rot": 7}
+DEV_FIXED = ("hadamard", "shadamard", "paulix", "pauliy", "sx", "sxdg")
+DG_MAX_MEMBERS = 8
 DGF_INVERSE, DGF_CONJ, DGF_DERIV, DGF_DIAG = 1, 2, 4, 8
-DEV_GATE = np.dtype([("params", "<u8"), ("rstride", "<i8"), ("cstride", "<i8"), ("dst", "<i8"), ("bstride", "<i8"),
-                     ("adj", "<i8"), ("abstride", "<i8"), ("code", "<i4"), ("flags", "<i4"), ("rows", "<i4"),
-                     ("diag_ab", "<i4")])
+DEV_OP = np.dtype([("dst", "<i8"), ("bstride", "<i8"), ("adj", "<i8"), ("abstride", "<i8"), ("rows", "<i4"),
+                   ("m0", "<i4"), ("nm", "<i4"), ("flags", "<i4")])
+DEV_MEMBER = np.dtype([("params", "<u8"), ("rstride", "<i8"), ("cstride", "<i8"), ("code", "<i4"), ("flags", "<i4"),
+                       ("fixed", "<f4", (8,))])
 FEAT_MAT2, FEAT_MATK = 1, 2
 
 
@@ -77,11 +82,14 @@ class EngineOptions:
                  carry every parameter's gradient through the Hermitian
                  generator i dU/dtheta U^dagger (else one inner product per
                  parameter).
-    device_gates: per-row (batched) parameters of RX, RY, RZ, PhaseShift/U1,
-                 U2 and U3 that live on the GPU (encoder inputs) have their
-                 matrices built by a kernel straight into the matrix pool: no
-                 device-to-host copy of the inputs and no host work per row.
-                 Such gates are not merged or fused with neighbours.
+    device_gates: single-wire ops (one gate or a merged run) whose
+                 parameters -- shared [1, P] or per-row [bsz, P], RX, RY, RZ,
+                 PhaseShift/U1, U2, U3, Rot, with fixed H, SH, X, Y, SX, SXDG
+                 in runs -- live on the GPU as float32 have their matrices
+                 and derivatives built by a kernel straight into the matrix
+                 pool: no device-to-host copy of the parameters (trainable
+                 weights after an optimizer step, encoder inputs) and no host
+                 matrix work.
     jit:         "auto" | "on" | "off": run stages as kernels compiled at run
                  time from their programs (straight-line code, hipRTC) instead
                  of the program-interpreting kernels.  "auto" compiles for
@@ -198,7 +206,7 @@ class KOp:
     real: bool = False              # matrix entries are real for every parameter value
     xperm: bool = False             # the 2x2 is Pauli X (a controlled exchange)
     sign: bool = False              # diagonal phase -1 where the controls match, no parity
-    dev: int = 0                    # > 0: matrix built on the GPU from device parameters (DEV_GATE_CODE)
+    dev: bool = False               # matrix built on the GPU from device parameters
 
     @property
     def cmask(self) -> int:
@@ -239,18 +247,36 @@ SIGN_GATES = {"pauliz", "cz", "ccz"}
 REAL_1Q_BLOCK = X_GATES | {"ry", "hadamard", "shadamard", "cry", "chadamard", "pauliz", "i"}
 
 
-def device_gate_code(op: TapeOp, bsz: int) -> int:
-    """DEV_GATE_CODE of a tape op whose per-row matrices the GPU builds (0: host)."""
+def dev_member_code(op: TapeOp, bsz: int) -> Optional[int]:
+    """Gate code of a tape op the device builder can take as a member of a
+    single-wire op (0: a fixed gate), None if it cannot."""
     p = op.params
-    if p is None or len(op.wires) != 1 or bsz < 2 or p.dim() != 2 or p.shape[0] != bsz:
-        return 0
-    if not p.is_cuda or p.dtype != torch.float32:
-        return 0
-    return DEV_GATE_CODE.get(op.name, 0)
+    if len(op.wires) != 1:
+        return None
+    if p is None:
+        return 0 if op.name in DEV_FIXED else None
+    if p.dim() != 2 or p.shape[0] not in (1, bsz) or not p.is_cuda or p.dtype != torch.float32:
+        return None
+    return DEV_GATE_CODE.get(op.name)
 
 
-def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int],
-          device_gates: bool = False) -> List[KOp]:
+def mark_device_built(kops: List[KOp], tape: Sequence[TapeOp], bsz: int) -> None:
+    """Flag the single-wire kernel ops whose matrices the GPU builds: an
+    uncontrolled dense 2x2, a diagonal, or a merged run, all of whose members
+    the builder takes, with at least one parameterised member."""
+    for k in kops:
+        if k.kind not in ("mat1", "diag") or len(k.src) > DG_MAX_MEMBERS:
+            continue
+        if k.kind == "mat1" and (k.cbits or k.extract not in ("full", "chain")):
+            continue
+        if k.kind == "diag" and (len(k.src) != 1 or len(tape[k.src[0]].wires) != 1):
+            continue
+        codes = [dev_member_code(tape[t], bsz) for t in k.src]
+        if all(c is not None for c in codes) and any(codes):
+            k.dev = True
+
+
+def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]) -> List[KOp]:
     """Tape ops -> kernel ops (no fusion).  Identity gates vanish."""
     out: List[KOp] = []
     for ti, op in enumerate(tape):
@@ -261,7 +287,6 @@ def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]
         if rows not in (1, bsz):
             raise ValueError(f"gate '{op.name}': parameter batch {rows} does not match bsz {bsz}")
         batched = rows > 1
-        dev = device_gate_code(op, bsz) if device_gates else 0
         slots = ()
         if op.wants_grad():
             slots = tuple(param_offsets[ti] + p for p in range(op.n_params()))
@@ -272,7 +297,7 @@ def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]
             out.append(KOp("diag", tuple(pos[i] for i in pidx), tuple(pos[i] for i in cidx),
                            tuple(1 for _ in cidx), (ti,), "diag",
                            G.diag_phase_indices(cidx, pidx, k), slots, batched,
-                           sign=s.name in SIGN_GATES, real=s.name in SIGN_GATES, dev=dev))
+                           sign=s.name in SIGN_GATES, real=s.name in SIGN_GATES))
         elif s.kind == "ctrl1":
             ci, cv, t = G.ctrl_values(s, k)
             out.append(KOp("mat1", (pos[t],), tuple(pos[i] for i in ci), tuple(cv), (ti,),
@@ -287,7 +312,7 @@ def lower(tape: Sequence[TapeOp], n: int, bsz: int, param_offsets: Sequence[int]
                                           "(qubitunitary*) are not supported")
             kind = "mat1" if k == 1 else ("mat2" if k == 2 else "matk")
             out.append(KOp(kind, tuple(pos), (), (), (ti,), "full", (0, 0), slots, batched,
-                           real=k == 1 and s.name in REAL_1Q_BLOCK, xperm=k == 1 and s.name in X_GATES, dev=dev))
+                           real=k == 1 and s.name in REAL_1Q_BLOCK, xperm=k == 1 and s.name in X_GATES))
     return out
 
 
@@ -301,7 +326,7 @@ def merge_single_wire(kops: List[KOp], tape: Sequence[TapeOp]) -> List[KOp]:
     pending: Dict[int, int] = {}   # bit -> index in out of the open run
 
     def single(k: KOp) -> bool:
-        if k.cbits or k.dev:
+        if k.cbits:
             return False
         if k.kind == "mat1":
             return True
@@ -358,7 +383,7 @@ def fuse_blocks(kops: List[KOp], tape: Sequence[TapeOp], n: int, width: int) -> 
 
     for i, k in enumerate(kops):
         bs = set(k.allbits)
-        if k.dev or len(k.src) != 1 or (k.kind == "matk" and len(k.bits) > R_BITS) or len(bs) > width:
+        if len(k.src) != 1 or (k.kind == "matk" and len(k.bits) > R_BITS) or len(bs) > width:
             place(dict(bits=bs, members=[i], solo=True, P=k.P, closed=False))
             continue
         cands = sorted({last[b] for b in bs if b in last})
@@ -645,6 +670,18 @@ class _PinnedRing:
 
 
 _RINGS: Dict[torch.device, _PinnedRing] = {}
+
+
+_DEV_LAYOUT_OK = False
+
+
+def _check_dev_layout():
+    global _DEV_LAYOUT_OK
+    if not _DEV_LAYOUT_OK:
+        want = DEV_OP.itemsize | (DEV_MEMBER.itemsize << 16)
+        if N.lib().tqsv_dev_gate_bytes() != want:
+            raise RuntimeError("DevOp / DevMember layout mismatch with libtqsv.so")
+        _DEV_LAYOUT_OK = True
 
 
 def _upload(host: np.ndarray, device: torch.device) -> torch.Tensor:
@@ -1101,7 +1138,7 @@ class Compiled:
         # (both fixed by the structure key, which holds shapes and requires_grad)
         self.param_idx = [i for i, op in enumerate(tape) if op.params is not None and not op.is_matrix_param]
         self.any_param_grad = any(tape[i].params.requires_grad for i in self.param_idx)
-        kops = lower(tape, n, bsz, self.param_offsets, opts.device_gates)
+        kops = lower(tape, n, bsz, self.param_offsets)
         if opts.mode == "fused" and opts.fusion_width >= 2:
             kops = fuse_blocks(kops, tape, n, opts.fusion_width)
             for k in kops:
@@ -1109,6 +1146,8 @@ class Compiled:
                     k.n_for_block = n
         elif opts.mode == "fused" and opts.merge_1q:
             kops = merge_single_wire(kops, tape)
+        if opts.device_gates:
+            mark_device_built(kops, tape, bsz)
         self.kops = kops
         # pool layout: kop data (fwd, with derivative matrices when trainable)
         self.data_off: Dict[int, tuple] = {}
@@ -1129,24 +1168,54 @@ class Compiled:
                 self.adj_off[i] = (pos, per if k.batched else 0)
                 pos += per * rows
         self.pool_len = max(pos, 1)
-        # gates whose matrices the GPU builds (device parameters): one DEV_GATE
-        # record each, appended to the uploaded pool; the parameter pointers
-        # and strides are filled per flush
-        dev = [i for i, k in enumerate(kops) if k.dev]
-        self.dev_kops = dev
-        self.dev_src = [kops[i].src[0] for i in dev]
-        self.dev_tape = frozenset(self.dev_src)
-        self.dev_items = np.zeros(len(dev), dtype=DEV_GATE)
-        for j, i in enumerate(dev):
-            k, op = kops[i], tape[kops[i].src[0]]
-            self.dev_items["dst"][j], self.dev_items["bstride"][j] = self.data_off[i]
-            self.dev_items["adj"][j], self.dev_items["abstride"][j] = self.adj_off.get(i, (-1, 0))
-            self.dev_items["code"][j] = k.dev
-            self.dev_items["flags"][j] = ((DGF_INVERSE if op.inverse else 0) | (DGF_CONJ if op.conj else 0)
-                                          | (DGF_DIAG if k.kind == "diag" else 0))
-            self.dev_items["rows"][j] = bsz
-            self.dev_items["diag_ab"][j] = (k.diag_idx[0] & 1) | ((k.diag_idx[1] & 1) << 1)
-        self.dev_tail = (len(dev) * DEV_GATE.itemsize + 7) // 8        # complex64 elements
+        # ops whose matrices the GPU builds (device parameters): one DEV_OP
+        # record each plus one DEV_MEMBER per member gate, appended to the
+        # uploaded pool; parameter pointers and strides are filled per flush
+        self.dev_kops = [i for i, k in enumerate(kops) if k.dev]
+        self.dev_tape = frozenset(t for i in self.dev_kops for t in kops[i].src)
+        self.dev_ops = np.zeros(len(self.dev_kops), dtype=DEV_OP)
+        members = []
+        self.dev_param_mem: List[int] = []      # member record index of each parameterised member
+        self.dev_param_tape: List[int] = []     # its tape index
+        deriv_ops = []
+        for j, i in enumerate(self.dev_kops):
+            k = kops[i]
+            dop = self.dev_ops[j:j + 1]
+            dop["dst"], dop["bstride"] = self.data_off[i]
+            dop["adj"], dop["abstride"] = self.adj_off.get(i, (-1, 0))
+            dop["rows"] = bsz if k.batched else 1
+            dop["m0"], dop["nm"] = len(members), len(k.src)
+            if k.kind == "diag":
+                dop["flags"] = DGF_DIAG | ((k.diag_idx[0] & 1) << 8) | ((k.diag_idx[1] & 1) << 9)
+            if k.P:
+                deriv_ops.append(j)
+            for t in k.src:
+                op = tape[t]
+                code = dev_member_code(op, bsz)
+                rec = np.zeros(1, dtype=DEV_MEMBER)
+                rec["code"] = code
+                fl = (DGF_INVERSE if op.inverse else 0) | (DGF_CONJ if op.conj else 0)
+                if code:
+                    fl |= DGF_DERIV if op.wants_grad() else 0
+                    self.dev_param_mem.append(len(members))
+                    self.dev_param_tape.append(t)
+                else:
+                    u = G.FIXED_NP[op.name]
+                    if op.inverse:
+                        u = u.conj().T
+                    if op.conj:
+                        u = u.conj()
+                    rec["fixed"][0] = np.stack([u.reshape(-1).real, u.reshape(-1).imag], -1).reshape(-1)
+                    fl = 0
+                rec["flags"] = fl
+                members.append(rec)
+        self.dev_members = np.concatenate(members) if members else np.zeros(0, dtype=DEV_MEMBER)
+        self.dev_deriv_ops = np.array(deriv_ops, dtype=np.int64)
+        self.dev_tail = (self.dev_ops.nbytes + self.dev_members.nbytes) // 8     # complex64 elements
+        # every parameterised op built on the GPU: the host part of the pool
+        # depends on the plan only and is built once
+        self.host_static = all(k.dev or all(tape[t].params is None for t in k.src) for k in kops)
+        self._static_host: Optional[np.ndarray] = None
         self.steps: List[tuple] = []
         self.perms: Dict[int, np.ndarray] = {}
         self._fill = None
@@ -1274,28 +1343,39 @@ class Compiled:
         if torch.device(self.device).type != "cuda":
             return torch.from_numpy(host).to(self.device)
         pool = _upload(host, torch.device(self.device))
-        if len(self.dev_items):
-            N.call("tqsv_build_dev_gates", N.stream_of(pool), pool.data_ptr(), pool.data_ptr() + 8 * self.pool_len,
-                   len(self.dev_items), self.bsz)
+        if len(self.dev_ops):
+            _check_dev_layout()
+            base = pool.data_ptr() + 8 * self.pool_len
+            N.call("tqsv_build_dev_gates", N.stream_of(pool), pool.data_ptr(), base, base + self.dev_ops.nbytes,
+                   len(self.dev_ops), self.bsz)
         return pool
 
     def pool_host(self, tape: Sequence[TapeOp], with_grad: bool, with_adj: bool) -> np.ndarray:
         """The matrix pool (complex64) for the current parameter values."""
+        tail = None
+        if len(self.dev_ops):
+            ops = self.dev_ops
+            if with_grad and len(self.dev_deriv_ops):
+                ops = ops.copy()
+                ops["flags"][self.dev_deriv_ops] |= DGF_DERIV
+            mem = self.dev_members.copy()
+            ps = [tape[t].params for t in self.dev_param_tape]
+            mem["params"][self.dev_param_mem] = [p.data_ptr() for p in ps]
+            mem["rstride"][self.dev_param_mem] = [p.stride(0) if p.shape[0] > 1 else 0 for p in ps]
+            mem["cstride"][self.dev_param_mem] = [p.stride(1) for p in ps]
+            tail = (ops, mem)
+        if self._static_host is not None:
+            host = np.empty(self.pool_len + self.dev_tail, dtype=np.complex64)
+            host[:self.pool_len] = self._static_host
+            self._put_tail(host, tail)
+            return host
+        if self.host_static:
+            with_adj = True          # built once for both uses
         if self._gv_layout is None:
             self._gv_layout = _GVLayout(tape, self.dev_tape)
         gv = GateValues(tape, with_grad, self._gv_layout)
         host = np.zeros(self.pool_len + self.dev_tail, dtype=np.complex64)
-        if len(self.dev_items):
-            items = self.dev_items.copy()
-            for j, ti in enumerate(self.dev_src):
-                p = tape[ti].params
-                if not p.is_cuda or p.dtype != torch.float32 or p.dim() != 2 or p.shape[0] != self.bsz:
-                    raise RuntimeError("device-built gate: its parameter tensor changed form after planning")
-                items["params"][j] = p.data_ptr()
-                items["rstride"][j], items["cstride"][j] = p.stride(0), p.stride(1)
-                if with_grad and self.kops[self.dev_kops[j]].P:
-                    items["flags"][j] |= DGF_DERIV
-            host[self.pool_len:].view(np.uint8)[:items.nbytes] = items.view(np.uint8)
+        self._put_tail(host, tail)
         if self._fill is None:
             self._fill = self._build_fill(tape)
         singles, chains, others = self._fill
@@ -1353,7 +1433,16 @@ class Compiled:
                 host[aoff[:, None] + np.arange(4)] = adj
         for i in others:
             self._fill_one(i, tape, gv, host, with_grad, with_adj)
+        if self.host_static:
+            self._static_host = host[:self.pool_len].copy()
         return host
+
+    def _put_tail(self, host: np.ndarray, tail):
+        if tail is not None:
+            ops, mem = tail
+            raw = host[self.pool_len:].view(np.uint8)
+            raw[:ops.nbytes] = ops.view(np.uint8)
+            raw[ops.nbytes:ops.nbytes + mem.nbytes] = mem.view(np.uint8)
 
     def _build_fill(self, tape):
         """Group kernel ops by how their pool data is formed, once per plan."""
