@@ -290,6 +290,7 @@ def time_batch(n_circ, steps, tq):
     torch.cuda.synchronize()
     t_loop = time.perf_counter() - t0
     alg = batch_pass_bytes(cb)
+    tuned = tune_batch(circs, tq)
     return {"workload": f"{n_circ} circuits, n ~ U{{8..22}}, depth ~ U{{4..40}} layers of n random gates "
                         "from {H,RX,RY,RZ,U3,CNOT,CZ,SWAP}, <Z_0> each, bsz 1",
             "roofline": {"bound": "hbm", "algorithmic_bytes_per_batch": alg,
@@ -302,7 +303,8 @@ def time_batch(n_circ, steps, tq):
             "levels": len(cb.levels), "launches_per_batch": cb.n_launches,
             "ms_graph_replay": t_graph * 1e3, "ms_eager_batch": t_eager * 1e3, "ms_per_circuit_loop": t_loop * 1e3,
             "amp_updates_per_s_graph": upd / t_graph, "speedup_vs_per_circuit_loop": t_loop / t_graph,
-            "build_s": t_build, "z0_max_abs_diff_vs_per_circuit": float((out - ref).abs().max())}, \
+            "build_s": t_build, "z0_max_abs_diff_vs_per_circuit": float((out - ref).abs().max()),
+            "plans": tuned}, \
         (out.reshape(-1).cpu().numpy(), circs)
 
 
@@ -348,6 +350,68 @@ def encoder_training(tq, n, bsz, blocks, steps, warmup):
                         f"({n_par} params) + <Z_i> + linear head, cross entropy, Adam step; new inputs each step",
             "ms_per_step": dt * 1e3, "steps_per_s": 1.0 / dt, "host_issue_ms_per_step": t_host / steps * 1e3,
             "loss_last": float(loss.detach())}
+
+
+def _replay_ms(cb, steps=5):
+    cb.capture()
+    cb.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cb.run()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def tune_batch(circs, tq):
+    """Per-circuit plans for the uneven batch: every circuit autotuned over
+    fusion widths 1..4 (exhaustive: each candidate timed with the
+    program-interpreting stage kernels, the op code the batch kernel runs),
+    and the learned planner trained on a quarter of those timings choosing
+    for all circuits from plan features alone.  Reports the batch time with
+    the default plans, the exhaustive winners and the learned choices, and
+    the learned planner's regret on the held-out circuits."""
+    from torchquantum_amd import batch as B
+    from torchquantum_amd.autotune import Autotuner, LearnedPlanner, BATCH_CANDIDATES, circuit_key, tune_circuits
+    from torchquantum_amd.costmodel import MLPCostModel
+    tapes = [(B.ops_to_tape(ops), n) for n, ops in circs]
+    tuner = Autotuner()
+    t0 = time.perf_counter()
+    best = tune_circuits(tuner, tapes, BATCH_CANDIDATES, reps=3)
+    t_tune = time.perf_counter() - t0
+    log(f"[bench] batch: exhaustive autotune {t_tune:.1f} s")
+    keys = [circuit_key(tape, n, 1) for tape, n in tapes]
+    train = set(keys[::4])
+    X, y = tuner.dataset(sorted(train))
+    model = MLPCostModel(device="cuda", seed=0)
+    hist = model.fit(X, y, epochs=1500, lr=3e-3)
+    planner = LearnedPlanner(model, BATCH_CANDIDATES)
+    t0 = time.perf_counter()
+    picks = [planner.choose(tape, n, 1) for tape, n in tapes]
+    t_pick = time.perf_counter() - t0
+    held = [i for i, k in enumerate(keys) if k not in train]
+
+    def ms_of(i, o):
+        by = {json.dumps(r["opts"], sort_keys=True): r["ms"] for r in tuner.db[keys[i]]["records"]}
+        o = dict(o.__dict__, stage_bits=min(o.stage_bits or 12, tapes[i][1]))
+        return by[json.dumps(o, sort_keys=True)]
+
+    sum_def = sum(ms_of(i, BATCH_CANDIDATES[0]) for i in held)
+    sum_best = sum(ms_of(i, best[i]) for i in held)
+    sum_pick = sum(ms_of(i, picks[i]) for i in held)
+    out = {"candidates": "fusion width 1..4, stage width min(12, n), interpreting stage kernels",
+           "autotune_s": t_tune, "learned_selection_s": t_pick, "train_circuits": len(train),
+           "held_out_circuits": len(held), "train_loss_first_last": [hist[0], hist[-1]],
+           "held_out_sum_ms_default": sum_def, "held_out_sum_ms_exhaustive": sum_best,
+           "held_out_sum_ms_learned": sum_pick, "learned_gain_vs_default": sum_def / sum_pick,
+           "learned_regret_vs_exhaustive": sum_pick / sum_best - 1,
+           "widths_picked": {str(k): sum(1 for o in picks if o.fusion_width == k) for k in (1, 2, 3, 4)}}
+    for label, opts in (("exhaustive", best), ("learned", picks)):
+        cb = B.CircuitBatch(circs, device="cuda", options=opts)
+        out[f"ms_graph_replay_{label}_plans"] = _replay_ms(cb)
+        del cb
+        torch.cuda.empty_cache()
+    return out
 
 
 def copy_calibration(nbytes=1 << 31):

@@ -54,54 +54,53 @@ def test_mlp_kernels_vs_oracle():
 
 @gpu
 def test_learned_planner_gain():
+    """The learned planner against the method without it.  Plans are fusion
+    widths 1..4 on the program-interpreting stage code (what the uneven batch
+    runs); every candidate of every circuit is timed (exhaustive autotuning),
+    the MLP is trained by this package's kernels on the timings of 24
+    circuits of 12..20 qubits and then chooses, from plan features alone,
+    for 8 held-out circuits of 20..23 qubits (larger than any it saw).  The
+    learned choices must be faster in sum than the default plan (width 1)
+    and within 15% of the exhaustive optimum."""
     from torchquantum_amd import batch as B
-    from torchquantum_amd import engine as E
-    from torchquantum_amd.autotune import Autotuner, LearnedPlanner, DEFAULT_CANDIDATES, time_plan
+    from torchquantum_amd.autotune import Autotuner, LearnedPlanner, BATCH_CANDIDATES, circuit_key, tune_circuits
     from torchquantum_amd.costmodel import MLPCostModel
 
-    circs = B.random_circuits(n_circ=24, n_range=(12, 20), depth_range=(4, 24), seed=42)
-    tapes = [(B.ops_to_tape(ops), n) for n, ops in circs]
+    train_c = B.random_circuits(n_circ=24, n_range=(12, 20), depth_range=(4, 24), seed=42)
+    test_c = B.random_circuits(n_circ=8, n_range=(20, 23), depth_range=(4, 24), seed=43)
+    assert max(n for n, _ in test_c) >= 22
+    tapes = [(B.ops_to_tape(ops), n) for n, ops in train_c + test_c]
     tuner = Autotuner()
     t0 = time.perf_counter()
-    keys = []
-    from torchquantum_amd.autotune import circuit_key
-    for tape, n in tapes:
-        tuner.tune(tape, n, 1, reps=3)
-        keys.append(circuit_key(tape, n, 1))
+    tune_circuits(tuner, tapes, BATCH_CANDIDATES, reps=3)
     t_exhaustive = (time.perf_counter() - t0) / len(tapes)
-    train, test = keys[:16], keys[16:]
-    X, y = tuner.dataset(train)
+    keys = [circuit_key(tape, n, 1) for tape, n in tapes]
+    X, y = tuner.dataset(keys[:24])
     model = MLPCostModel(device="cuda", seed=0)
     hist = model.fit(X, y, epochs=1500, lr=3e-3)
     assert hist[-1] < hist[0]
-    planner = LearnedPlanner(model)
+    planner = LearnedPlanner(model, BATCH_CANDIDATES)
     tot_default = tot_learned = tot_best = 0.0
     t_learned = 0.0
     rel_err = []
-    for (tape, n), key in zip(tapes[16:], test):
+    for (tape, n), key in zip(tapes[24:], keys[24:]):
         recs = tuner.db[key]["records"]
-        best = min(r["ms"] for r in recs)
-        default = E.EngineOptions(stage_bits=min(12, n))
+        by = {json.dumps(r["opts"], sort_keys=True): r["ms"] for r in recs}
         t1 = time.perf_counter()
         pick = planner.choose(tape, n, 1)
         t_learned += time.perf_counter() - t1
-        by = {json.dumps(r["opts"], sort_keys=True): r["ms"] for r in recs}
-        ms_pick = by.get(json.dumps(pick.__dict__, sort_keys=True))
-        if ms_pick is None:
-            ms_pick = time_plan(tape, n, 1, pick, reps=3)
-        ms_def = by.get(json.dumps(default.__dict__, sort_keys=True)) or time_plan(tape, n, 1, default, reps=3)
-        tot_default += ms_def
-        tot_learned += ms_pick
-        tot_best += best
+        default = dict(BATCH_CANDIDATES[0].__dict__, stage_bits=min(12, n))
+        tot_default += by[json.dumps(default, sort_keys=True)]
+        tot_learned += by[json.dumps(pick.__dict__, sort_keys=True)]
+        tot_best += min(r["ms"] for r in recs)
         opts, pred = planner.score(tape, n, 1)
         for o, p in zip(opts, pred):
-            m = by.get(json.dumps(o.__dict__, sort_keys=True))
-            if m:
-                rel_err.append(abs(p - m) / m)
-    t_learned /= len(test)
-    report = {"held_out_circuits": len(test), "sum_ms_default_plan": tot_default,
-              "sum_ms_learned_plan": tot_learned, "sum_ms_exhaustive_best": tot_best,
-              "gain_vs_default": tot_default / tot_learned, "regret_vs_exhaustive": tot_learned / tot_best - 1,
+            rel_err.append(abs(p - by[json.dumps(o.__dict__, sort_keys=True)]) / by[json.dumps(o.__dict__, sort_keys=True)])
+    t_learned /= len(test_c)
+    report = {"held_out_circuits": len(test_c), "held_out_qubits": [n for n, _ in test_c],
+              "sum_ms_default_plan": tot_default, "sum_ms_learned_plan": tot_learned,
+              "sum_ms_exhaustive_best": tot_best, "gain_vs_default": tot_default / tot_learned,
+              "regret_vs_exhaustive": tot_learned / tot_best - 1,
               "median_rel_prediction_error": float(np.median(rel_err)),
               "selection_s_learned": t_learned, "selection_s_exhaustive": t_exhaustive,
               "train_loss_first_last": [hist[0], hist[-1]]}
@@ -111,5 +110,6 @@ def test_learned_planner_gain():
         with open(os.path.join(out_dir, "costmodel_report.json"), "w") as f:
             json.dump(report, f, indent=1)
     print(report)
-    assert tot_learned <= 1.05 * tot_default
+    assert tot_learned < tot_default, report
+    assert tot_learned <= 1.15 * tot_best, report
     assert t_learned < t_exhaustive

@@ -24,6 +24,10 @@ from .costmodel import MLPCostModel, plan_features
 
 DEFAULT_CANDIDATES = ([E.EngineOptions(fusion_width=k, stage_bits=m) for k in (1, 2, 3, 4) for m in (10, 12, 13)]
                       + [E.EngineOptions(fusion_width=k, stage_bits=12) for k in (5, 6)])
+# Plans of circuits run by CircuitBatch: every op inside stages (fusion width
+# <= 4, stage width <= 12) and the program-interpreting stage code, which is
+# what the batch kernel executes; timed with the interpreting stage kernels.
+BATCH_CANDIDATES = [E.EngineOptions(fusion_width=k, stage_bits=12, jit="off") for k in (1, 2, 3, 4)]
 
 
 def circuit_key(tape: Sequence[E.TapeOp], n: int, bsz: int) -> str:
@@ -125,3 +129,16 @@ class LearnedPlanner:
     def choose(self, tape, n, bsz) -> E.EngineOptions:
         opts, pred = self.score(tape, n, bsz)
         return opts[int(np.argmin(pred))]
+
+    def choose_from_records(self, records: Sequence[dict]) -> E.EngineOptions:
+        """The candidate with the smallest predicted time among an autotuner
+        entry's records, from their stored plan features (no plan rebuilt)."""
+        pred = self.model.predict_ms(np.array([r["features"] for r in records]))
+        return E.EngineOptions(**records[int(np.argmin(pred))]["opts"])
+
+
+def tune_circuits(tuner: "Autotuner", tapes: Sequence[tuple], candidates: Sequence[E.EngineOptions],
+                  reps: int = 3) -> List[E.EngineOptions]:
+    """Exhaustive autotuning of many (tape, n) circuits at bsz 1; returns the
+    winner per circuit (the tuner keeps every timing as training data)."""
+    return [tuner.tune(tape, n, 1, candidates=candidates, reps=reps) for tape, n in tapes]
