@@ -436,3 +436,34 @@ def test_pauli_sums_grouped(n):
     assert np.abs(val.detach().cpu().numpy() - ref).max() < 1e-5
     gref = 2 * sum(c * _apply_pauli_np(init, o) for o, c in zip(obs, coef))
     assert np.abs(x.grad.cpu().numpy() - gref).max() < 1e-5 * max(1.0, np.abs(gref).max())
+
+
+@gpu
+@pytest.mark.parametrize("k", [3, 4, 5, 6])
+@pytest.mark.parametrize("batched", [False, True])
+def test_dense_kq_lds_staged(k, batched):
+    """Dense k-qubit gates (3 <= k <= 6) on the LDS-staged kernel: one matrix
+    or one per batch row, unsorted target bits, 0-2 controls, and a shape
+    whose rows hold fewer groups than a workgroup pass (the global-read
+    kernel); through the C ABI, vs numpy complex128, 1e-5."""
+    from torchquantum_amd import _native as N
+    rng = np.random.default_rng(300 + 10 * k + batched)
+    for n, bsz, nctrl in ((11, 3, 0), (12, 2, 2), (k + 2, 3, 1)):
+        bits = [int(b) for b in rng.permutation(n)[:k]]
+        rest = [b for b in range(n) if b not in bits]
+        cb = [int(b) for b in rng.choice(rest, min(nctrl, len(rest)), replace=False)]
+        cmask = sum(1 << b for b in cb)
+        cval = sum(1 << b for b in cb if rng.random() < 0.5)
+        mats = []
+        for _ in range(bsz if batched else 1):
+            z = rng.normal(size=(2 ** k, 2 ** k)) + 1j * rng.normal(size=(2 ** k, 2 ** k))
+            mats.append(np.linalg.qr(z)[0])
+        init = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+        init /= np.linalg.norm(init, axis=1, keepdims=True)
+        st = torch.tensor(init, dtype=torch.complex64, device=DEV)
+        m = torch.tensor(np.stack(mats), dtype=torch.complex64, device=DEV).contiguous()
+        N.call("tqsv_apply_kq", N.stream_of(st), st.data_ptr(), bsz, n, k, N.int_array(bits), cmask, cval,
+               m.data_ptr(), (4 ** k) if batched else 0)
+        ref = np.stack([_dense_apply_np(init[b:b + 1], mats[b if batched else 0], bits, cmask, cval, n)[0]
+                        for b in range(bsz)])
+        assert np.abs(st.cpu().numpy() - ref).max() < 1e-5, (k, batched, n, bsz, nctrl)

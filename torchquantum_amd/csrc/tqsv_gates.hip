@@ -414,6 +414,50 @@ __global__ __launch_bounds__(256) void k_kq(float2* __restrict__ st, uint64_t n_
   }
 }
 
+// Dense k-qubit gate, 3 <= k <= 6, with the matrix staged in LDS (rows padded
+// to d + 1 elements so that the lanes of a wave, which read different rows,
+// spread over the banks).  Each workgroup owns a contiguous range of groups
+// (a multiple of its groups per pass, so with per-row matrices a pass never
+// straddles two batch rows) and loads a matrix once per batch row it enters;
+// the reads of the inner product then come from LDS instead of a strided
+// walk of global memory per thread.
+template <bool BATCHED>
+__global__ __launch_bounds__(256) void k_kq_lds(float2* __restrict__ st, uint64_t n_groups, InsertSpec ins,
+                                               uint64_t cval, KSpec ks, const float2* __restrict__ mat,
+                                               int64_t bstride, int lg_groups_row, uint64_t chunk) {
+  extern __shared__ float2 lds[];
+  const int d = 1 << ks.k, dp = d + 1;
+  const int gpb = 256 / d;
+  float2* us = lds;                 // d rows of dp
+  float2* xs = lds + d * dp;        // gpb groups of d
+  const int t = threadIdx.x, r = t % d, gl = t / d;
+  const uint64_t gbeg = (uint64_t)blockIdx.x * chunk;
+  const uint64_t gend = gbeg + chunk < n_groups ? gbeg + chunk : n_groups;
+  const uint64_t off_r = kq_offset(ks, r);
+  int64_t cur = -1;
+  for (uint64_t g0 = gbeg; g0 < gend; g0 += gpb) {
+    const int64_t row = BATCHED ? (int64_t)(g0 >> lg_groups_row) : 0;
+    if (row != cur) {              // block-uniform
+      __syncthreads();
+      const float2* U = mat + (BATCHED ? row * bstride : 0);
+      for (int e = t; e < d * d; e += 256) us[(e / d) * dp + (e % d)] = U[e];
+      cur = row;
+    }
+    const uint64_t g = g0 + gl;
+    const uint64_t a = (insert_zeros(g, ins) | cval) + off_r;
+    if (g < gend) xs[t] = st[a];
+    __syncthreads();
+    if (g < gend) {
+      const float2* up = us + r * dp;
+      const float2* x = xs + gl * d;
+      float2 acc = make_float2(0.f, 0.f);
+      for (int c = 0; c < d; ++c) acc = cfma(up[c], x[c], acc);
+      st[a] = acc;
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // dense k-qubit gate on the matrix cores, k = 5, 6 (one matrix for all rows).
 // Out = U In over groups of d = 2^k amplitudes is the real GEMM
@@ -826,6 +870,22 @@ TQSV_API int tqsv_apply_kq(hipStream_t s, void* state, int64_t bsz, int n, int k
   const int d = 1 << k;
   const int lg_groups_row = n - k - __builtin_popcountll(cmask);
   const uint64_t n_groups = (uint64_t)bsz << lg_groups_row;
+  if (k >= 3 && k <= 6 && (mat_bstride == 0 || (1ll << lg_groups_row) >= 256 / d)) {
+    // matrix in LDS; contiguous group ranges of a multiple of the groups per pass
+    const uint64_t gpb = 256 / d;
+    const uint64_t want = (uint64_t)num_cu() * 8;
+    uint64_t chunk = (n_groups + want - 1) / want;
+    chunk = ((chunk + gpb - 1) / gpb) * gpb;
+    const unsigned grid = (unsigned)((n_groups + chunk - 1) / chunk);
+    const size_t shm = sizeof(float2) * ((size_t)d * (d + 1) + (size_t)gpb * d);
+    if (mat_bstride)
+      k_kq_lds<true><<<grid, 256, shm, s>>>((float2*)state, n_groups, ins, cval, ks, (const float2*)mat, mat_bstride,
+                                            lg_groups_row, chunk);
+    else
+      k_kq_lds<false><<<grid, 256, shm, s>>>((float2*)state, n_groups, ins, cval, ks, (const float2*)mat, 0,
+                                             lg_groups_row, chunk);
+    return check_launch("tqsv_apply_kq");
+  }
   const unsigned threads = 256;
   const int gpb = threads >= (unsigned)d ? threads / d : 1;
   const size_t shm = sizeof(float2) * (size_t)gpb * d;
