@@ -71,8 +71,10 @@ def time_forward(n, bsz, ops, mode, steps, warmup, tq, jit="auto"):
     t0 = time.perf_counter()
     for _ in range(steps):
         out = step()
+    t_issue = time.perf_counter() - t0
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    time_forward.host_issue_s = t_issue / steps
     return dt, out
 
 
@@ -827,10 +829,19 @@ def main():
         upd1 = len(ops1) * 32 * (1 << 10)
         log("[bench] 10-qubit bsz-32 circuit (160 gates)")
         t1_pg, _ = time_forward(10, 32, ops1, "pergate", 20, 3, tq)
-        t1_f, _ = time_forward(10, 32, ops1, "fused", 20, 3, tq)
+        t1_f, _ = time_forward(10, 32, ops1, "fused", 50, 3, tq)
+        host1 = time_forward.host_issue_s
+        rec1 = []
+        from torchquantum_amd import engine as E
+        E.set_stage_timer(rec1)
+        time_forward(10, 32, ops1, "fused", 20, 0, tq)
+        E.set_stage_timer(None)
+        torch.cuda.synchronize()
+        stage_ms = sum(e0.elapsed_time(e1) for _, _, e0, e1 in rec1) / 20
         extra["small_circuit"] = {"workload": "10 qubits, bsz 32, 8 x [U3 all wires, CNOT ring], <Z_i> (160 gates)",
                                   "pergate_ms_per_circuit_batch": t1_pg * 1e3,
-                                  "fused_ms_per_circuit_batch": t1_f * 1e3, "fused_amp_updates_per_s": upd1 / t1_f}
+                                  "fused_ms_per_circuit_batch": t1_f * 1e3, "fused_amp_updates_per_s": upd1 / t1_f,
+                                  "fused_host_issue_ms": host1 * 1e3, "fused_stage_kernel_ms": stage_ms}
         log("[bench] dense k = 5, 6 blocks")
         extra["dense_block"] = dense_block_bench()
         log("[bench] encoder + ansatz training steps")

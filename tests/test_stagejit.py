@@ -107,7 +107,7 @@ def test_generated_code_follows_program():
     src = J.gen_forward(s.words, "k")
     assert "const float2 v" not in src
     for j in range(16):
-        assert f"{J._amp(sp, 'out', j, False)} = i{j ^ (1 << K)};" in src
+        assert J._store(sp, "out", j, f"i{j ^ (1 << K)}", False) in src
 
 
 def test_code_object_disk_cache(tmp_path, monkeypatch):

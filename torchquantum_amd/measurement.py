@@ -50,6 +50,29 @@ def _norm(qdev) -> torch.Tensor:
     return expval_pauli_strings(qdev.get_states_1d(), qdev.n_wires, ["I" * qdev.n_wires])[:, 0]
 
 
+# (observable eigenvalue tensors, device) -> (coefficients on the device, any half-sum != 0)
+_COEF_CACHE: dict = {}
+
+
+def _expval_coef(observables, device):
+    """[2, len] coefficients of <Z> and of |psi|^2 per observable: <O> = p0 e0
+    + p1 e1 = (e0 - e1)/2 <Z> + (e0 + e1)/2 |psi|^2.  Cached per observable
+    set (class-level eigenvalues) and device."""
+    key = (tuple(id(obs.eigvals) for obs in observables), str(device))
+    hit = _COEF_CACHE.get(key)
+    if hit is not None:
+        return hit
+    ev = torch.stack([obs.eigvals.real.detach().to("cpu", torch.float32).reshape(-1)[:2] for obs in observables])
+    half_diff, half_sum = 0.5 * (ev[:, 0] - ev[:, 1]), 0.5 * (ev[:, 0] + ev[:, 1])
+    coef = torch.stack([half_diff, half_sum]).to(device)
+    out = (coef, bool((half_sum != 0).any()), bool((half_diff == 1).all() and (half_sum == 0).all()))
+    if all(isinstance(getattr(type(obs), "eigvals", None), torch.Tensor) for obs in observables):
+        if len(_COEF_CACHE) > 256:
+            _COEF_CACHE.clear()
+        _COEF_CACHE[key] = out
+    return out
+
+
 def expval(qdev, wires: Union[int, List[int]], observables):
     if isinstance(wires, int):
         wires = [wires]
@@ -58,19 +81,13 @@ def expval(qdev, wires: Union[int, List[int]], observables):
         for rotation in obs.diagonalizing_gates():
             rotation(qdev, wires=wire)
     z = _z_all(qdev)
-    # <O> = p0 * e0 + p1 * e1 with p0 - p1 = <Z_wire>, p0 + p1 = |psi|^2
-    ev = torch.stack([obs.eigvals.real.detach().to("cpu", torch.float32).reshape(-1)[:2] for obs in observables])
-    half_diff, half_sum = 0.5 * (ev[:, 0] - ev[:, 1]), 0.5 * (ev[:, 0] + ev[:, 1])
-    coef = torch.stack([half_diff, half_sum])
-    if z.is_cuda:
-        coef = coef.pin_memory()
-    coef = coef.to(z.device, non_blocking=True)
-    if list(wires) == list(range(qdev.n_wires)):
-        zc = z
-    else:
-        zc = z[:, torch.as_tensor(list(wires), dtype=torch.long).to(z.device)]
+    coef, any_sum, pure_z = _expval_coef(observables, z.device)
+    all_wires = list(wires) == list(range(qdev.n_wires))
+    if pure_z and all_wires:
+        return z                       # PauliZ on every wire: <Z_w> itself
+    zc = z if all_wires else z[:, torch.as_tensor(list(wires), dtype=torch.long).to(z.device)]
     out = zc * coef[0]
-    if bool((half_sum != 0).any()):
+    if any_sum:
         # marginals sum to the squared norm, which need not be 1 (set_states, cu2, reset...)
         out = out + _norm(qdev)[:, None] * coef[1]
     return out
@@ -85,7 +102,11 @@ class MeasureAll(QuantumModule):
         self.v_c_reg_mapping = v_c_reg_mapping
 
     def forward(self, qdev):
-        x = expval(qdev, list(range(qdev.n_wires)), [self.obs()] * qdev.n_wires)
+        inst = self.__dict__.get("_obs_inst")
+        if inst is None or type(inst) is not self.obs:
+            inst = self.obs()                 # one observable instance, reused across calls
+            self.__dict__["_obs_inst"] = inst
+        x = expval(qdev, list(range(qdev.n_wires)), [inst] * qdev.n_wires)
         if self.v_c_reg_mapping is not None:
             c2v = self.v_c_reg_mapping["c2v"]
             perm = [c2v[k] for k in range(x.shape[-1]) if k in c2v.keys()]

@@ -94,24 +94,27 @@ class Operator(QuantumModule):
 
     def forward(self, q_device, wires=None, params=None, inverse=None):
         """op_types.py:182-264"""
+        d = self.__dict__        # plain attributes are set directly (nn.Module.__setattr__ costs microseconds)
         if inverse is not None:
-            self.inverse = inverse
+            d["inverse"] = inverse
         if params is not None:
             self.params = params
-        if self.params is not None and isinstance(self.params, torch.Tensor) and self.params.dim() == 1:
-            self.params = self.params.unsqueeze(-1)
+        p = self.params
+        if p is not None and isinstance(p, torch.Tensor) and p.dim() == 1:
+            self.params = p = p.unsqueeze(-1)
         if wires is not None:
-            self.wires = [wires] if isinstance(wires, int) else wires
-        if self.params is None:
-            if self.n_wires is None:
-                self.func(q_device, self.wires, inverse=self.inverse)
+            d["wires"] = [wires] if isinstance(wires, int) else wires
+        w, nw, inv = d["wires"], d["n_wires"], d["inverse"]
+        if p is None:
+            if nw is None:
+                self.func(q_device, w, inverse=inv)
             else:
-                self.func(q_device, self.wires, n_wires=self.n_wires, inverse=self.inverse)
+                self.func(q_device, w, n_wires=nw, inverse=inv)
         else:
-            if self.n_wires is None:
-                self.func(q_device, self.wires, params=self.params, inverse=self.inverse)
+            if nw is None:
+                self.func(q_device, w, params=p, inverse=inv)
             else:
-                self.func(q_device, self.wires, params=self.params, n_wires=self.n_wires, inverse=self.inverse)
+                self.func(q_device, w, params=p, n_wires=nw, inverse=inv)
 
     def __repr__(self):
         return f" class: {self.name} \n parameters: {self.params} \n wires: {self.wires} \n inverse: {self.inverse}"

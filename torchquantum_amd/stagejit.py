@@ -95,6 +95,18 @@ __device__ __forceinline__ float allsum8(float v) {
 }
 """
 
+NT_HELPERS = r"""
+typedef float tq_v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ldnt(const float2* p) {
+  const tq_v2f v = __builtin_nontemporal_load(reinterpret_cast<const tq_v2f*>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void stnt(float2* p, float2 v) {
+  tq_v2f t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<tq_v2f*>(p));
+}
+"""
+
 ONE = "make_float2(1.f, 0.f)"
 ZERO = "make_float2(0.f, 0.f)"
 
@@ -618,12 +630,22 @@ def _head(sp: StageProgram, g: _Gen) -> List[str]:
     return out
 
 
-def _amp(sp: StageProgram, arr: str, j: int, const: bool) -> str:
-    """lvalue of amplitude slot j of this lane's group in array arr."""
+def _ptr(sp: StageProgram, arr: str, j: int, const: bool) -> str:
+    """Address of amplitude slot j of this lane's group in array arr."""
+    c = "const " if const else ""
     if _split_addr(sp):
-        c = "const " if const else ""
-        return f"*({c}float2*)(({c}char*)({arr} + gbase) + {8 * sp.roff[j]}ull + toff8)"
-    return f"{arr}[base + {sp.roff[j]}]"
+        return f"({c}float2*)(({c}char*)({arr} + gbase) + {8 * sp.roff[j]}ull + toff8)"
+    return f"({arr} + base + {sp.roff[j]})"
+
+
+def _load(sp: StageProgram, arr: str, j: int, nt: bool) -> str:
+    p = _ptr(sp, arr, j, True)
+    return f"ldnt({p})" if nt else f"*{p}"
+
+
+def _store(sp: StageProgram, arr: str, j: int, val: str, nt: bool) -> str:
+    p = _ptr(sp, arr, j, False)
+    return f"stnt({p}, {val});" if nt else f"*{p} = {val};"
 
 
 def _toff(sp: StageProgram) -> str:
@@ -631,8 +653,9 @@ def _toff(sp: StageProgram) -> str:
     return " | ".join(terms) if terms else "0u"
 
 
-def gen_forward(words: np.ndarray, name: str) -> str:
-    """HIP source of one forward stage kernel (k_stage_fwd semantics)."""
+def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False) -> str:
+    """HIP source of one forward stage kernel (k_stage_fwd semantics);
+    nontemporal: the state is read and written with non-temporal hints."""
     sp = StageProgram(words)
     if sp.T < MIN_T:
         raise ValueError("compiled stages need T >= 6")
@@ -668,17 +691,18 @@ def gen_forward(words: np.ndarray, name: str) -> str:
         src.append(f"      i{j} = make_float2((goff + toff + {sp.roff[j]}u) == 0u ? 1.f : 0.f, 0.f);")
     src.append("    } else {")
     for j in range(NS):
-        src.append(f"      i{j} = {_amp(sp, 'in', j, True)};")
+        src.append(f"      i{j} = {_load(sp, 'in', j, nontemporal)};")
     src.append("    }")
     src += body
     for j in range(NS):
-        src.append(f"    {_amp(sp, 'out', j, False)} = {a[j]};")
+        src.append(f"    {_store(sp, 'out', j, a[j], nontemporal)}")
     src += ["  }", "}"]
     return "\n".join(src) + "\n"
 
 
-def gen_adjoint(words: np.ndarray, name: str) -> str:
-    """HIP source of one adjoint stage kernel (k_stage_adj semantics)."""
+def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False) -> str:
+    """HIP source of one adjoint stage kernel (k_stage_adj semantics);
+    nontemporal: psi and lam are read and written with non-temporal hints."""
     sp = StageProgram(words)
     if sp.T < MIN_T:
         raise ValueError("compiled stages need T >= 6")
@@ -739,14 +763,14 @@ def gen_adjoint(words: np.ndarray, name: str) -> str:
                 "      cur_row = (long long)row;",
                 "    }"]
     for j in range(NS):
-        src.append(f"    const float2 p{j} = {_amp(sp, 'psi_in', j, True)};")
+        src.append(f"    const float2 p{j} = {_load(sp, 'psi_in', j, nontemporal)};")
     for j in range(NS):
-        src.append(f"    const float2 l{j} = {_amp(sp, 'lam_in', j, True)};")
+        src.append(f"    const float2 l{j} = {_load(sp, 'lam_in', j, nontemporal)};")
     src += g.lines
     for j in range(NS):
-        src.append(f"    {_amp(sp, 'psi', j, False)} = {p[j]};")
+        src.append(f"    {_store(sp, 'psi', j, p[j], nontemporal)}")
     for j in range(NS):
-        src.append(f"    {_amp(sp, 'lam', j, False)} = {l[j]};")
+        src.append(f"    {_store(sp, 'lam', j, l[j], nontemporal)}")
     src.append("  }")
     if ngp:
         src += ["  if (cur_row >= 0) {",
@@ -763,9 +787,11 @@ def gen_adjoint(words: np.ndarray, name: str) -> str:
 # ----------------------------------------------------------------------------
 # compile + cache
 # ----------------------------------------------------------------------------
-def kernel_name(kind: str, words: np.ndarray) -> str:
+def kernel_name(kind: str, words: np.ndarray, nt: bool = False) -> str:
     hs = hashlib.sha1(np.ascontiguousarray(words, dtype=np.int32).tobytes())
     hs.update(COMPILE_OPTS[kind].encode())
+    if nt:
+        hs.update(b"nt")
     h = hs.hexdigest()[:20]
     return f"tqsv_j{kind}_{h}"
 
@@ -839,14 +865,14 @@ _funcs: Dict[tuple, int] = {}          # (device index, kernel name) -> hipFunct
 _modules: List[int] = []                # loaded modules (kept for the process lifetime)
 
 
-def get_kernels(programs: Sequence[np.ndarray], kind: str, device_index: int) -> List[int]:
+def get_kernels(programs: Sequence[np.ndarray], kind: str, device_index: int, nt: bool = False) -> List[int]:
     """hipFunction_t handles (as ints) of the compiled stage kernels for the
     given programs; kind "f" = forward, "a" = adjoint.  Compiles and loads
     (on the calling thread) what is not loaded yet.  hipRTC serialises
     compiles within a process, so sources are compiled one after another."""
     from . import _native as N
     gen = gen_forward if kind == "f" else gen_adjoint
-    names = [kernel_name(kind, w) for w in programs]
+    names = [kernel_name(kind, w, nt) for w in programs]
     with _lock:
         missing = {}
         for nm, w in zip(names, programs):
@@ -860,7 +886,8 @@ def get_kernels(programs: Sequence[np.ndarray], kind: str, device_index: int) ->
         for ci, ch in enumerate(chunks):
             if _shutdown.is_set():
                 raise RuntimeError("stage kernel compile cancelled at interpreter exit")
-            codes.append(compile_source(PRELUDE + "\n".join(gen(w, nm) for nm, w in ch), COMPILE_OPTS[kind]))
+            codes.append(compile_source(PRELUDE + (NT_HELPERS if nt else "") + "\n".join(gen(w, nm, nt) for nm, w in ch),
+                                        COMPILE_OPTS[kind]))
             if verbose:
                 import sys
                 print(f"[stagejit] {kind}: {min((ci + 1) * KERNELS_PER_SOURCE, len(items))}/{len(items)} stage "
@@ -890,8 +917,8 @@ def get_kernels(programs: Sequence[np.ndarray], kind: str, device_index: int) ->
 class CompileJob:
     """One plan's stage kernels being compiled on the background thread."""
 
-    def __init__(self, programs, kind, device_index):
-        self.programs, self.kind, self.device_index = list(programs), kind, device_index
+    def __init__(self, programs, kind, device_index, nt=False):
+        self.programs, self.kind, self.device_index, self.nt = list(programs), kind, device_index, nt
         self.done = threading.Event()
         self.result: Optional[List[int]] = None
         self.error: Optional[BaseException] = None
@@ -908,7 +935,7 @@ def _work():
         job = _jobs.get()
         try:
             torch.cuda.set_device(job.device_index)
-            job.result = get_kernels(job.programs, job.kind, job.device_index)
+            job.result = get_kernels(job.programs, job.kind, job.device_index, job.nt)
         except BaseException as e:   # reported to the plan that asked
             job.error = e
         finally:
@@ -918,10 +945,10 @@ def _work():
                     _pending.remove(job)
 
 
-def submit(programs: Sequence[np.ndarray], kind: str, device_index: int) -> CompileJob:
+def submit(programs: Sequence[np.ndarray], kind: str, device_index: int, nt: bool = False) -> CompileJob:
     """Queue a compile on the background thread; returns at once."""
     global _worker
-    job = CompileJob(programs, kind, device_index)
+    job = CompileJob(programs, kind, device_index, nt)
     with _lock:
         _pending.append(job)
         if _worker is None or not _worker.is_alive():
