@@ -271,11 +271,13 @@ def time_batch(n_circ, steps, tq):
         out = cb.run()
     torch.cuda.synchronize()
     t_graph = (time.perf_counter() - t0) / steps
-    # one engine call per circuit (fused plans, same kernels, Python loop over circuits)
+    # one engine call per circuit (fused plans, the same interpreting op code as
+    # the batch kernel, Python loop over circuits)
     def per_circuit():
         res = []
         for n, ops in circs:
             q = tq.QuantumDevice(n_wires=n, bsz=1, device="cuda")
+            q.engine_options = tq.EngineOptions(jit="off")
             for op in ops:
                 p = op["params"]
                 tqf.func_name_dict[op["name"]](q, op["wires"], params=None if p is None else
@@ -284,7 +286,6 @@ def time_batch(n_circ, steps, tq):
         return torch.stack(res)
     ref = per_circuit()
     torch.cuda.synchronize()
-    tq.wait_for_compiled_stages()   # large circuits' compiled stages: ready, and no compile left running
     per_circuit()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
