@@ -78,12 +78,15 @@ def time_forward(n, bsz, ops, mode, steps, warmup, tq, jit="auto"):
     return dt, out
 
 
+STAGE_BITS = 0      # local bits per stage of the gradient workload's plans (0 = the engine's choice)
+
+
 def _ansatz_loss_step(n, bsz, layers, mode, tq, jit, params, phi, ops, obs, coef):
     import torchquantum_amd.functional as tqf
     for p in params:
         p.grad = None
     q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
-    q.engine_options = tq.EngineOptions(mode=mode, jit=jit)
+    q.engine_options = tq.EngineOptions(mode=mode, jit=jit, stage_bits=STAGE_BITS)
     for w in range(n):
         tqf.ry(q, w, params=phi)
     k = 0
@@ -713,6 +716,8 @@ def main():
                     help="run only the fused forward of the 20-qubit RY/CZ circuit (--steps, --warmup)")
     ap.add_argument("--only-pergate", action="store_true",
                     help="run only the per-gate forward of the 20-qubit RY/CZ circuit (--steps, --warmup)")
+    ap.add_argument("--stage-bits", type=int, default=0,
+                    help="local bits per stage for the gradient workload (default: the engine's choice)")
     ap.add_argument("--only-gradient", action="store_true",
                     help="run only the fused adjoint gradient (--adj-qubits, --adj-bsz, --steps)")
     ap.add_argument("--only-batch", action="store_true", help="run only the uneven batch")
@@ -731,6 +736,8 @@ def main():
         sys.exit(2)
     import torchquantum_amd as tq
     from circuits import config2_ops
+    global STAGE_BITS
+    STAGE_BITS = args.stage_bits
 
     if args.only_roofline:
         cb, cms = copy_calibration()
@@ -758,6 +765,7 @@ def main():
         amps = float(ab) * (1 << an)
         print(json.dumps({"fused_gradient_s": dt, "workload": f"{an} qubits, bsz {ab}", "loss": loss,
                           "adjoint_stage": stage_launch_stats(rec, args.steps, "a", amps, 32.0),
+                          "final_adjoint_stage": stage_launch_stats(rec, args.steps, "a_read", amps, 16.0),
                           "forward_stage": stage_launch_stats(rec, args.steps, "f", amps, 16.0)}), flush=True)
         return
     if args.only_fused or args.only_pergate:
@@ -780,6 +788,7 @@ def main():
     dt_af, loss_f, g_f = time_gradient(an, ab, 10, "fused", args.steps, args.warmup, tq, timer=rec)
     log(f"[bench] fused {dt_af:.4f} s/gradient")
     adj = stage_launch_stats(rec, args.steps, "a", amps, 32.0)
+    adj_last = stage_launch_stats(rec, args.steps, "a_read", amps, 16.0)
     fwd = stage_launch_stats(rec, args.steps, "f", amps, 16.0)
     del rec
     torch.cuda.empty_cache()
@@ -802,6 +811,10 @@ def main():
         "fused_interpreting_stage_kernels": {"grads_per_s": 1.0 / dt_ai, "s_per_gradient": dt_ai},
         "forward_stage": None if fwd is None else {**fwd, "frac": fwd["achieved_GBs"] / HBM_PEAK_GBS,
                                                    "kernel": "compiled forward stage tqsv_jf_* (16 B/amplitude)"},
+        "final_adjoint_stage": None if adj_last is None else {
+            **adj_last, "frac": adj_last["achieved_GBs"] / HBM_PEAK_GBS,
+            "kernel": "last stage of the pruned reverse sweep (no input-state gradient): reads psi and lam, "
+                      "stores nothing (16 B/amplitude)"},
         "dense_gate": {"kernel": "dense 1-qubit gate k_1q_t0/k_1q_tile/k_1q_pair, RY cycling over every target "
                                  f"bit, {an} qubits bsz {ab} ({nbytes_d / 2**30:.0f} GiB, HBM-resident)",
                        "avg_launch_ms": ms_d, "achieved": nbytes_d / (ms_d * 1e-3) / 1e9,

@@ -28,7 +28,10 @@ def _op_mat(mats, w, row):
     return mats[off:]
 
 
-def stage_apply(words, st, mats, n, init_zero=False, adjoint=False, lam=None, grad=None):
+def stage_apply(words, st, mats, n, init_zero=False, adjoint=False, lam=None, grad=None, skip=(), no_store=False):
+    """One stage over every group.  Adjoint: skip = program positions whose
+    un-apply is left out (their gradient partials still count); no_store =
+    psi and lam are not written back (the pruned kernels of stagejit)."""
     h = words
     n_ops, m, T, nnl = int(h[E.H_NOPS]), int(h[E.H_M]), int(h[E.H_T]), int(h[E.H_NNL])
     tpos = [int(v) for v in h[E.H_POS:E.H_POS + T]]
@@ -82,12 +85,16 @@ def stage_apply(words, st, mats, n, init_zero=False, adjoint=False, lam=None, gr
                 if not adjoint:
                     a = _fwd(typ, a, M, w, tok, rm, rv, tid)
                 else:
-                    a, l, g = _bwd(typ, a, l, M, w, tok, rm, rv, tid, P)
+                    a2, l2, g = _bwd(typ, a, l, M, w, tok, rm, rv, tid, P)
+                    if o not in skip:
+                        a, l = a2, l2
                     for q in range(P):
                         gacc[w[7] + q] += g[q]
-            st[row, idx] = a
+            if not no_store:
+                st[row, idx] = a
             if adjoint:
-                lam[row, idx] = l
+                if not no_store:
+                    lam[row, idx] = l
                 for t, s in enumerate(gslot):
                     grad[row, s] += gacc[t]
 
@@ -339,14 +346,21 @@ def run_compiled(c: E.Compiled, pool: np.ndarray, state_in, init_zero=False):
     return st
 
 
-def adjoint_compiled(c: E.Compiled, pool: np.ndarray, psi, lam):
-    """Reverse sweep; mirrors engine.adjoint.  Returns grads [bsz, n_slots]."""
+def adjoint_compiled(c: E.Compiled, pool: np.ndarray, psi, lam, need_input_grad=True):
+    """Reverse sweep; mirrors engine.adjoint (without need_input_grad: the
+    pruned sweep of Compiled.prune_info, as the pruned compiled kernels run
+    it).  Returns grads [bsz, n_slots]."""
     n, bsz = c.n, c.bsz
     grad = np.zeros((bsz, c.grad_width))
-    for kind, payload in reversed(c.steps):
+    steps = list(enumerate(c.steps))
+    stop, skip = 0, {}
+    if not need_input_grad:
+        stop, skip = c.prune_info()
+    for si, (kind, payload) in reversed(steps[stop:]):
         if kind == "stage":
             base, stg = payload
-            stage_apply(c.prog_host[base:], psi, pool, n, adjoint=True, lam=lam, grad=grad)
+            stage_apply(c.prog_host[base:], psi, pool, n, adjoint=True, lam=lam, grad=grad,
+                        skip=frozenset(skip.get(si, ())), no_store=not need_input_grad and si == stop)
             continue
         k = c.kops[payload]
         if k.P > 0:

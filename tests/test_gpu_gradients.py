@@ -155,3 +155,39 @@ def test_adjoint_vs_parameter_shift_full_size():
             fm, _ = _engine_loss_and_grad(n, bsz, ops, obs, coef, phi, tm, opts, want_grad=False)
         gps = (fp - fm) / 2
         assert abs(g[i] - gps) <= 2e-5 + 1e-4 * abs(gps), (int(i), float(g[i]), gps)
+
+
+@gpu
+def test_pruned_sweep_matches_full_sweep():
+    """Compiled stages of a 12-qubit bsz-3 circuit with an untrained input
+    layer and fixed gates ahead of a trainable random part: the reverse sweep
+    without the input-state gradient (stopped at the first trainable step,
+    un-applies of ops on untouched wires left out, nothing stored by its last
+    stage) gives the gradients of the full sweep, |dg| <= 1e-5 (1 + |g|)."""
+    from circuits import random_ops
+    from torchquantum_amd import engine as E, gates as G
+    rng = np.random.default_rng(11)
+    n, bsz = 12, 3
+    ops = [dict(name="ry", wires=[w], params=rng.uniform(0, np.pi, size=(bsz, 1)), inverse=False) for w in range(n)]
+    ops += [dict(name="hadamard", wires=[3], params=None, inverse=False),
+            dict(name="cnot", wires=[7, 9], params=None, inverse=False)]
+    train = random_ops(n, 40, bsz, rng, names=["rx", "ry", "rz", "u3", "cnot", "cz", "crx", "rzz", "rot", "swap",
+                                                 "toffoli", "phaseshift"])
+    tape = []
+    for i, op in enumerate(ops + train):
+        p = op["params"]
+        if p is not None:
+            p = torch.tensor(p, dtype=torch.float32, device=DEV, requires_grad=i >= len(ops))
+        tape.append(E.TapeOp(G.canonical(op["name"]), tuple(op["wires"]), p, op["inverse"]))
+    c = E.Compiled(tape, n, bsz, E.EngineOptions(mode="fused", stage_bits=10, jit="on"), DEV)
+    stop, skip = c.prune_info()
+    assert skip and c.jit_kernels("a") and c.jit_kernels("p")
+    pool = c.pool(tape, True, True)
+    out = E.forward(c, None, pool, init_zero=True)
+    g = torch.Generator(device=DEV).manual_seed(5)
+    lam = torch.randn(bsz, 1 << n, dtype=torch.complex64, device=DEV, generator=g)
+    full, lam_out = E.adjoint(c, out, lam, pool, need_input_grad=True)
+    pruned, none = E.adjoint(c, out, lam, pool, need_input_grad=False)
+    assert none is None and lam_out is not None
+    full, pruned = full.double().cpu().numpy(), pruned.double().cpu().numpy()
+    assert np.all(np.abs(pruned - full) <= 1e-5 * (1 + np.abs(full))), float(np.abs(pruned - full).max())

@@ -502,3 +502,53 @@ def test_host_mirror_invalidation():
     assert m.get(base[:3].unsqueeze(0)) is not None   # a fresh view object of the same elements
     base.mul_(2.0)
     assert m.get(v1) is None
+
+
+def _pruning_case(seed):
+    """A circuit whose first ops touch wires no trainable op has touched yet:
+    an input layer (per-row RY, no gradient), fixed gates, then a trainable
+    random part; fused plan with stages of 10 local bits (compiled-stage
+    eligible)."""
+    rng = np.random.default_rng(700 + seed)
+    n, bsz = 11, 2
+    ops = [dict(name="ry", wires=[w], params=rng.uniform(0, np.pi, size=(bsz, 1)), inverse=False) for w in range(n)]
+    ops += [dict(name="hadamard", wires=[3], params=None, inverse=False),
+            dict(name="cnot", wires=[7, 9], params=None, inverse=False)]
+    train = random_ops(n, 30, bsz, rng, names=["rx", "ry", "rz", "u3", "cnot", "cz", "crx", "rzz", "rot", "swap",
+                                                 "toffoli", "phaseshift"])
+    tape = _tape_from_ops(ops) + _tape_from_ops(train, grad=True)
+    c = E.Compiled(tape, n, bsz, E.EngineOptions(mode="fused", stage_bits=10), "cpu")
+    return rng, n, bsz, tape, c
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_pruned_adjoint_emulated(seed):
+    """Without an input-state gradient the sweep stops at the first
+    trainable step and leaves out the un-apply of ops whose wires no earlier
+    trainable or un-applied op touches (Compiled.prune_info).  The emulated
+    pruned sweep gives the gradients of the full sweep; skipping one op
+    more than the analysis allows does not."""
+    rng, n, bsz, tape, c = _pruning_case(seed)
+    stop, skip = c.prune_info()
+    assert skip and sum(len(v) for v in skip.values()) >= n     # at least the input layer is left applied
+    pool = c.pool_host(tape, True, True).astype(np.complex128)
+    psi = EM.run_compiled(c, pool, None, init_zero=True)
+    cvec = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+    full = EM.adjoint_compiled(c, pool, psi.copy(), cvec.copy())
+    pruned = EM.adjoint_compiled(c, pool, psi.copy(), cvec.copy(), need_input_grad=False)
+    # the pool holds complex64 matrices, so U^dag U = 1 only to ~1e-7
+    assert np.abs(pruned - full).max() < 1e-6 * (1 + np.abs(full).max())
+    # planted mistakes: leaving out, in addition, the un-apply of an op the
+    # analysis keeps changes the gradients (the first kept op, in sweep order,
+    # that does so within a few tries)
+    kept = [(si, pos) for si, (kind, payload) in enumerate(c.steps) if kind == "stage" and si >= stop
+            for pos, (ki, _) in enumerate(payload[1].enc) if ki >= 0 and pos not in skip.get(si, ())]
+    worst = 0.0
+    for si, pos in list(reversed(kept))[:8]:
+        c._prune = (stop, {**skip, si: tuple(sorted(set(skip.get(si, ())) | {pos}))})
+        wrong = EM.adjoint_compiled(c, pool, psi.copy(), cvec.copy(), need_input_grad=False)
+        worst = max(worst, float(np.abs(wrong - full).max()))
+        if worst > 1e-3:
+            break
+    c._prune = None
+    assert worst > 1e-3

@@ -93,7 +93,8 @@ def test_generated_code_follows_program():
         sp = J.StageProgram(s.words)
         src = J.gen_forward(s.words, "k")
         n_remap = sum(1 for w in sp.ops if (w[0] & 15) == E.OP_REMAP)
-        assert src.count("__syncthreads();") == 2 * n_remap
+        # the body appears twice: the |0..0>-start loop and the loop that reads
+        assert src.count("__syncthreads();") == 4 * n_remap
         assert "csel" not in src
     # a lone X on wire 0 (bit 11, a register bit of the canonical layout):
     # no instructions, the stores name the loaded values exchanged in pairs
@@ -106,8 +107,9 @@ def test_generated_code_follows_program():
     K = sp.ops[0][3]
     src = J.gen_forward(s.words, "k")
     assert "const float2 v" not in src
+    mode = J._mem_mode(sp, False, J.CODEGEN["buf"])
     for j in range(16):
-        assert J._store(sp, "out", j, f"i{j ^ (1 << K)}", False) in src
+        assert J._store(sp, "out", j, f"i{j ^ (1 << K)}", mode) in src
 
 
 def test_code_object_disk_cache(tmp_path, monkeypatch):
@@ -122,3 +124,38 @@ def test_code_object_disk_cache(tmp_path, monkeypatch):
     assert J.compile_source(src) == b"cached"
     monkeypatch.setenv("TQSV_JIT_CACHE", "off")
     assert J.compile_source(src) == a
+
+
+def test_pruned_adjoint_code():
+    """Pruned adjoint kernels (engine.Compiled.prune_info): the ops at the
+    skip positions keep their gradient partials but are not un-applied, and
+    the sweep's last stage stores nothing; both compile for gfx950."""
+    import torchquantum_amd as tq
+    import torchquantum_amd.functional as tqf
+    from torchquantum_amd import engine as E, stagejit as J
+    n, bsz = 12, 2
+    q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cpu")
+    for w in range(n):
+        tqf.ry(q, w, params=torch.rand(bsz, 1))                       # input layer, no gradient
+    for w in range(n):
+        tqf.rx(q, w, params=torch.rand(1, 1, requires_grad=True))
+    for w in range(n - 1):
+        tqf.cnot(q, [w, w + 1])
+    for w in range(n):
+        tqf.ry(q, w, params=torch.rand(1, 1, requires_grad=True))
+    c = E.Compiled(q._tape, n, bsz, tq.EngineOptions(), "cpu")
+    stop, skip = c.prune_info()
+    assert skip and stop < len(c.steps)
+    si = min(skip)
+    st = c.steps[si][1][1]
+    full = J.gen_adjoint(st.words, "kf")
+    pruned = J.gen_adjoint(st.words, "kp", skip=skip[si], no_store=(si == stop))
+    # the merged input + first-layer rotations keep their sums (same number
+    # of gradient accumulations) but lose their U^dagger products
+    assert pruned.count("atomicAdd(&gacc") == full.count("atomicAdd(&gacc")
+    assert pruned.count("cmulc(") < full.count("cmulc(")
+    if si == stop:
+        assert "tq_st(" not in pruned and "tq_st(" in full
+    assert J.kernel_name("p", (st.words, skip[si], si == stop)) != J.kernel_name("a", st.words)
+    code = J.compile_source(J.PRELUDE + pruned, J.COMPILE_OPTS["p"])
+    assert len(code) > 1000

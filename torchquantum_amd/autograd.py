@@ -46,7 +46,7 @@ class CircuitFunction(torch.autograd.Function):
         payload = ctx.payload
         c = payload.c
         lam_in = gout.to(torch.complex64).reshape(out.shape).contiguous()
-        grad, lam = E.adjoint(c, out, lam_in, ctx.pool)
+        grad, lam = E.adjoint(c, out, lam_in, ctx.pool, need_input_grad=ctx.needs_input_grad[1])
         gsum = None            # batch-row sums for parameters shared by all rows, one reduction
         grads: List = []
         for j, ti in enumerate(payload.param_idx):

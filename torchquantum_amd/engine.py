@@ -1227,6 +1227,7 @@ class Compiled:
         self._gv_layout: Optional[_GVLayout] = None
         self._jit: Dict[str, Optional[Dict[int, int]]] = {}
         self._jit_jobs: Dict[str, object] = {}
+        self._prune: Optional[tuple] = None
         self.mg_items = np.zeros(0, dtype=MG_ITEM)
         self.grad_width = max(self.n_slots, 1)
         if opts.mode == "fused" and n >= R_BITS:
@@ -1315,8 +1316,14 @@ class Compiled:
             self._jit[kind] = None
             return None
         from . import stagejit
-        stages = [(base, st.words) for kind_, (base, st) in
-                  [(k, p) for k, p in self.steps if k == "stage"] if stagejit.supported(st.words)]
+        if kind == "p":
+            stop, skip = self.prune_info()
+            stages = [(base, (st.words, skip.get(si, ()), si == stop)) for si, (kind_, (base, st)) in
+                      [(i, (k, p)) for i, (k, p) in enumerate(self.steps) if k == "stage"]
+                      if stagejit.supported(st.words) and (si in skip or si == stop)]
+        else:
+            stages = [(base, st.words) for kind_, (base, st) in
+                      [(k, p) for k, p in self.steps if k == "stage"] if stagejit.supported(st.words)]
         if not stages:
             self._jit[kind] = None
             return None
@@ -1340,6 +1347,53 @@ class Compiled:
         table = None if funcs is None else {base: f for (base, _), f in zip(stages, funcs)}
         self._jit[kind] = table
         return table
+
+    def prune_info(self) -> tuple:
+        """How the reverse sweep shrinks when no gradient with respect to the
+        input state is wanted: (stop, skip).  stop: index in self.steps of
+        the first step, in forward order, holding a trainable op -- the sweep
+        ends with it (nothing earlier is run).  skip: {stage step index:
+        program positions of ops whose un-apply is left out}.
+
+        An op's un-apply is needed only when an earlier op (forward order)
+        that is trainable or itself un-applied shares a wire with it.  The
+        others touch wires that no such op uses: they commute with every
+        generator and every un-applied op left in the sweep, so leaving them
+        applied to both psi and lam changes none of the sums
+        Im<lam|H_p|psi>.  Ops of per-gate steps and of stages without
+        compiled kernels are always un-applied."""
+        if self._prune is not None:
+            return self._prune
+        from . import stagejit
+        used: set = set()          # wires of trainable or un-applied ops seen so far
+        stop = None
+        skip: Dict[int, tuple] = {}
+        for si, (kind, payload) in enumerate(self.steps):
+            if kind == "solo":
+                k = self.kops[payload]
+                if k.P and stop is None:
+                    stop = si
+                used |= k.allbits
+                continue
+            _, st = payload
+            ok = stagejit.supported(st.words)
+            left = []
+            for pos, (ki, _) in enumerate(st.enc):
+                if ki < 0:
+                    continue
+                k = self.kops[ki]
+                if k.P and stop is None:
+                    stop = si
+                if ok and not (k.allbits & used):
+                    left.append(pos)
+                    if k.P:
+                        used |= k.allbits
+                else:
+                    used |= k.allbits
+            if left:
+                skip[si] = tuple(left)
+        self._prune = (len(self.steps) if stop is None else stop, skip)
+        return self._prune
 
     # ---------------------------------------------------------------- values
     def pool(self, tape: Sequence[TapeOp], with_grad: bool, with_adj: bool) -> torch.Tensor:
@@ -1647,18 +1701,31 @@ def forward(c: Compiled, state_in: torch.Tensor, pool: torch.Tensor, init_zero: 
     return out
 
 
-def adjoint(c: Compiled, psi_src: torch.Tensor, lam_src: torch.Tensor, pool: torch.Tensor):
+def adjoint(c: Compiled, psi_src: torch.Tensor, lam_src: torch.Tensor, pool: torch.Tensor,
+            need_input_grad: bool = True):
     """Reverse sweep from psi_src (forward output) and lam_src (dL/dpsi),
     which are left unchanged.  Returns (grads [bsz, n_slots] float32, lam
-    after the sweep = dL/d(input state)).  When the sweep starts with a
-    compiled stage, that stage reads psi_src / lam_src and writes fresh
-    buffers; otherwise they are copied first."""
+    after the sweep = dL/d(input state), or None without need_input_grad).
+    When the sweep starts with a compiled stage, that stage reads psi_src /
+    lam_src and writes fresh buffers; otherwise they are copied first.
+    Without need_input_grad the sweep stops at the first trainable step and,
+    once their kernels are compiled, runs the pruned stages of
+    Compiled.prune_info."""
     n, bsz = c.n, c.bsz
     grad = torch.zeros((bsz, c.grad_width), dtype=torch.float32, device=c.device)
     stream = N.stream_of(psi_src)
     gstride = grad.shape[1]
     jit = c.jit_kernels("a") if c.prog_host is not None else None
     steps = list(reversed(c.steps))
+    pj = None
+    stop = 0
+    if not need_input_grad and c.prog_host is not None:
+        stop, _ = c.prune_info()
+        steps = steps[:len(c.steps) - stop]
+        pj = c.jit_kernels("p")          # (queued behind the full kernels' compile when it runs in the background)
+        if not jit:
+            pj = None
+    last_base = c.steps[stop][1][0] if stop < len(c.steps) and c.steps[stop][0] == "stage" else None
     first_src = None
     if steps and steps[0][0] == "stage" and jit and jit.get(steps[0][1][0]) is not None:
         psi, lam = torch.empty_like(psi_src), torch.empty_like(lam_src)
@@ -1670,7 +1737,11 @@ def adjoint(c: Compiled, psi_src: torch.Tensor, lam_src: torch.Tensor, pool: tor
             base, stg = payload
             hp = c.prog_host[base:]
             f = jit.get(base) if jit else None
-            tok = _timed("a", f is not None)
+            read_only = False
+            if pj and base in pj:
+                f = pj[base]
+                read_only = base == last_base     # the pruned sweep's last stage stores nothing
+            tok = _timed("a_read" if read_only else "a", f is not None)
             if f is not None:
                 pin, lin = first_src if first_src is not None else (None, None)
                 first_src = None
@@ -1717,4 +1788,4 @@ def adjoint(c: Compiled, psi_src: torch.Tensor, lam_src: torch.Tensor, pool: tor
     for i, col, sl in c.scatter:
         idx = torch.as_tensor(sl, dtype=torch.long, device=grad.device)
         grad.index_add_(1, idx, grad[:, col:col + len(sl)].clone())
-    return grad[:, :max(c.n_slots, 1)], lam
+    return grad[:, :max(c.n_slots, 1)], (lam if need_input_grad else None)

@@ -45,6 +45,16 @@ KERNELS_PER_SOURCE = 4
 # to 160-190 VGPRs.  Measured on the 24-qubit gradient: 531 -> 500 ms.  The
 # forward kernels gain nothing from it.
 COMPILE_OPTS = {"f": "", "a": "-mllvm -amdgpu-sched-strategy=iterative-maxocc"}
+COMPILE_OPTS["p"] = COMPILE_OPTS["a"]     # pruned adjoint stages (gen_adjoint with skip / no_store)
+# Code-generation defaults (gen_forward / gen_adjoint arguments of the same
+# names), chosen by timing the stages of the 24-qubit bsz-64 gradient on
+# MI355X: buffer instructions for the state (no per-access 64-bit address
+# arithmetic), the padded remap buffer (one address register per layout
+# instead of one per slot: 99 instead of 142 VGPRs on an adjoint stage, so
+# more waves per SIMD), next-group loads issued ahead in forward stages only
+# (in adjoint stages their 64 registers cost more occupancy than the overlap
+# gains), no register accumulation of gradient partials (same trade).
+CODEGEN = {"buf": True, "prefetch_f": True, "prefetch_a": False, "lds": "pad4", "racc": 0}
 
 PRELUDE = r"""
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -66,6 +76,20 @@ __device__ __forceinline__ float2 cfmac(float2 a, float2 b, float2 acc) {
   acc.y = fmaf(a.x, b.y, acc.y);
   acc.y = fmaf(-a.y, b.x, acc.y);
   return acc;
+}
+// Buffer loads and stores of the state: a scalar resource (base address of a
+// group), the lane's byte offset and a constant scalar offset per slot.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tq_rsrc(const float2* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(p), (short)0, -1, 0x00020000);
+}
+__device__ __forceinline__ float2 tq_ld(__amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0);
+  return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+__device__ __forceinline__ void tq_st(__amdgpu_buffer_rsrc_t r, unsigned vo, int so, float2 x) {
+  typedef unsigned tq_v2u __attribute__((__vector_size__(8)));
+  const tq_v2u v = {__float_as_uint(x.x), __float_as_uint(x.y)};
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, vo, so, 0);
 }
 __device__ __forceinline__ float redot(float2 a, float2 b) { return fmaf(a.x, b.x, a.y * b.y); }
 __device__ __forceinline__ float2 csel(bool c, float2 a, float2 b) {
@@ -116,6 +140,22 @@ def swz(l: int) -> int:
     return l ^ ((l >> 5) & 31) ^ ((l >> 10) & 31)
 
 
+PAD_SHIFT = {"pad4": 4, "pad5": 5}
+
+
+def pad_index(l: int, mode: str) -> int:
+    """Padded LDS index: one spare float2 after every 16 ("pad4") or 32
+    ("pad5").  Additive over local indices with disjoint bits (pad(t | r) =
+    pad(t) + pad(r)), so a lane's part is one register and each slot's part
+    is the constant offset of the LDS instruction."""
+    return l + (l >> PAD_SHIFT[mode])
+
+
+def lds_words(m: int, mode: str) -> int:
+    """float2 entries of the remap buffer of a 2^m-amplitude group."""
+    return pad_index((1 << m) - 1, mode) + 1 if mode in PAD_SHIFT else 1 << m
+
+
 class StageProgram:
     """Decoded stage words (format: csrc/tqsv_stage.hip)."""
 
@@ -159,15 +199,26 @@ def supported(words: np.ndarray) -> bool:
 
 
 class _Gen:
-    def __init__(self, sp: StageProgram):
+    def __init__(self, sp: StageProgram, lds_mode: str = "xor"):
         self.sp = sp
         self.lines: List[str] = []
         self.k = 0
         self.ind = 2
         self.addr: Dict[tuple, str] = {}     # thread layout -> hoisted LDS index variable
+        if lds_mode not in ("xor", "pad4", "pad5"):
+            raise ValueError(f"unknown LDS layout {lds_mode!r}")
+        self.lds_mode = lds_mode
 
     def line(self, s: str):
         self.lines.append("  " * self.ind + s)
+
+    def detached(self, fn) -> List[str]:
+        """Lines that fn() emits, returned instead of appended to the body
+        (variable numbering continues, so names stay unique)."""
+        saved, self.lines = self.lines, []
+        fn()
+        out, self.lines = self.lines, saved
+        return [x.strip() for x in out]
 
     def var(self, expr: str, ty: str = "float2") -> str:
         name = f"v{self.k}"
@@ -208,9 +259,19 @@ class _Gen:
     def hoisted(self) -> List[str]:
         out = []
         for key, name in self.addr.items():
-            terms = [f"((tid & {1 << i}) ? {swz(int(key[i]))}u : 0u)" for i in range(len(key))]
-            out.append(f"  const unsigned {name} = {' ^ '.join(terms) if terms else '0u'};")
+            if self.lds_mode in PAD_SHIFT:
+                terms = [f"((tid & {1 << i}) ? {pad_index(int(key[i]), self.lds_mode)}u : 0u)"
+                         for i in range(len(key))]
+                out.append(f"  const unsigned {name} = {' + '.join(terms) if terms else '0u'};")
+            else:
+                terms = [f"((tid & {1 << i}) ? {swz(int(key[i]))}u : 0u)" for i in range(len(key))]
+                out.append(f"  const unsigned {name} = {' ^ '.join(terms) if terms else '0u'};")
         return out
+
+    def _slot(self, ra: str, lr: int) -> str:
+        if self.lds_mode in PAD_SHIFT:
+            return f"{ra} + {pad_index(lr, self.lds_mode)}u"
+        return f"{ra} ^ {swz(lr)}u"
 
     def remap(self, a: List[str], frm: Sequence[int], to: Sequence[int], lds: str = "lds") -> List[str]:
         T = self.sp.T
@@ -221,7 +282,7 @@ class _Gen:
             for k in range(R_BITS):
                 if (j >> k) & 1:
                     lr |= frm[T + k]
-            self.line(f"{lds}[{ra} ^ {swz(lr)}u] = {a[j]};")
+            self.line(f"{lds}[{self._slot(ra, lr)}] = {a[j]};")
         self.line("__syncthreads();")
         out = []
         for j in range(NS):
@@ -229,7 +290,7 @@ class _Gen:
             for k in range(R_BITS):
                 if (j >> k) & 1:
                     lr |= to[T + k]
-            out.append(self.var(f"{lds}[{rb} ^ {swz(lr)}u]"))
+            out.append(self.var(f"{lds}[{self._slot(rb, lr)}]"))
         return out
 
     # ---- forward ops (k_stage_fwd's fwd_op)
@@ -418,7 +479,7 @@ class _Gen:
                 out.append(self.var(f"{tk} ? {x} : 0.f", "float"))
         return out
 
-    def b_mat1(self, w, p: List[str], l: List[str], P: int) -> List[str]:
+    def b_mat1(self, w, p: List[str], l: List[str], P: int, update: bool = True) -> List[str]:
         K = w[3]
         rm, rv = self.reg_ctrl(w)
         M = self.mat_base(w)
@@ -439,15 +500,16 @@ class _Gen:
                 v0 = self.var(f"cfma({D[1]}, {y1}, cmul({D[0]}, {y0}))")
                 v1 = self.var(f"cfma({D[3]}, {y1}, cmul({D[2]}, {y0}))")
                 g[q] = self.acc(g[q], f"(redot({l0}, {v0}) + redot({l1}, {v1}))")
-            n0 = self.var(f"cfmac({u[2]}, {l1}, cmulc({u[0]}, {l0}))")
-            n1 = self.var(f"cfmac({u[3]}, {l1}, cmulc({u[1]}, {l0}))")
-            p[j], p[j1], l[j], l[j1] = y0, y1, n0, n1
+            if update:
+                n0 = self.var(f"cfmac({u[2]}, {l1}, cmulc({u[0]}, {l0}))")
+                n1 = self.var(f"cfmac({u[3]}, {l1}, cmulc({u[1]}, {l0}))")
+                p[j], p[j1], l[j], l[j1] = y0, y1, n0, n1
         return self.lane_gate(tk, g)
 
-    def b_mat1_mg(self, w, p: List[str], l: List[str]) -> List[str]:
+    def b_mat1_mg(self, w, p: List[str], l: List[str], update: bool = True) -> List[str]:
         """Hermitian-generator form (see k_mg_to_grad): the four sums
         S = Im<lam|s|x>, s = I, X, Y, Z, over x = psi before the un-apply,
-        12 FMA per pair, then psi <- U^dag psi, lam <- U^dag lam."""
+        12 FMA per pair, then (update) psi <- U^dag psi, lam <- U^dag lam."""
         K = w[3]
         nc = bool(w[0] & F_NOCTRL)
         rm, rv = (0, 0) if nc else self.reg_ctrl(w)
@@ -466,6 +528,8 @@ class _Gen:
             b = self.var(f"fmaf({l1}.x, {x1}.y, fmaf(-{l1}.y, {x1}.x, {b if b else '0.f'}))", "float")
             c = self.var(f"cfmac({l0}, {x1}, {c if c else ZERO})")
             d = self.var(f"cfmac({l1}, {x0}, {d if d else ZERO})")
+            if not update:
+                continue
             y0 = self.var(f"cfmac({u[2]}, {x1}, cmulc({u[0]}, {x0}))")
             y1 = self.var(f"cfmac({u[3]}, {x1}, cmulc({u[1]}, {x0}))")
             n0 = self.var(f"cfmac({u[2]}, {l1}, cmulc({u[0]}, {l0}))")
@@ -476,7 +540,7 @@ class _Gen:
         g = [f"{a} + {b}", f"{c}.y + {d}.y", f"{d}.x - {c}.x", f"{a} - {b}"]
         return self.lane_gate(tk, g)
 
-    def b_mat2(self, w, p: List[str], l: List[str], P: int) -> List[str]:
+    def b_mat2(self, w, p: List[str], l: List[str], P: int, update: bool = True) -> List[str]:
         KH, KL = w[3] & 15, (w[3] >> 4) & 15
         rm, rv = self.reg_ctrl(w)
         M = self.mat_base(w)
@@ -504,6 +568,8 @@ class _Gen:
                     v = self.var(e)
                     t = self.acc(t, f"redot({lm[r]}, {v})")
                 g[q] = self.acc(g[q], t)
+            if not update:
+                continue
             for r in range(4):
                 e = f"cmulc({u[r]}, {lm[0]})"
                 for c in range(1, 4):
@@ -512,7 +578,7 @@ class _Gen:
                 p[idx[r]] = y[r]
         return self.lane_gate(tk, g)
 
-    def b_diag(self, w, p: List[str], l: List[str], P: int) -> List[str]:
+    def b_diag(self, w, p: List[str], l: List[str], P: int, update: bool = True) -> List[str]:
         rm, rv = self.reg_ctrl(w)
         rpm = (w[6] >> 16) & 255
         M = self.mat_base(w)
@@ -538,30 +604,35 @@ class _Gen:
             for q in range(P):
                 d = dB[q] if odd else dA[q]
                 g[q] = self.acc(g[q], f"redot({lj}, cmul({d}, {y}))")
-            p[j] = y
-            l[j] = self.var(f"cmulc({f}, {lj})")
+            if update:
+                p[j] = y
+                l[j] = self.var(f"cmulc({f}, {lj})")
         return self.lane_gate(tk, g)
 
-    def bwd_op(self, w, p: List[str], l: List[str]) -> Optional[List[str]]:
+    def bwd_op(self, w, p: List[str], l: List[str], update: bool = True) -> Optional[List[str]]:
+        """One op of the reverse sweep: its gradient partials (None for ops
+        without parameters) and, with update, psi and lam un-applied."""
         t = w[0] & 15
         fl = w[0]
         P = ((fl >> 4) & 15) if w[7] >= 0 else 0
+        if not update and w[7] < 0:
+            return None
         if t == OP_MAT1:
             if fl & F_XPERM:
                 self.f_xperm(w, p)
                 self.f_xperm(w, l)
                 return None
             if fl & F_MG:
-                return self.b_mat1_mg(w, p, l)
-            return self.b_mat1(w, p, l, P)
+                return self.b_mat1_mg(w, p, l, update)
+            return self.b_mat1(w, p, l, P, update)
         if t == OP_DIAG:
             if fl & F_SIGN:
                 self.f_sign(w, p)
                 self.f_sign(w, l)
                 return None
-            return self.b_diag(w, p, l, P)
+            return self.b_diag(w, p, l, P, update)
         if t == OP_MAT2:
-            return self.b_mat2(w, p, l, P)
+            return self.b_mat2(w, p, l, P, update)
         if t == OP_MATK:
             self.f_matk(w, p, adjoint=True)
             self.f_matk(w, l, adjoint=True)
@@ -616,36 +687,59 @@ def _split_addr(sp: StageProgram) -> bool:
     return max(sp.tpos, default=0) <= 28
 
 
-def _head(sp: StageProgram, g: _Gen) -> List[str]:
-    """Group-loop prologue: row, group offset, base index."""
-    terms = [f"((unsigned)((rest >> {i}) & 1ull) << {sp.nlpos[i]})" for i in range(sp.nnl)]
+def _buf_ok(sp: StageProgram) -> bool:
+    """Buffer instructions address base + 32-bit lane offset + 32-bit scalar
+    offset: every in-group byte offset 8 * (toff + roff) must stay below 2^31,
+    i.e. no local bit above 27."""
+    return max(list(sp.tpos) + list(sp.rpos), default=0) <= 27
+
+
+def _mem_mode(sp: StageProgram, nontemporal: bool, buf: bool) -> str:
+    """How the state is addressed: "buf" (buffer loads/stores: one scalar
+    resource per array and group, the lane offset toff8 and a scalar constant
+    per slot, no per-access vector address arithmetic), "nt" (pointers with
+    non-temporal hints) or "ptr" (pointers)."""
+    if nontemporal:
+        return "nt"
+    return "buf" if buf and _buf_ok(sp) else "ptr"
+
+
+def _group_vars(sp: StageProgram, gv: str, sfx: str, mode: str, arrays: Sequence[str]) -> List[str]:
+    """Row, group offset and base index of group `gv` (names suffixed by
+    `sfx`), and what addressing `mode` needs for each array."""
+    terms = [f"((unsigned)(({gv} >> {i}) & 1ll) << {sp.nlpos[i]})" for i in range(sp.nnl)]
     out = [
-        f"    const unsigned long long row = (unsigned long long)g >> {sp.nnl};",
-        f"    const unsigned long long rest = (unsigned long long)g & {(1 << sp.nnl) - 1}ull;",
-        f"    const unsigned goff = {' | '.join(terms) if terms else '0u'};",
-        f"    const unsigned long long gbase = (row << {sp.n}) + goff;",
+        f"const unsigned long long row{sfx} = (unsigned long long)({gv}) >> {sp.nnl};",
+        f"const unsigned goff{sfx} = {' | '.join(terms) if terms else '0u'};",
+        f"const unsigned long long gbase{sfx} = (row{sfx} << {sp.n}) + goff{sfx};",
     ]
-    if not _split_addr(sp):
-        out.append("    const unsigned long long base = gbase + toff;")
+    if mode == "buf":
+        out += [f"const __amdgpu_buffer_rsrc_t r_{a}{sfx} = tq_rsrc({a} + gbase{sfx});" for a in arrays]
+    elif not _split_addr(sp):
+        out.append(f"const unsigned long long base{sfx} = gbase{sfx} + toff;")
     return out
 
 
-def _ptr(sp: StageProgram, arr: str, j: int, const: bool) -> str:
+def _ptr(sp: StageProgram, arr: str, j: int, const: bool, sfx: str = "") -> str:
     """Address of amplitude slot j of this lane's group in array arr."""
     c = "const " if const else ""
     if _split_addr(sp):
-        return f"({c}float2*)(({c}char*)({arr} + gbase) + {8 * sp.roff[j]}ull + toff8)"
-    return f"({arr} + base + {sp.roff[j]})"
+        return f"({c}float2*)(({c}char*)({arr} + gbase{sfx}) + {8 * sp.roff[j]}ull + toff8)"
+    return f"({arr} + base{sfx} + {sp.roff[j]})"
 
 
-def _load(sp: StageProgram, arr: str, j: int, nt: bool) -> str:
-    p = _ptr(sp, arr, j, True)
-    return f"ldnt({p})" if nt else f"*{p}"
+def _load(sp: StageProgram, arr: str, j: int, mode: str, sfx: str = "") -> str:
+    if mode == "buf":
+        return f"tq_ld(r_{arr}{sfx}, toff8, {8 * sp.roff[j]})"
+    p = _ptr(sp, arr, j, True, sfx)
+    return f"ldnt({p})" if mode == "nt" else f"*{p}"
 
 
-def _store(sp: StageProgram, arr: str, j: int, val: str, nt: bool) -> str:
-    p = _ptr(sp, arr, j, False)
-    return f"stnt({p}, {val});" if nt else f"*{p} = {val};"
+def _store(sp: StageProgram, arr: str, j: int, val: str, mode: str, sfx: str = "") -> str:
+    if mode == "buf":
+        return f"tq_st(r_{arr}{sfx}, toff8, {8 * sp.roff[j]}, {val});"
+    p = _ptr(sp, arr, j, False, sfx)
+    return f"stnt({p}, {val});" if mode == "nt" else f"*{p} = {val};"
 
 
 def _toff(sp: StageProgram) -> str:
@@ -653,13 +747,30 @@ def _toff(sp: StageProgram) -> str:
     return " | ".join(terms) if terms else "0u"
 
 
-def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False) -> str:
-    """HIP source of one forward stage kernel (k_stage_fwd semantics);
-    nontemporal: the state is read and written with non-temporal hints."""
+def _codegen(buf: Optional[bool], prefetch: Optional[bool], kind: str, lds: Optional[str] = None,
+             racc: Optional[int] = None):
+    kind = "a" if kind == "p" else kind
+    return (CODEGEN["buf"] if buf is None else buf,
+            CODEGEN["prefetch_" + kind] if prefetch is None else prefetch,
+            CODEGEN["lds"] if lds is None else lds,
+            (CODEGEN["racc"] if racc is None else racc) if kind == "a" else 0)
+
+
+def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Optional[bool] = None,
+                prefetch: Optional[bool] = None, lds: Optional[str] = None) -> str:
+    """HIP source of one forward stage kernel (k_stage_fwd semantics).
+    nontemporal: the state is read and written with non-temporal hints;
+    buf: buffer instructions for the state where the offsets allow;
+    prefetch: each group's loads are issued one group ahead (software
+    pipelining), so that they are in flight while the previous group
+    computes; lds: the remap buffer's index map, "xor" (swizzled) or "pad4" /
+    "pad5" (padded, additive).  None takes the module default (CODEGEN)."""
     sp = StageProgram(words)
     if sp.T < MIN_T:
         raise ValueError("compiled stages need T >= 6")
-    g = _Gen(sp)
+    buf, prefetch, lds, _ = _codegen(buf, prefetch, "f", lds)
+    mode = _mem_mode(sp, nontemporal, buf)
+    g = _Gen(sp, lds)
     a = [f"i{j}" for j in range(NS)]
     has_remap = False
     for w in sp.ops:
@@ -669,57 +780,106 @@ def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False) -> str:
             has_remap = True
         else:
             g.fwd_op(w, a)
-    body = g.lines
+    body = ["  " + s for s in g.lines]
+    stores = [f"      {_store(sp, 'out', j, a[j], mode)}" for j in range(NS)]
     nt = 1 << sp.T
     src = [f'extern "C" __global__ void __launch_bounds__({nt}) {name}(float2* __restrict__ out, '
            "const float2* __restrict__ in, const float2* __restrict__ mats, int init_zero, long long n_groups, "
            "long long gpw) {"]
     if has_remap:
-        src.append(f"  __shared__ float2 lds[{1 << sp.m}];")
+        src.append(f"  __shared__ float2 lds[{lds_words(sp.m, lds)}];")
     src.append("  const int tid = threadIdx.x;")
     src.append(f"  const unsigned toff = {_toff(sp)};")
     src.append("  const unsigned toff8 = toff << 3;")
     src += g.hoisted()
     src += ["  const long long g0 = (long long)blockIdx.x * gpw;",
             "  long long g1 = g0 + gpw;",
-            "  if (g1 > n_groups) g1 = n_groups;",
-            "  for (long long g = g0; g < g1; ++g) {"]
-    src += _head(sp, g)
-    src.append(f"    float2 {', '.join(f'i{j}' for j in range(NS))};")
-    src.append("    if (init_zero) {")
-    for j in range(NS):
-        src.append(f"      i{j} = make_float2((goff + toff + {sp.roff[j]}u) == 0u ? 1.f : 0.f, 0.f);")
-    src.append("    } else {")
-    for j in range(NS):
-        src.append(f"      i{j} = {_load(sp, 'in', j, nontemporal)};")
-    src.append("    }")
-    src += body
-    for j in range(NS):
-        src.append(f"    {_store(sp, 'out', j, a[j], nontemporal)}")
-    src += ["  }", "}"]
+            "  if (g1 > n_groups) g1 = n_groups;"]
+    # the first stage of a pass starts from |0..0> and reads nothing
+    src += ["  if (init_zero) {",
+            "    for (long long g = g0; g < g1; ++g) {"]
+    src += ["      " + s for s in _group_vars(sp, "g", "", mode, ["out"])]
+    src += [f"      const float2 i{j} = make_float2((goff + toff + {sp.roff[j]}u) == 0u ? 1.f : 0.f, 0.f);"
+            for j in range(NS)]
+    src += body + stores + ["    }", "    return;", "  }"]
+    if prefetch:
+        src += [f"  float2 {', '.join(f'c{j}' for j in range(NS))};",
+                "  if (g0 < g1) {"]
+        src += ["    " + s for s in _group_vars(sp, "g0", "n", mode, ["in"])]
+        src += [f"    c{j} = {_load(sp, 'in', j, mode, 'n')};" for j in range(NS)]
+        src += ["  }",
+                "  for (long long g = g0; g < g1; ++g) {"]
+        src += ["    " + s for s in _group_vars(sp, "g", "", mode, ["out"])]
+        src += ["    const long long gn = g + 1 < g1 ? g + 1 : g;"]
+        src += ["    " + s for s in _group_vars(sp, "gn", "n", mode, ["in"])]
+        src += [f"    const float2 q{j} = {_load(sp, 'in', j, mode, 'n')};" for j in range(NS)]
+        src += [f"    const float2 i{j} = c{j};" for j in range(NS)]
+        src += body + stores
+        src += [f"    c{j} = q{j};" for j in range(NS)]
+        src += ["  }"]
+    else:
+        src += ["  for (long long g = g0; g < g1; ++g) {"]
+        src += ["    " + s for s in _group_vars(sp, "g", "", mode, ["in", "out"])]
+        src += [f"    const float2 i{j} = {_load(sp, 'in', j, mode)};" for j in range(NS)]
+        src += body + stores + ["  }"]
+    src.append("}")
     return "\n".join(src) + "\n"
 
 
-def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False) -> str:
+def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Optional[bool] = None,
+                prefetch: Optional[bool] = None, lds: Optional[str] = None, racc: Optional[int] = None,
+                skip: Sequence[int] = (), no_store: bool = False) -> str:
     """HIP source of one adjoint stage kernel (k_stage_adj semantics);
-    nontemporal: psi and lam are read and written with non-temporal hints."""
+    nontemporal, buf, prefetch and lds as for gen_forward (prefetch loads
+    psi and lam of the next group).  racc: when the stage has at most this
+    many gradient values, each lane keeps its partial sums in registers
+    across the groups of a row and the cross-lane reduction runs once per
+    row instead of once per group.
+
+    Pruned sweeps (no gradient with respect to the input state): the ops at
+    the program positions in `skip` are not un-applied (a trainable one still
+    adds its gradient partials), and with no_store nothing is written back --
+    the sweep ends with this stage, so the code after its last trainable op
+    (in sweep order) is dropped too."""
     sp = StageProgram(words)
     if sp.T < MIN_T:
         raise ValueError("compiled stages need T >= 6")
-    g = _Gen(sp)
+    buf, prefetch, lds, racc = _codegen(buf, prefetch, "a", lds, racc)
+    mode = _mem_mode(sp, nontemporal, buf)
+    g = _Gen(sp, lds)
     p = [f"p{j}" for j in range(NS)]
     l = [f"l{j}" for j in range(NS)]
     has_remap = False
-    for w in reversed(sp.ops):
+    in_regs = 0 < sp.ngp <= racc
+    acc_ops = []            # (op words, register accumulators of its partials)
+    skip = frozenset(int(x) for x in skip)
+    grad_pos = [i for i, w in enumerate(sp.ops) if (w[0] & 15) != OP_REMAP and w[7] >= 0]
+    first_grad = min(grad_pos) if grad_pos else len(sp.ops)
+    for pos in reversed(range(len(sp.ops))):
+        w = sp.ops[pos]
+        if no_store and pos < first_grad:
+            break
         if (w[0] & 15) == OP_REMAP:
             frm, to = sp.remap_layouts(w)
             p = g.remap(p, to, frm)
             l = g.remap(l, to, frm)
             has_remap = True
             continue
-        parts = g.bwd_op(w, p, l)
+        parts = g.bwd_op(w, p, l, update=pos not in skip)
         if w[7] >= 0 and parts is not None:
-            g.reduce(w, parts)
+            if in_regs:
+                names = [f"ga{len(acc_ops)}_{q}" for q in range(len(parts))]
+                for nm, e in zip(names, parts):
+                    if e != "0.f":
+                        g.line(f"{nm} += {e};")
+                acc_ops.append((w, names))
+            else:
+                g.reduce(w, parts)
+    # register accumulators -> the workgroup's LDS accumulators (full waves)
+    drain: List[str] = []
+    for w, names in acc_ops:
+        drain += g.detached(lambda w=w, names=names: g.reduce(w, names))
+        drain += [f"{nm} = 0.f;" for nm in names]
     nt = 1 << sp.T
     ngp = sp.ngp
     src = []
@@ -733,10 +893,12 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False) -> str:
                "const float2* __restrict__ mats, float* __restrict__ grad, "
                "long long grad_stride, long long n_groups, long long gpw) {")
     if has_remap:
-        src.append(f"  __shared__ float2 lds[{1 << sp.m}];")
+        src.append(f"  __shared__ float2 lds[{lds_words(sp.m, lds)}];")
     if ngp:
         src.append(f"  __shared__ float gacc[{ngp}];")
         src.append(f"  for (int t = threadIdx.x; t < {ngp}; t += {nt}) gacc[t] = 0.f;")
+    for _, names in acc_ops:
+        src.append(f"  float {', '.join(nm + ' = 0.f' for nm in names)};")
     src.append("  const int tid = threadIdx.x;")
     src.append("  const int lane = tid & 63;")
     src.append(f"  const unsigned toff = {_toff(sp)};")
@@ -746,13 +908,23 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False) -> str:
             "  long long g1 = g0 + gpw;",
             "  if (g1 > n_groups) g1 = n_groups;",
             "  long long cur_row = -1;",
-            "  __syncthreads();",
-            "  for (long long g = g0; g < g1; ++g) {"]
-    src += _head(sp, g)
+            "  __syncthreads();"]
+    if prefetch:
+        src += [f"  float2 {', '.join(f'pc{j}' for j in range(NS))};",
+                f"  float2 {', '.join(f'lc{j}' for j in range(NS))};",
+                "  if (g0 < g1) {"]
+        src += ["    " + s for s in _group_vars(sp, "g0", "n", mode, ["psi_in", "lam_in"])]
+        src += [f"    pc{j} = {_load(sp, 'psi_in', j, mode, 'n')};" for j in range(NS)]
+        src += [f"    lc{j} = {_load(sp, 'lam_in', j, mode, 'n')};" for j in range(NS)]
+        src += ["  }"]
+    src += ["  for (long long g = g0; g < g1; ++g) {"]
+    src += ["    " + s for s in _group_vars(sp, "g", "", mode, ["psi", "lam"] if prefetch else
+                                            ["psi", "lam", "psi_in", "lam_in"])]
     if ngp:
         src += ["    if ((long long)row != cur_row) {",
-                "      if (cur_row >= 0) {",
-                "        __syncthreads();",
+                "      if (cur_row >= 0) {"]
+        src += ["        " + x for x in drain]
+        src += ["        __syncthreads();",
                 f"        for (int t = tid; t < {ngp}; t += {nt}) {{",
                 "          const float v = gacc[t];",
                 f"          if (v != 0.f) atomicAdd(grad + cur_row * grad_stride + {name}_gs[t], v);",
@@ -762,19 +934,28 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False) -> str:
                 "      }",
                 "      cur_row = (long long)row;",
                 "    }"]
-    for j in range(NS):
-        src.append(f"    const float2 p{j} = {_load(sp, 'psi_in', j, nontemporal)};")
-    for j in range(NS):
-        src.append(f"    const float2 l{j} = {_load(sp, 'lam_in', j, nontemporal)};")
+    if prefetch:
+        src += ["    const long long gn = g + 1 < g1 ? g + 1 : g;"]
+        src += ["    " + s for s in _group_vars(sp, "gn", "n", mode, ["psi_in", "lam_in"])]
+        src += [f"    const float2 pq{j} = {_load(sp, 'psi_in', j, mode, 'n')};" for j in range(NS)]
+        src += [f"    const float2 lq{j} = {_load(sp, 'lam_in', j, mode, 'n')};" for j in range(NS)]
+        src += [f"    const float2 p{j} = pc{j};" for j in range(NS)]
+        src += [f"    const float2 l{j} = lc{j};" for j in range(NS)]
+    else:
+        src += [f"    const float2 p{j} = {_load(sp, 'psi_in', j, mode)};" for j in range(NS)]
+        src += [f"    const float2 l{j} = {_load(sp, 'lam_in', j, mode)};" for j in range(NS)]
     src += g.lines
-    for j in range(NS):
-        src.append(f"    {_store(sp, 'psi', j, p[j], nontemporal)}")
-    for j in range(NS):
-        src.append(f"    {_store(sp, 'lam', j, l[j], nontemporal)}")
+    if not no_store:
+        src += [f"    {_store(sp, 'psi', j, p[j], mode)}" for j in range(NS)]
+        src += [f"    {_store(sp, 'lam', j, l[j], mode)}" for j in range(NS)]
+    if prefetch:
+        src += [f"    pc{j} = pq{j};" for j in range(NS)]
+        src += [f"    lc{j} = lq{j};" for j in range(NS)]
     src.append("  }")
     if ngp:
-        src += ["  if (cur_row >= 0) {",
-                "    __syncthreads();",
+        src += ["  if (cur_row >= 0) {"]
+        src += ["    " + x for x in drain]
+        src += ["    __syncthreads();",
                 f"    for (int t = tid; t < {ngp}; t += {nt}) {{",
                 "      const float v = gacc[t];",
                 f"      if (v != 0.f) atomicAdd(grad + cur_row * grad_stride + {name}_gs[t], v);",
@@ -787,9 +968,22 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False) -> str:
 # ----------------------------------------------------------------------------
 # compile + cache
 # ----------------------------------------------------------------------------
-def kernel_name(kind: str, words: np.ndarray, nt: bool = False) -> str:
+def _spec(prog):
+    """(words, extra gen_adjoint arguments) of a program entry: stage words,
+    or for kind "p" a tuple (words, skip positions, no_store)."""
+    if isinstance(prog, tuple):
+        words, skip, no_store = prog
+        return words, {"skip": tuple(sorted(int(x) for x in skip)), "no_store": bool(no_store)}
+    return prog, {}
+
+
+def kernel_name(kind: str, prog, nt: bool = False) -> str:
+    words, extra = _spec(prog)
     hs = hashlib.sha1(np.ascontiguousarray(words, dtype=np.int32).tobytes())
+    if extra:
+        hs.update(repr(sorted(extra.items())).encode())
     hs.update(COMPILE_OPTS[kind].encode())
+    hs.update(repr(_codegen(None, None, kind)).encode())
     if nt:
         hs.update(b"nt")
     h = hs.hexdigest()[:20]
@@ -867,11 +1061,18 @@ _modules: List[int] = []                # loaded modules (kept for the process l
 
 def get_kernels(programs: Sequence[np.ndarray], kind: str, device_index: int, nt: bool = False) -> List[int]:
     """hipFunction_t handles (as ints) of the compiled stage kernels for the
-    given programs; kind "f" = forward, "a" = adjoint.  Compiles and loads
-    (on the calling thread) what is not loaded yet.  hipRTC serialises
+    given programs; kind "f" = forward, "a" = adjoint, "p" = pruned adjoint
+    (programs given as (words, skip positions, no_store)).  Compiles and
+    loads (on the calling thread) what is not loaded yet.  hipRTC serialises
     compiles within a process, so sources are compiled one after another."""
     from . import _native as N
-    gen = gen_forward if kind == "f" else gen_adjoint
+
+    def gen(prog, nm, nt):
+        words, extra = _spec(prog)
+        if kind == "f":
+            return gen_forward(words, nm, nt)
+        return gen_adjoint(words, nm, nt, **extra)
+
     names = [kernel_name(kind, w, nt) for w in programs]
     with _lock:
         missing = {}
