@@ -81,6 +81,42 @@ def time_forward(n, bsz, ops, mode, steps, warmup, tq, jit="auto"):
 STAGE_BITS = 0      # local bits per stage of the gradient workload's plans (0 = the engine's choice)
 
 
+def time_replay(n, bsz, ops, steps, warmup, tq):
+    """The same forward recorded once with tq.CircuitReplay and replayed:
+    seconds per batch, host issue seconds per batch, the <Z_i> output and the
+    stage-kernel milliseconds per batch (HIP events)."""
+    import torchquantum_amd.functional as tqf
+    from torchquantum_amd import engine as E
+    params = [None if op["params"] is None else torch.tensor(op["params"], dtype=torch.float32, device="cuda")
+              for op in ops]
+
+    def record(q):
+        for op, p in zip(ops, params):
+            tqf.func_name_dict[op["name"]](q, op["wires"], params=p)
+
+    rep = tq.CircuitReplay(record, n, bsz, device="cuda")
+    meas = tq.MeasureAll(tq.PauliZ)
+    for _ in range(warmup):
+        meas(rep())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = meas(rep())
+    t_issue = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    rec = []
+    E.set_stage_timer(rec)
+    try:
+        for _ in range(steps):
+            meas(rep())
+    finally:
+        E.set_stage_timer(None)
+    torch.cuda.synchronize()
+    stage_ms = sum(e0.elapsed_time(e1) for _, _, e0, e1 in rec) / steps
+    return dt, t_issue / steps, out, stage_ms
+
+
 def _ansatz_loss_step(n, bsz, layers, mode, tq, jit, params, phi, ops, obs, coef):
     import torchquantum_amd.functional as tqf
     for p in params:
@@ -718,6 +754,8 @@ def main():
                     help="run only the per-gate forward of the 20-qubit RY/CZ circuit (--steps, --warmup)")
     ap.add_argument("--stage-bits", type=int, default=0,
                     help="local bits per stage for the gradient workload (default: the engine's choice)")
+    ap.add_argument("--only-small", action="store_true",
+                    help="only the 10-qubit bsz-32 circuit: functional API and CircuitReplay")
     ap.add_argument("--only-gradient", action="store_true",
                     help="run only the fused adjoint gradient (--adj-qubits, --adj-bsz, --steps)")
     ap.add_argument("--only-batch", action="store_true", help="run only the uneven batch")
@@ -757,6 +795,17 @@ def main():
     if args.only_batch:
         res, _ = time_batch(256, 5, tq)
         print(json.dumps({"batch": res}), flush=True)
+        return
+    if args.only_small:
+        from circuits import config1_ops
+        ops1 = config1_ops(seed=0)
+        t1_f, ev_f1 = time_forward(10, 32, ops1, "fused", 50, 3, tq)
+        host1 = time_forward.host_issue_s
+        t1_r, host1_r, ev_r1, stage_r1 = time_replay(10, 32, ops1, 50, 3, tq)
+        print(json.dumps({"small_circuit": {"fused_ms_per_circuit_batch": t1_f * 1e3, "fused_host_issue_ms": host1 * 1e3,
+                                            "replay_ms_per_circuit_batch": t1_r * 1e3,
+                                            "replay_host_issue_ms": host1_r * 1e3, "replay_stage_kernel_ms": stage_r1,
+                                            "max_abs_diff": float((ev_r1 - ev_f1).abs().max())}}), flush=True)
         return
     if args.only_gradient:
         an, ab = args.adj_qubits, args.adj_bsz
@@ -852,10 +901,17 @@ def main():
         E.set_stage_timer(None)
         torch.cuda.synchronize()
         stage_ms = sum(e0.elapsed_time(e1) for _, _, e0, e1 in rec1) / 20
+        _, ev_f1 = time_forward(10, 32, ops1, "fused", 1, 0, tq)
+        t1_r, host1_r, ev_r1, stage_r1 = time_replay(10, 32, ops1, 50, 3, tq)
         extra["small_circuit"] = {"workload": "10 qubits, bsz 32, 8 x [U3 all wires, CNOT ring], <Z_i> (160 gates)",
                                   "pergate_ms_per_circuit_batch": t1_pg * 1e3,
                                   "fused_ms_per_circuit_batch": t1_f * 1e3, "fused_amp_updates_per_s": upd1 / t1_f,
-                                  "fused_host_issue_ms": host1 * 1e3, "fused_stage_kernel_ms": stage_ms}
+                                  "fused_host_issue_ms": host1 * 1e3, "fused_stage_kernel_ms": stage_ms,
+                                  "replay": {"api": "tq.CircuitReplay (recorded once, compiled stage kernel)",
+                                             "ms_per_circuit_batch": t1_r * 1e3, "host_issue_ms": host1_r * 1e3,
+                                             "stage_kernel_ms": stage_r1, "amp_updates_per_s": upd1 / t1_r,
+                                             "expval_max_abs_diff_vs_functional":
+                                                 float((ev_r1 - ev_f1).abs().max())}}
         log("[bench] dense k = 5, 6 blocks")
         extra["dense_block"] = dense_block_bench()
         log("[bench] encoder + ansatz training steps")

@@ -23,6 +23,7 @@ from .layer import (Op1QAllLayer, Op2QAllLayer, EntanglePairwise, LayerTemplate0
 from .encoding import (Encoder, GeneralEncoder, PhaseEncoder, MultiPhaseEncoder, StateEncoder, AmplitudeEncoder,
                        MagnitudeEncoder, encoder_op_list_name_dict)
 from .engine import EngineOptions, wait_for_compiled_stages
+from .replay import CircuitReplay
 from . import device as _device_mod
 
 _device_mod._bind_methods()
