@@ -553,6 +553,59 @@ def config5(args, tq):
     return res
 
 
+def config5_plans(args, tq):
+    """Plan choice for the 30-qubit workload.  Candidates: fusion width
+    1..6 (dense blocks; widths 5 and 6 run on the matrix-core kernel), stage
+    width 12, timed with the interpreting stage kernels (no run-time compile
+    per candidate).  The learned planner -- the MLP trained by this package's
+    kernels on exhaustive timings of six smaller circuits of the same family
+    (16, 20 and 24 qubits, bsz 1) -- chooses from plan features alone;
+    exhaustive timing of every candidate at full size judges the choice."""
+    from torchquantum_amd import batch as B
+    from torchquantum_amd.autotune import Autotuner, LearnedPlanner, circuit_key, time_plan
+    from torchquantum_amd.costmodel import MLPCostModel
+    from circuits import config5_ops
+    cands = [tq.EngineOptions(fusion_width=k, stage_bits=12, jit="off") for k in range(1, 7)]
+    tuner = Autotuner()
+    keys = []
+    t0 = time.perf_counter()
+    for n in (16, 20, 24):
+        for seed in (1, 2):
+            tape = B.ops_to_tape(config5_ops(seed=seed, n=n))
+            tuner.tune(tape, n, 1, candidates=cands, reps=3)
+            keys.append(circuit_key(tape, n, 1))
+            log(f"[config5-plans] training circuit {len(keys)}/6 ({n} qubits) timed")
+    t_data = time.perf_counter() - t0
+    X, y = tuner.dataset(keys)
+    model = MLPCostModel(device="cuda", seed=0)
+    hist = model.fit(X, y, epochs=1500, lr=3e-3)
+    planner = LearnedPlanner(model, cands)
+    n, bsz = args.c5_qubits, args.c5_bsz
+    tape = B.ops_to_tape(config5_ops(seed=0, n=n))
+    t0 = time.perf_counter()
+    opts, pred = planner.score(tape, n, bsz)
+    t_pick = time.perf_counter() - t0
+    pick = opts[int(np.argmin(pred))]
+    measured = []
+    t0 = time.perf_counter()
+    for o in opts:
+        measured.append(time_plan(tape, n, bsz, o, reps=1))
+        torch.cuda.empty_cache()
+        log(f"[config5-plans] fusion width {o.fusion_width}: {measured[-1]:.0f} ms")
+    t_judge = time.perf_counter() - t0
+    best = int(np.argmin(measured))
+    ip = opts.index(pick)
+    return {"workload": f"{n} qubits, bsz {bsz}, the config-5 circuit, forward",
+            "candidates": "fusion width 1..6 (5, 6: matrix-core blocks), stage width 12, interpreting stage kernels",
+            "train_circuits": "config-5 family, seeds 1-2, 16/20/24 qubits, bsz 1 (36 timed plans)",
+            "train_timing_s": t_data, "train_loss_first_last": [hist[0], hist[-1]],
+            "ms_by_fusion_width": {str(o.fusion_width): m for o, m in zip(opts, measured)},
+            "predicted_ms_by_fusion_width": {str(o.fusion_width): float(p) for o, p in zip(opts, pred)},
+            "learned_pick": pick.fusion_width, "exhaustive_pick": opts[best].fusion_width,
+            "learned_regret_vs_exhaustive": measured[ip] / measured[best] - 1,
+            "default_plan_ms": measured[0], "learned_selection_s": t_pick, "exhaustive_timing_s": t_judge}
+
+
 def cpu_threads():
     return int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
 
@@ -754,6 +807,8 @@ def main():
                     help="run only the per-gate forward of the 20-qubit RY/CZ circuit (--steps, --warmup)")
     ap.add_argument("--stage-bits", type=int, default=0,
                     help="local bits per stage for the gradient workload (default: the engine's choice)")
+    ap.add_argument("--config5-plans", action="store_true",
+                    help="only the 30-qubit plan choice: learned planner vs exhaustive timing")
     ap.add_argument("--only-small", action="store_true",
                     help="only the 10-qubit bsz-32 circuit: functional API and CircuitReplay")
     ap.add_argument("--only-gradient", action="store_true",
@@ -827,6 +882,9 @@ def main():
         return
     if args.config5:
         print(json.dumps({"config5": config5(args, tq)}), flush=True)
+        return
+    if args.config5_plans:
+        print(json.dumps({"config5_plans": config5_plans(args, tq)}), flush=True)
         return
 
     # ---------------------------------------------------------------- headline: adjoint gradient

@@ -87,9 +87,16 @@ int tqsv_stage_forward(tqsv_stream_t s, void* state_out, const void* state_in, i
                        const int* prog_host, const void* prog_dev, const void* mats, int init_zero, int grid_cap);
 /* Reverse sweep of one stage for the adjoint method: psi <- U^dag psi,
  * lam <- U^dag lam, grad[row * grad_stride + slot] += Re<lam|dU/dtheta psi>.
- * Replaces torch autograd through gate_wrapper.py:127-136. */
+ * Replaces torch autograd through gate_wrapper.py:127-136.  work: device
+ * buffer of at least tqsv_stage_adjoint_work_floats() floats for the
+ * per-workgroup partials, which a second launch adds into grad in a fixed
+ * order (gradients are bitwise reproducible; no float atomics). */
 int tqsv_stage_adjoint(tqsv_stream_t s, void* psi, void* lam, int64_t bsz, int n, const int* prog_host,
-                       const void* prog_dev, const void* mats, float* grad, int64_t grad_stride, int grid_cap);
+                       const void* prog_dev, const void* mats, float* grad, int64_t grad_stride, float* work,
+                       int64_t work_floats, int grid_cap);
+/* Partial-buffer floats tqsv_stage_adjoint / tqsv_jit_stage_adjoint need for
+ * this stage and launch shape (< 0 on a bad program). */
+int64_t tqsv_stage_adjoint_work_floats(const int* prog_host, int n, int64_t bsz, int grid_cap);
 
 /* ---- run-time compiled stages: the same stage program turned by the host
  *      planner into straight-line HIP source (every op unrolled with its bits,
@@ -110,26 +117,34 @@ int tqsv_jit_unload(void* module);
 int tqsv_jit_stage_forward(tqsv_stream_t s, const void* func, void* state_out, const void* state_in, int64_t bsz,
                            int n, const int* prog_host, const void* mats, int init_zero, int grid_cap);
 /* func: adjoint kernel compiled from prog_host's program (parameters: psi,
- * lam, psi_in, lam_in, mats, grad, grad_stride, n_groups, groups_per_wg).
- * psi_in / lam_in: where psi and lam are read (NULL = in place), so the first
- * stage of a sweep can read the forward output and the incoming gradient
- * without copying them. */
+ * lam, psi_in, lam_in, mats, partials, rows per workgroup, n_groups,
+ * groups_per_wg).  psi_in / lam_in: where psi and lam are read (NULL = in
+ * place), so the first stage of a sweep can read the forward output and the
+ * incoming gradient without copying them.  prog_dev, work: as
+ * tqsv_stage_adjoint (the fixed-order reduction reads the gradient slot
+ * table from prog_dev). */
 int tqsv_jit_stage_adjoint(tqsv_stream_t s, const void* func, void* psi, void* lam, const void* psi_in,
-                           const void* lam_in, int64_t bsz, int n, const int* prog_host, const void* mats,
-                           float* grad, int64_t grad_stride, int grid_cap);
+                           const void* lam_in, int64_t bsz, int n, const int* prog_host, const void* prog_dev,
+                           const void* mats, float* grad, int64_t grad_stride, float* work, int64_t work_floats,
+                           int grid_cap);
 
 /* ---- per-gate adjoint steps (the unfused gradient path); mat holds U then
  *      P derivative matrices dU/dtheta_p (P <= 4); grad slots gslot..gslot+P-1.
- *      Replace autograd through apply_unitary_bmm (gate_wrapper.py:127-136). */
+ *      Replace autograd through apply_unitary_bmm (gate_wrapper.py:127-136).
+ *      work: at least tqsv_adjoint_work_floats(bsz, n) floats of device
+ *      memory for the per-workgroup partials (fixed-order sum, no float
+ *      atomics). */
 int tqsv_adjoint_1q(tqsv_stream_t s, void* psi, void* lam, int64_t bsz, int n, int tbit, uint64_t cmask,
                     uint64_t cval, const void* mat, int64_t mat_bstride, int P, float* grad, int64_t grad_stride,
-                    int gslot);
+                    int gslot, float* work, int64_t work_floats);
 int tqsv_adjoint_2q(tqsv_stream_t s, void* psi, void* lam, int64_t bsz, int n, int bit_hi, int bit_lo,
                     uint64_t cmask, uint64_t cval, const void* mat, int64_t mat_bstride, int P, float* grad,
-                    int64_t grad_stride, int gslot);
+                    int64_t grad_stride, int gslot, float* work, int64_t work_floats);
 int tqsv_adjoint_diag(tqsv_stream_t s, void* psi, void* lam, int64_t bsz, int n, uint64_t cmask, uint64_t cval,
                       uint64_t pmask, const void* phases, int64_t ph_bstride, int P, float* grad,
-                      int64_t grad_stride, int gslot);
+                      int64_t grad_stride, int gslot, float* work, int64_t work_floats);
+/* Partial-buffer floats any per-gate adjoint step on a [bsz, 2^n] state needs. */
+int64_t tqsv_adjoint_work_floats(int64_t bsz, int n);
 
 /* ---- gate matrices from device parameters ------------------------------- */
 /* sizeof(DevOp) | sizeof(DevMember) << 16 (layouts checked by the host). */

@@ -191,3 +191,21 @@ def test_pruned_sweep_matches_full_sweep():
     assert none is None and lam_out is not None
     full, pruned = full.double().cpu().numpy(), pruned.double().cpu().numpy()
     assert np.all(np.abs(pruned - full) <= 1e-5 * (1 + np.abs(full))), float(np.abs(pruned - full).max())
+
+
+@gpu
+@pytest.mark.parametrize("label", ["pergate", "fused-interpreting", "fused-compiled"])
+def test_gradients_bitwise_reproducible(label):
+    """The same gradient evaluated twice is bitwise equal: every adjoint
+    kernel writes per-workgroup partials that a second launch adds up in a
+    fixed order (no float atomics).  12 qubits, bsz 4, the gradient ansatz."""
+    import torchquantum_amd as tq
+    opts = {"pergate": tq.EngineOptions(mode="pergate"),
+            "fused-interpreting": tq.EngineOptions(mode="fused", jit="off"),
+            "fused-compiled": tq.EngineOptions(mode="fused", jit="on")}[label]
+    n, bsz = 12, 4
+    ops, obs, coef, phi, theta0 = _ansatz_inputs(n, bsz)
+    runs = [_engine_loss_and_grad(n, bsz, ops, obs, coef, phi, theta0, opts) for _ in range(3)]
+    for loss, g in runs[1:]:
+        assert loss == runs[0][0]
+        assert np.array_equal(g, runs[0][1]), float(np.abs(g - runs[0][1]).max())

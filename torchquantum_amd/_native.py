@@ -36,13 +36,15 @@ SIGNATURES = {
     "tqsv_run_gates": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_i64_p, c_int, c_void_p]),
     "tqsv_stage_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int_p, c_void_p, c_void_p, c_int, c_int]),
     "tqsv_stage_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int_p, c_void_p, c_void_p,
-                                   c_float_p, c_int64, c_int]),
+                                   c_float_p, c_int64, c_void_p, c_int64, c_int]),
+    "tqsv_stage_adjoint_work_floats": (c_int64, [c_int_p, c_int, c_int64, c_int]),
+    "tqsv_adjoint_work_floats": (c_int64, [c_int64, c_int]),
     "tqsv_adjoint_1q": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int, c_uint64, c_uint64, c_void_p,
-                                c_int64, c_int, c_float_p, c_int64, c_int]),
+                                c_int64, c_int, c_float_p, c_int64, c_int, c_void_p, c_int64]),
     "tqsv_adjoint_2q": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int, c_int, c_uint64, c_uint64,
-                                c_void_p, c_int64, c_int, c_float_p, c_int64, c_int]),
+                                c_void_p, c_int64, c_int, c_float_p, c_int64, c_int, c_void_p, c_int64]),
     "tqsv_adjoint_diag": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_uint64, c_uint64, c_uint64,
-                                  c_void_p, c_int64, c_int, c_float_p, c_int64, c_int]),
+                                  c_void_p, c_int64, c_int, c_float_p, c_int64, c_int, c_void_p, c_int64]),
     "tqsv_expval_workspace_bytes": (c_int64, [c_int64, c_int, c_int]),
     "tqsv_expval_z_all": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_float_p, c_void_p, c_int64]),
     "tqsv_expval_z_all_noalloc": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_float_p]),
@@ -67,7 +69,7 @@ SIGNATURES = {
     "tqsv_jit_stage_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int_p, c_void_p,
                                        c_int, c_int]),
     "tqsv_jit_stage_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int,
-                                       c_int_p, c_void_p, c_float_p, c_int64, c_int]),
+                                       c_int_p, c_void_p, c_void_p, c_float_p, c_int64, c_void_p, c_int64, c_int]),
     "tqsv_gemm": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
                           c_float, c_void_p, c_int, c_void_p, c_int]),
     "tqsv_relu_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),

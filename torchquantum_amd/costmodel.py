@@ -1,5 +1,5 @@
 """Learned cost model for fusion plans: an MLP from plan features to
-predicted log-milliseconds, trained and evaluated on the GPU with this
+predicted log-milliseconds per amplitude, trained and evaluated on the GPU with this
 package's own HIP kernels (tqsv_mlp.hip: GEMM + bias + ReLU, ReLU backward,
 column sums, MSE gradient, Adam).  torch only holds the buffers.
 
@@ -73,6 +73,8 @@ class MLPCostModel:
         self.t = 0
         self.mu = None
         self.sd = None
+        self.x_min = None
+        self.x_max = None
         self.y_mu = 0.0
         self.y_sd = 1.0
 
@@ -136,15 +138,31 @@ class MLPCostModel:
 
     # ------------------------------------------------------------ training API
     def _std(self, X: np.ndarray) -> torch.Tensor:
-        Z = (np.asarray(X, dtype=np.float64) - self.mu) / self.sd
+        """Standardised network input.  Features are held to the range seen in
+        training: outside it a ReLU network extrapolates linearly without
+        bound, while the cost per amplitude of a plan changes slowly with its
+        size (the size itself enters through the per-amplitude target)."""
+        X = np.asarray(X, dtype=np.float64)
+        if self.x_min is not None:
+            X = np.clip(X, self.x_min, self.x_max)
+        Z = (X - self.mu) / self.sd
         return torch.tensor(Z, dtype=torch.float32, device=self.device)
 
+    @staticmethod
+    def _log_amps(X: np.ndarray) -> np.ndarray:
+        """ln(amplitudes) of each plan (feature 0 is log2 of the state size)."""
+        return np.asarray(X, dtype=np.float64)[:, 0] * math.log(2.0)
+
     def fit(self, X: np.ndarray, y_ms: np.ndarray, epochs: int = 1500, lr: float = 3e-3, batch: int = 0):
-        """Train on log(ms) targets; returns the loss history (host floats)."""
+        """Train on log(ms per amplitude) targets -- the time of a plan grows
+        with the state it sweeps, so the network learns the cost per amplitude
+        and extrapolates to larger states through that factor; returns the
+        loss history (host floats)."""
         X = np.asarray(X, dtype=np.float64)
-        y = np.log(np.asarray(y_ms, dtype=np.float64))
+        y = np.log(np.asarray(y_ms, dtype=np.float64)) - self._log_amps(X)
         self.mu = X.mean(0)
         self.sd = X.std(0) + 1e-6
+        self.x_min, self.x_max = X.min(0), X.max(0)
         self.y_mu, self.y_sd = float(y.mean()), float(y.std() + 1e-6)
         Xt = self._std(X)
         yt = torch.tensor((y - self.y_mu) / self.y_sd, dtype=torch.float32, device=self.device)
@@ -162,8 +180,9 @@ class MLPCostModel:
         return hist
 
     def predict_ms(self, X: np.ndarray) -> np.ndarray:
-        pred, _ = self.forward(self._std(np.atleast_2d(X)))
-        return np.exp(pred.detach().cpu().numpy().astype(np.float64) * self.y_sd + self.y_mu)
+        X = np.atleast_2d(np.asarray(X, dtype=np.float64))
+        pred, _ = self.forward(self._std(X))
+        return np.exp(pred.detach().cpu().numpy().astype(np.float64) * self.y_sd + self.y_mu + self._log_amps(X))
 
     def state_dict(self):
         return {"W": [w.cpu().tolist() for w in self.W], "b": [b.cpu().tolist() for b in self.b],

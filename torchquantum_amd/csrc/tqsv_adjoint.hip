@@ -12,7 +12,10 @@
 // Gradients land in grad[row * grad_stride + gslot + p] (one row per batch
 // row; the caller sums rows for parameters shared across the batch).  Each
 // workgroup covers part of one row (grid.y = row), reduces its partials with
-// a wave shuffle tree + LDS and adds them with one atomic per parameter.
+// a wave shuffle tree + LDS and writes them to its own slots of a caller-
+// provided partial buffer; k_rowpart_reduce then adds the partials of each
+// row in a fixed order (no float atomics: gradients are bitwise
+// reproducible from run to run).
 #include "tqsv_common.h"
 
 namespace tqsv {
@@ -23,7 +26,11 @@ namespace tqsv {
 
 constexpr int AJ_THREADS = 256;
 
-__device__ __forceinline__ void block_accum(float (&g)[4], int P, float* __restrict__ dst) {
+// The workgroup's P partial sums, written (not added) to its own 4 slots of
+// the partial buffer: part[(row * gridDim.x + blockIdx.x) * 4 + q].
+// k_rowpart_reduce adds them up in a fixed order afterwards, so gradients do
+// not depend on the order in which workgroups finish.
+__device__ __forceinline__ void block_partial(float (&g)[4], int P, float* __restrict__ dst) {
   __shared__ float red[4][AJ_THREADS / 64];
   const int w = threadIdx.x >> 6;
 #pragma unroll
@@ -37,7 +44,29 @@ __device__ __forceinline__ void block_accum(float (&g)[4], int P, float* __restr
   if (threadIdx.x < (unsigned)P) {
     float s = 0.f;
     for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[threadIdx.x][i];
-    atomicAdd(dst + threadIdx.x, s);
+    dst[threadIdx.x] = s;
+  }
+}
+
+// grad[row * gstride + gslot + q] += sum over x of part[(row * nx + x) * 4 + q],
+// one workgroup per batch row: each thread sums a strided subset in order,
+// then a fixed tree in LDS -- the same order on every run.
+__global__ __launch_bounds__(AJ_THREADS) void k_rowpart_reduce(const float* __restrict__ part, int64_t nx, int P,
+                                                               float* __restrict__ grad, int64_t gstride,
+                                                               int gslot) {
+  __shared__ float red[AJ_THREADS];
+  const uint64_t row = blockIdx.x;
+  for (int q = 0; q < P; ++q) {
+    float s = 0.f;
+    for (int64_t x = threadIdx.x; x < nx; x += AJ_THREADS) s += part[((int64_t)row * nx + x) * 4 + q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = AJ_THREADS / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) grad[row * gstride + gslot + q] += red[0];
+    __syncthreads();
   }
 }
 
@@ -45,8 +74,7 @@ template <bool BATCHED>
 __global__ __launch_bounds__(AJ_THREADS) void k_adj_1q(float2* __restrict__ psi, float2* __restrict__ lam, int n,
                                                       uint64_t items_row, InsertSpec ins, uint64_t cval,
                                                       uint64_t tmask, const float2* __restrict__ mat,
-                                                      int64_t bstride, int P, float* __restrict__ grad,
-                                                      int64_t gstride, int gslot) {
+                                                      int64_t bstride, int P, float* __restrict__ part) {
   const uint64_t row = blockIdx.y;
   const float2* M = BATCHED ? mat + row * bstride : mat;
   const float2 u0 = M[0], u1 = M[1], u2 = M[2], u3 = M[3];
@@ -74,15 +102,14 @@ __global__ __launch_bounds__(AJ_THREADS) void k_adj_1q(float2* __restrict__ psi,
     lam[i0] = cfmac(u2, l1, cmulc(u0, l0));
     lam[i1] = cfmac(u3, l1, cmulc(u1, l0));
   }
-  block_accum(g, P, grad + row * gstride + gslot);
+  block_partial(g, P, part + (row * gridDim.x + blockIdx.x) * 4);
 }
 
 template <bool BATCHED>
 __global__ __launch_bounds__(AJ_THREADS) void k_adj_2q(float2* __restrict__ psi, float2* __restrict__ lam, int n,
                                                       uint64_t items_row, InsertSpec ins, uint64_t cval,
                                                       uint64_t mhi, uint64_t mlo, const float2* __restrict__ mat,
-                                                      int64_t bstride, int P, float* __restrict__ grad,
-                                                      int64_t gstride, int gslot) {
+                                                      int64_t bstride, int P, float* __restrict__ part) {
   const uint64_t row = blockIdx.y;
   const float2* M = BATCHED ? mat + row * bstride : mat;
   float g[4] = {0.f, 0.f, 0.f, 0.f};
@@ -125,15 +152,14 @@ __global__ __launch_bounds__(AJ_THREADS) void k_adj_2q(float2* __restrict__ psi,
       lam[idx[r]] = acc;
     }
   }
-  block_accum(g, P, grad + row * gstride + gslot);
+  block_partial(g, P, part + (row * gridDim.x + blockIdx.x) * 4);
 }
 
 template <bool BATCHED>
 __global__ __launch_bounds__(AJ_THREADS) void k_adj_diag(float2* __restrict__ psi, float2* __restrict__ lam, int n,
                                                         uint64_t items_row, InsertSpec ins, uint64_t cval,
                                                         uint64_t pmask, const float2* __restrict__ ph,
-                                                        int64_t bstride, int P, float* __restrict__ grad,
-                                                        int64_t gstride, int gslot) {
+                                                        int64_t bstride, int P, float* __restrict__ part) {
   const uint64_t row = blockIdx.y;
   const float2* M = BATCHED ? ph + row * bstride : ph;
   const float2 p0 = M[0], p1 = M[1];
@@ -157,7 +183,7 @@ __global__ __launch_bounds__(AJ_THREADS) void k_adj_diag(float2* __restrict__ ps
     psi[i] = y;
     lam[i] = cmulc(f, l);
   }
-  block_accum(g, P, grad + row * gstride + gslot);
+  block_partial(g, P, part + (row * gridDim.x + blockIdx.x) * 4);
 }
 
 // Bit 0 free of targets and controls: items are float4 (two adjacent
@@ -168,8 +194,7 @@ template <bool BATCHED, bool NT>
 __global__ __launch_bounds__(AJ_THREADS) void k_adj_1q4(float4* __restrict__ psi, float4* __restrict__ lam, int n,
                                                        uint64_t items_row, InsertSpec ins4, uint64_t cval4,
                                                        uint64_t S, const float2* __restrict__ mat, int64_t bstride,
-                                                       int P, float* __restrict__ grad, int64_t gstride, int gslot,
-                                                       int reps) {
+                                                       int P, float* __restrict__ part, int reps) {
   const uint64_t row = blockIdx.y;
   const float2* M = BATCHED ? mat + row * bstride : mat;
   const float2 u0 = M[0], u1 = M[1], u2 = M[2], u3 = M[3];
@@ -207,15 +232,14 @@ __global__ __launch_bounds__(AJ_THREADS) void k_adj_1q4(float4* __restrict__ psi
     st4<NT>(lam + f0, make_float4(m0[0].x, m0[0].y, m0[1].x, m0[1].y));
     st4<NT>(lam + f1, make_float4(m1[0].x, m1[0].y, m1[1].x, m1[1].y));
   }
-  block_accum(g, P, grad + row * gstride + gslot);
+  block_partial(g, P, part + (row * gridDim.x + blockIdx.x) * 4);
 }
 
 template <bool BATCHED, bool NT>
 __global__ __launch_bounds__(AJ_THREADS) void k_adj_diag4(float4* __restrict__ psi, float4* __restrict__ lam, int n,
                                                          uint64_t items_row, InsertSpec ins4, uint64_t cval4,
                                                          uint64_t pmask, const float2* __restrict__ ph,
-                                                         int64_t bstride, int P, float* __restrict__ grad,
-                                                         int64_t gstride, int gslot, int reps) {
+                                                         int64_t bstride, int P, float* __restrict__ part, int reps) {
   const uint64_t row = blockIdx.y;
   const float2* M = BATCHED ? ph + row * bstride : ph;
   const float2 p0 = M[0], p1 = M[1];
@@ -246,7 +270,7 @@ __global__ __launch_bounds__(AJ_THREADS) void k_adj_diag4(float4* __restrict__ p
     st4<NT>(psi + rbase + j4, make_float4(y[0].x, y[0].y, y[1].x, y[1].y));
     st4<NT>(lam + rbase + j4, make_float4(m[0].x, m[0].y, m[1].x, m[1].y));
   }
-  block_accum(g, P, grad + row * gstride + gslot);
+  block_partial(g, P, part + (row * gridDim.x + blockIdx.x) * 4);
 }
 
 // 2-qubit gate, bit 0 free: items are float4 quadruples (positions in float4 bits)
@@ -254,8 +278,7 @@ template <bool BATCHED, bool NT>
 __global__ __launch_bounds__(AJ_THREADS) void k_adj_2q4(float4* __restrict__ psi, float4* __restrict__ lam, int n,
                                                        uint64_t items_row, InsertSpec ins4, uint64_t cval4,
                                                        uint64_t mhi4, uint64_t mlo4, const float2* __restrict__ mat,
-                                                       int64_t bstride, int P, float* __restrict__ grad,
-                                                       int64_t gstride, int gslot, int reps) {
+                                                       int64_t bstride, int P, float* __restrict__ part, int reps) {
   const uint64_t row = blockIdx.y;
   const float2* M = BATCHED ? mat + row * bstride : mat;
   float g[4] = {0.f, 0.f, 0.f, 0.f};
@@ -325,7 +348,7 @@ __global__ __launch_bounds__(AJ_THREADS) void k_adj_2q4(float4* __restrict__ psi
       st4<NT>(lam + idx[r], lv[r]);
     }
   }
-  block_accum(g, P, grad + row * gstride + gslot);
+  block_partial(g, P, part + (row * gridDim.x + blockIdx.x) * 4);
 }
 
 // tiles per row for the float4 adjoint kernels: 4 items per thread, more when
@@ -362,18 +385,37 @@ static dim3 adj_grid(uint64_t items_row, int64_t bsz) {
 
 using namespace tqsv;
 
-static int adj_common(void* psi, void* lam, int64_t bsz, int n, const void* mat, int P, float* grad) {
-  if (!psi || !lam || !mat || (P > 0 && !grad)) { set_error("adjoint: null pointer"); return E_ARG; }
+static int adj_common(void* psi, void* lam, int64_t bsz, int n, const void* mat, int P, float* grad,
+                      float* work) {
+  if (!psi || !lam || !mat || (P > 0 && (!grad || !work))) { set_error("adjoint: null pointer"); return E_ARG; }
   if (bsz < 1 || bsz > 65535 || n < 1 || n > 40) { set_error("adjoint: bad bsz/n (bsz <= 65535)"); return E_ARG; }
   if (P < 0 || P > 4) { set_error("adjoint: 0..4 parameters per gate (got %d)", P); return E_LIMIT; }
   return OK;
 }
 
+// The partial buffer must hold 4 floats per workgroup of the launch.
+static int adj_work_check(const dim3& grid, int P, int64_t work_floats, const char* who) {
+  if (P > 0 && (int64_t)grid.x * grid.y * 4 > work_floats) {
+    set_error("%s: partial buffer of %lld floats, the launch needs %lld (tqsv_adjoint_work_floats)", who,
+              (long long)work_floats, (long long)grid.x * grid.y * 4);
+    return E_ARG;
+  }
+  return OK;
+}
+
+static int adj_finish(hipStream_t s, const dim3& grid, int P, const float* work, float* grad, int64_t gstride,
+                      int gslot, const char* who) {
+  int rc = check_launch(who);
+  if (rc || P == 0) return rc;
+  k_rowpart_reduce<<<grid.y, AJ_THREADS, 0, s>>>(work, grid.x, P, grad, gstride, gslot);
+  return check_launch(who);
+}
+
 // mat: U (4 complex) followed by P derivative matrices dU/dtheta_p (4 each).
 TQSV_API int tqsv_adjoint_1q(hipStream_t s, void* psi, void* lam, int64_t bsz, int n, int tbit, uint64_t cmask,
                              uint64_t cval, const void* mat, int64_t mat_bstride, int P, float* grad,
-                             int64_t grad_stride, int gslot) {
-  int rc = adj_common(psi, lam, bsz, n, mat, P, grad);
+                             int64_t grad_stride, int gslot, float* work, int64_t work_floats) {
+  int rc = adj_common(psi, lam, bsz, n, mat, P, grad, work);
   if (rc) return rc;
   if (tbit < 0 || tbit >= n) { set_error("adjoint_1q: target out of range"); return E_ARG; }
   const uint64_t tmask = 1ull << tbit;
@@ -385,33 +427,35 @@ TQSV_API int tqsv_adjoint_1q(hipStream_t s, void* psi, void* lam, int64_t bsz, i
     const uint64_t items_row = 1ull << (n - 2 - c);
     int reps;
     const dim3 grid = adj_tiles(items_row, bsz, reps);
+    if ((rc = adj_work_check(grid, P, work_floats, "tqsv_adjoint_1q"))) return rc;
     const bool nt = stream_nt(2 * bsz, n);
     const uint64_t S = 1ull << (tbit - 1);
 #define ADJ1Q4(B, NTV)                                                                                        \
   k_adj_1q4<B, NTV><<<grid, AJ_THREADS, 0, s>>>((float4*)psi, (float4*)lam, n, items_row, ins, cval >> 1, S, \
-                                                (const float2*)mat, mat_bstride, P, grad, grad_stride, gslot, reps)
+                                                (const float2*)mat, mat_bstride, P, work, reps)
     if (mat_bstride) { if (nt) ADJ1Q4(true, true); else ADJ1Q4(true, false); }
     else { if (nt) ADJ1Q4(false, true); else ADJ1Q4(false, false); }
 #undef ADJ1Q4
-    return check_launch("tqsv_adjoint_1q");
+    return adj_finish(s, grid, P, work, grad, grad_stride, gslot, "tqsv_adjoint_1q");
   }
   if (!build_insert_adj(ins, cmask | tmask, n)) { set_error("adjoint_1q: too many controls"); return E_LIMIT; }
   const uint64_t items_row = 1ull << (n - 1 - c);
   const dim3 grid = adj_grid(items_row, bsz);
+  if ((rc = adj_work_check(grid, P, work_floats, "tqsv_adjoint_1q"))) return rc;
   if (mat_bstride)
     k_adj_1q<true><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, tmask,
-                                               (const float2*)mat, mat_bstride, P, grad, grad_stride, gslot);
+                                               (const float2*)mat, mat_bstride, P, work);
   else
     k_adj_1q<false><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, tmask,
-                                                (const float2*)mat, 0, P, grad, grad_stride, gslot);
-  return check_launch("tqsv_adjoint_1q");
+                                                (const float2*)mat, 0, P, work);
+  return adj_finish(s, grid, P, work, grad, grad_stride, gslot, "tqsv_adjoint_1q");
 }
 
 // mat: U (16 complex, index 2*b(hi)+b(lo)) followed by P derivative matrices.
 TQSV_API int tqsv_adjoint_2q(hipStream_t s, void* psi, void* lam, int64_t bsz, int n, int bit_hi, int bit_lo,
                              uint64_t cmask, uint64_t cval, const void* mat, int64_t mat_bstride, int P,
-                             float* grad, int64_t grad_stride, int gslot) {
-  int rc = adj_common(psi, lam, bsz, n, mat, P, grad);
+                             float* grad, int64_t grad_stride, int gslot, float* work, int64_t work_floats) {
+  int rc = adj_common(psi, lam, bsz, n, mat, P, grad, work);
   if (rc) return rc;
   if (bit_hi < 0 || bit_hi >= n || bit_lo < 0 || bit_lo >= n || bit_hi == bit_lo) {
     set_error("adjoint_2q: bad target bits");
@@ -427,33 +471,35 @@ TQSV_API int tqsv_adjoint_2q(hipStream_t s, void* psi, void* lam, int64_t bsz, i
     const uint64_t items_row = 1ull << (n - 3 - c);
     int reps;
     const dim3 grid = adj_tiles(items_row, bsz, reps);
+    if ((rc = adj_work_check(grid, P, work_floats, "tqsv_adjoint_2q"))) return rc;
     const bool nt = stream_nt(2 * bsz, n);
 #define ADJ2Q4(B, NTV)                                                                                        \
   k_adj_2q4<B, NTV><<<grid, AJ_THREADS, 0, s>>>((float4*)psi, (float4*)lam, n, items_row, ins, cval >> 1,    \
-                                                mhi >> 1, mlo >> 1, (const float2*)mat, mat_bstride, P, grad, \
-                                                grad_stride, gslot, reps)
+                                                mhi >> 1, mlo >> 1, (const float2*)mat, mat_bstride, P, work, \
+                                                reps)
     if (mat_bstride) { if (nt) ADJ2Q4(true, true); else ADJ2Q4(true, false); }
     else { if (nt) ADJ2Q4(false, true); else ADJ2Q4(false, false); }
 #undef ADJ2Q4
-    return check_launch("tqsv_adjoint_2q");
+    return adj_finish(s, grid, P, work, grad, grad_stride, gslot, "tqsv_adjoint_2q");
   }
   if (!build_insert_adj(ins, all, n)) { set_error("adjoint_2q: too many controls"); return E_LIMIT; }
   const uint64_t items_row = 1ull << (n - 2 - c);
   const dim3 grid = adj_grid(items_row, bsz);
+  if ((rc = adj_work_check(grid, P, work_floats, "tqsv_adjoint_2q"))) return rc;
   if (mat_bstride)
     k_adj_2q<true><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, mhi, mlo,
-                                               (const float2*)mat, mat_bstride, P, grad, grad_stride, gslot);
+                                               (const float2*)mat, mat_bstride, P, work);
   else
     k_adj_2q<false><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, mhi, mlo,
-                                                (const float2*)mat, 0, P, grad, grad_stride, gslot);
-  return check_launch("tqsv_adjoint_2q");
+                                                (const float2*)mat, 0, P, work);
+  return adj_finish(s, grid, P, work, grad, grad_stride, gslot, "tqsv_adjoint_2q");
 }
 
 // phases: (phi_even, phi_odd) followed by P derivative pairs.
 TQSV_API int tqsv_adjoint_diag(hipStream_t s, void* psi, void* lam, int64_t bsz, int n, uint64_t cmask,
                                uint64_t cval, uint64_t pmask, const void* phases, int64_t ph_bstride, int P,
-                               float* grad, int64_t grad_stride, int gslot) {
-  int rc = adj_common(psi, lam, bsz, n, phases, P, grad);
+                               float* grad, int64_t grad_stride, int gslot, float* work, int64_t work_floats) {
+  int rc = adj_common(psi, lam, bsz, n, phases, P, grad, work);
   if (rc) return rc;
   if ((n < 64 && ((cmask | pmask) >> n)) || (cval & ~cmask) || (cmask & pmask)) { set_error("adjoint_diag: bad masks"); return E_ARG; }
   InsertSpec ins;
@@ -463,24 +509,37 @@ TQSV_API int tqsv_adjoint_diag(hipStream_t s, void* psi, void* lam, int64_t bsz,
     const uint64_t items_row = 1ull << (n - 1 - c);
     int reps;
     const dim3 grid = adj_tiles(items_row, bsz, reps);
+    if ((rc = adj_work_check(grid, P, work_floats, "tqsv_adjoint_diag"))) return rc;
     const bool nt = stream_nt(2 * bsz, n);
 #define ADJD4(B, NTV)                                                                                           \
   k_adj_diag4<B, NTV><<<grid, AJ_THREADS, 0, s>>>((float4*)psi, (float4*)lam, n, items_row, ins, cval >> 1,    \
-                                                  pmask, (const float2*)phases, ph_bstride, P, grad, grad_stride, \
-                                                  gslot, reps)
+                                                  pmask, (const float2*)phases, ph_bstride, P, work, reps)
     if (ph_bstride) { if (nt) ADJD4(true, true); else ADJD4(true, false); }
     else { if (nt) ADJD4(false, true); else ADJD4(false, false); }
 #undef ADJD4
-    return check_launch("tqsv_adjoint_diag");
+    return adj_finish(s, grid, P, work, grad, grad_stride, gslot, "tqsv_adjoint_diag");
   }
   if (!build_insert_adj(ins, cmask, n)) { set_error("adjoint_diag: too many controls"); return E_LIMIT; }
   const uint64_t items_row = 1ull << (n - c);
   const dim3 grid = adj_grid(items_row, bsz);
+  if ((rc = adj_work_check(grid, P, work_floats, "tqsv_adjoint_diag"))) return rc;
   if (ph_bstride)
     k_adj_diag<true><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, pmask,
-                                                 (const float2*)phases, ph_bstride, P, grad, grad_stride, gslot);
+                                                 (const float2*)phases, ph_bstride, P, work);
   else
     k_adj_diag<false><<<grid, AJ_THREADS, 0, s>>>((float2*)psi, (float2*)lam, n, items_row, ins, cval, pmask,
-                                                  (const float2*)phases, 0, P, grad, grad_stride, gslot);
-  return check_launch("tqsv_adjoint_diag");
+                                                  (const float2*)phases, 0, P, work);
+  return adj_finish(s, grid, P, work, grad, grad_stride, gslot, "tqsv_adjoint_diag");
+}
+
+// Floats of partial buffer that any tqsv_adjoint_1q / _2q / _diag launch on a
+// [bsz, 2^n] state needs (4 per workgroup; the float4 tile kernels have the
+// most workgroups per row).
+TQSV_API int64_t tqsv_adjoint_work_floats(int64_t bsz, int n) {
+  if (bsz < 1 || n < 1 || n > 40) return 0;
+  int reps;
+  const dim3 t = adj_tiles(n >= 2 ? 1ull << (n - 1) : 1ull, bsz, reps);
+  const dim3 g = adj_grid(1ull << n, bsz);
+  const int64_t gx = t.x > g.x ? t.x : g.x;
+  return gx * bsz * 4;
 }

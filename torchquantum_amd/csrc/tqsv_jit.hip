@@ -23,6 +23,10 @@
 
 int stage_launch_shape(const int* prog_host, int n, int64_t bsz, int grid_cap, int& T, int64_t& n_groups,
                        int& grid, int64_t& gpw, size_t& shm);
+int64_t stage_rows_per_wg(const int* prog_host, int64_t gpw);
+int64_t stage_adjoint_work(const int* prog_host, int64_t gpw, int grid);
+int stage_adjoint_reduce(hipStream_t s, const int* prog_host, const int* prog_dev, int64_t bsz, int64_t gpw,
+                         int grid, const float* work, float* grad, int64_t grad_stride);
 using namespace tqsv;
 
 namespace {
@@ -157,17 +161,24 @@ TQSV_API int tqsv_jit_stage_forward(hipStream_t s, const void* func, void* state
 
 // One stage of the adjoint sweep with a compiled stage kernel.  Arguments as
 // tqsv_stage_adjoint, plus psi_in / lam_in: where psi and lam are read (NULL =
-// in place); the swept states are written to psi / lam.  Kernel parameters:
-// psi, lam, psi_in, lam_in, mats, grad, grad_stride, n_groups, groups_per_wg.
+// in place); the swept states are written to psi / lam.  The kernel writes
+// per-workgroup gradient partials to `work`, which a fixed-order reduction
+// then adds into grad (its gslot table read from prog_dev).  Kernel
+// parameters: psi, lam, psi_in, lam_in, mats, partials, rows per workgroup,
+// n_groups, groups_per_wg.
 TQSV_API int tqsv_jit_stage_adjoint(hipStream_t s, const void* func, void* psi, void* lam, const void* psi_in,
-                                    const void* lam_in, int64_t bsz, int n, const int* prog_host, const void* mats,
-                                    float* grad, int64_t grad_stride, int grid_cap) {
-  if (!func || !psi || !lam || !prog_host || !mats) {
+                                    const void* lam_in, int64_t bsz, int n, const int* prog_host,
+                                    const int* prog_dev, const void* mats, float* grad, int64_t grad_stride,
+                                    float* work, int64_t work_floats, int grid_cap) {
+  if (!func || !psi || !lam || !prog_host || !prog_dev || !mats) {
     set_error("tqsv_jit_stage_adjoint: null pointer argument");
     return E_ARG;
   }
   if (bsz < 1 || n < 4 || n > 32) { set_error("tqsv_jit_stage_adjoint: bad bsz/n"); return E_ARG; }
-  if (prog_host[kHdrNgp] > 0 && !grad) { set_error("tqsv_jit_stage_adjoint: grad buffer required"); return E_ARG; }
+  if (prog_host[kHdrNgp] > 0 && (!grad || !work)) {
+    set_error("tqsv_jit_stage_adjoint: grad and partial buffers required");
+    return E_ARG;
+  }
   int T, grid;
   int64_t n_groups, gpw;
   size_t shm;
@@ -175,15 +186,24 @@ TQSV_API int tqsv_jit_stage_adjoint(hipStream_t s, const void* func, void* psi, 
   if (Th < 6 || Th > 9) { set_error("tqsv_jit_stage_adjoint: compiled stages need 64..512 threads (6 <= T <= 9)"); return E_ARG; }
   int rc = stage_launch_shape(prog_host, n, bsz, grid_cap, T, n_groups, grid, gpw, shm);
   if (rc) return rc;
+  const int64_t need = stage_adjoint_work(prog_host, gpw, grid);
+  if (need > work_floats) {
+    set_error("tqsv_jit_stage_adjoint: partial buffer of %lld floats, the stage needs %lld "
+              "(tqsv_stage_adjoint_work_floats)", (long long)work_floats, (long long)need);
+    return E_ARG;
+  }
   void* p = psi;
   void* l = lam;
   const void* pin = psi_in ? psi_in : psi;
   const void* lin = lam_in ? lam_in : lam;
   const void* m = mats;
-  float* g = grad;
-  long long gs = grad_stride, ng = n_groups, gp = gpw;
-  void* args[] = {&p, &l, &pin, &lin, &m, &g, &gs, &ng, &gp};
+  float* part = work;
+  long long rpw = stage_rows_per_wg(prog_host, gpw), ng = n_groups, gp = gpw;
+  void* args[] = {&p, &l, &pin, &lin, &m, &part, &rpw, &ng, &gp};
   hipError_t e = hipModuleLaunchKernel((hipFunction_t)func, grid, 1, 1, 1u << T, 1, 1, 0, s, args, nullptr);
+  if (e != hipSuccess) { set_error("tqsv_jit_stage_adjoint: %s", hipGetErrorString(e)); return E_LAUNCH; }
+  stage_adjoint_reduce(s, prog_host, prog_dev, bsz, gpw, grid, work, grad, grad_stride);
+  e = hipGetLastError();
   if (e != hipSuccess) { set_error("tqsv_jit_stage_adjoint: %s", hipGetErrorString(e)); return E_LAUNCH; }
   return OK;
 }

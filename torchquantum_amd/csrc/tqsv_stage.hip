@@ -569,8 +569,8 @@ struct StageArgs {
   int init_zero;          // forward: 1 = start from |0..0> instead of reading psi_in
   int64_t n_groups;       // bsz << (n - m)
   int64_t groups_per_wg;
-  float* grad;            // adjoint: [bsz, grad_stride] accumulated with atomics
-  int64_t grad_stride;
+  float* part;            // adjoint: per-workgroup gradient partials [grid][rpw][n_gp]
+  int64_t rpw;            // adjoint: rows per workgroup slot count of `part`
 };
 
 __device__ __forceinline__ uint32_t group_offset(const int* __restrict__ nlpos, int nnl, uint64_t rest) {
@@ -638,14 +638,21 @@ __global__ __launch_bounds__(512) void k_stage_fwd(float2* __restrict__ out, con
   }
 }
 
+// Gradient partials: each wave adds its per-op totals into its own LDS row
+// (gacc[wave][n_gp], after the remap buffer; no atomics), and at every row
+// change the workgroup writes the sum over waves, in wave order, to its slot
+// of A.part: part[(blockIdx.x * rpw + row - first row) * n_gp + t].
+// k_stagepart_reduce adds those up per row in workgroup order, so gradients
+// are the same on every run.
 template <int FEAT>
 __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, float2* __restrict__ lam,
                                                    const int* __restrict__ prog, const float2* __restrict__ mats,
-                                                   float* __restrict__ grad, StageArgs A) {
+                                                   StageArgs A) {
   extern __shared__ float2 lds[];
-  __shared__ float gacc[MAX_GP];
   const int* __restrict__ h = prog;
   const int n_ops = h[H_NOPS], T = h[H_T], nnl = h[H_NNL], n_gp = h[H_NGP];
+  const int nwave = blockDim.x >= 64 ? (int)(blockDim.x >> 6) : 1;
+  float* __restrict__ gacc = reinterpret_cast<float*>(lds + (1 << h[H_M]));
   const int* __restrict__ tpos = h + H_POS;
   const int* __restrict__ rpos = tpos + T;
   const int* __restrict__ nlpos = rpos + R;
@@ -654,7 +661,8 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wwidth = blockDim.x < 64 ? (int)blockDim.x : 64;
-  for (int t = tid; t < n_gp; t += blockDim.x) gacc[t] = 0.f;
+  float* __restrict__ wacc = gacc + (tid >> 6) * n_gp;   // this wave's row
+  for (int t = tid; t < nwave * n_gp; t += blockDim.x) gacc[t] = 0.f;
   uint32_t toff = 0;
   for (int i = 0; i < T; ++i) toff |= (uint32_t)((tid >> i) & 1) << tpos[i];
   uint32_t roff[NS];
@@ -669,6 +677,8 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
   int64_t g1 = g0 + A.groups_per_wg;
   if (g1 > A.n_groups) g1 = A.n_groups;
   const uint64_t rest_mask = (1ull << nnl) - 1ull;
+  const int64_t row0 = g0 >> nnl;
+  float* __restrict__ wpart = A.part + (int64_t)blockIdx.x * A.rpw * n_gp;
   int64_t cur_row = -1;
   __syncthreads();
   for (int64_t g = g0; g < g1; ++g) {
@@ -677,9 +687,12 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
       if (cur_row >= 0) {
         __syncthreads();
         for (int t = tid; t < n_gp; t += blockDim.x) {
-          const float v = gacc[t];
-          if (v != 0.f) atomicAdd(grad + cur_row * A.grad_stride + gslot[t], v);
-          gacc[t] = 0.f;
+          float v = 0.f;
+          for (int w = 0; w < nwave; ++w) {
+            v += gacc[w * n_gp + t];
+            gacc[w * n_gp + t] = 0.f;
+          }
+          wpart[(cur_row - row0) * n_gp + t] = v;
         }
         __syncthreads();
       }
@@ -714,26 +727,26 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
           // halving exchange: every lane ends with one value's wave total
           if (P > 4) {
             const float c = wave_reduce_t<8>(gq, lane);
-            if ((lane & 7) == 0) atomicAdd(&gacc[w[7] + (lane >> 3)], c);
+            if ((lane & 7) == 0) wacc[w[7] + (lane >> 3)] += c;
           } else if (P > 2) {
             const float v4[4] = {gq[0], gq[1], gq[2], gq[3]};
             const float c = wave_reduce_t<4>(v4, lane);
-            if ((lane & 15) == 0 && (lane >> 4) < P) atomicAdd(&gacc[w[7] + (lane >> 4)], c);
+            if ((lane & 15) == 0 && (lane >> 4) < P) wacc[w[7] + (lane >> 4)] += c;
           } else if (P == 2) {
             const float v2[2] = {gq[0], gq[1]};
             const float c = wave_reduce_t<2>(v2, lane);
-            if ((lane & 31) == 0) atomicAdd(&gacc[w[7] + (lane >> 5)], c);
+            if ((lane & 31) == 0) wacc[w[7] + (lane >> 5)] += c;
           } else {
             const float v1[1] = {gq[0]};
             const float c = wave_reduce_t<1>(v1, lane);
-            if (lane == 0) atomicAdd(&gacc[w[7]], c);
+            if (lane == 0) wacc[w[7]] += c;
           }
         } else {
 #pragma unroll
           for (int q = 0; q < 8; ++q) {
             if (q < P) {
               const float s = wave_sum(gq[q], wwidth);
-              if (lane == 0) atomicAdd(&gacc[w[7] + q], s);
+              if (lane == 0) wacc[w[7] + q] += s;
             }
           }
         }
@@ -753,10 +766,32 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
   if (n_gp > 0 && cur_row >= 0) {
     __syncthreads();
     for (int t = tid; t < n_gp; t += blockDim.x) {
-      const float v = gacc[t];
-      if (v != 0.f) atomicAdd(grad + cur_row * A.grad_stride + gslot[t], v);
+      float v = 0.f;
+      for (int w = 0; w < nwave; ++w) v += gacc[w * n_gp + t];
+      wpart[(cur_row - row0) * n_gp + t] = v;
     }
   }
+}
+
+// grad[row * gstride + gslot[t]] += the partials of every workgroup that
+// covered `row`, added in workgroup order (one thread per (row, t)).
+// Workgroup w covers groups [w * gpw, (w + 1) * gpw) and wrote its partials of
+// row r to part[(w * rpw + r - (w * gpw >> nnl)) * n_gp + t].
+__global__ void k_stagepart_reduce(const float* __restrict__ part, const int* __restrict__ prog, int64_t bsz,
+                                   int64_t gpw, int64_t grid, int64_t rpw, float* __restrict__ grad,
+                                   int64_t gstride) {
+  const int n_gp = prog[H_NGP], nnl = prog[H_NNL];
+  const int* __restrict__ gslot = prog + prog[H_OFF_GSLOT];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= bsz * n_gp) return;
+  const int64_t row = i / n_gp;
+  const int t = (int)(i - row * n_gp);
+  const int64_t w0 = (row << nnl) / gpw;
+  int64_t w1 = (((row + 1) << nnl) - 1) / gpw;
+  if (w1 > grid - 1) w1 = grid - 1;
+  float s = 0.f;
+  for (int64_t w = w0; w <= w1; ++w) s += part[(w * rpw + row - ((w * gpw) >> nnl)) * n_gp + t];
+  grad[row * gstride + gslot[t]] += s;
 }
 
 // ------------------------------------------------------------------ batch
@@ -990,32 +1025,76 @@ TQSV_API int tqsv_stage_forward(hipStream_t s, void* state_out, const void* stat
 
 // psi / lam are updated in place; grad ([bsz, grad_stride] float) is
 // accumulated into (the caller zeroes it once per backward pass).
+// Rows a workgroup of gpw consecutive groups can touch (2^nnl groups per row).
+int64_t stage_rows_per_wg(const int* prog_host, int64_t gpw) {
+  const int64_t gpr = 1ll << prog_host[H_NNL];
+  return (gpw - 1 + gpr - 1) / gpr + 1;
+}
+
+// Floats of partial buffer an adjoint launch of this stage needs.
+int64_t stage_adjoint_work(const int* prog_host, int64_t gpw, int grid) {
+  return (int64_t)grid * stage_rows_per_wg(prog_host, gpw) * prog_host[H_NGP];
+}
+
+// After an adjoint stage launch: add its partials into grad in a fixed order.
+int stage_adjoint_reduce(hipStream_t s, const int* prog_host, const int* prog_dev, int64_t bsz, int64_t gpw,
+                         int grid, const float* work, float* grad, int64_t grad_stride) {
+  const int ngp = prog_host[H_NGP];
+  if (ngp == 0) return OK;
+  const int64_t total = bsz * ngp;
+  const int threads = 256;
+  k_stagepart_reduce<<<(unsigned)((total + threads - 1) / threads), threads, 0, s>>>(
+      work, prog_dev, bsz, gpw, grid, stage_rows_per_wg(prog_host, gpw), grad, grad_stride);
+  return OK;
+}
+
 TQSV_API int tqsv_stage_adjoint(hipStream_t s, void* psi, void* lam, int64_t bsz, int n, const int* prog_host,
                                 const int* prog_dev, const void* mats, float* grad, int64_t grad_stride,
-                                int grid_cap) {
+                                float* work, int64_t work_floats, int grid_cap) {
   if (!psi || !lam || !prog_host || !prog_dev || !mats) {
     set_error("tqsv_stage_adjoint: null pointer argument");
     return E_ARG;
   }
   if (bsz < 1 || n < R || n > 32) { set_error("tqsv_stage_adjoint: bad bsz/n"); return E_ARG; }
-  if (prog_host[H_NGP] > 0 && !grad) { set_error("tqsv_stage_adjoint: grad buffer required"); return E_ARG; }
+  if (prog_host[H_NGP] > 0 && (!grad || !work)) {
+    set_error("tqsv_stage_adjoint: grad and partial buffers required");
+    return E_ARG;
+  }
   int T, grid;
   int64_t n_groups, gpw;
   size_t shm;
   int rc = stage_launch_shape(prog_host, n, bsz, grid_cap, T, n_groups, grid, gpw, shm);
   if (rc) return rc;
-  StageArgs A{prog_dev, (const float2*)mats, n, 0, n_groups, gpw, grad, grad_stride};
+  const int64_t need = stage_adjoint_work(prog_host, gpw, grid);
+  if (need > work_floats) {
+    set_error("tqsv_stage_adjoint: partial buffer of %lld floats, the stage needs %lld "
+              "(tqsv_stage_adjoint_work_floats)", (long long)work_floats, (long long)need);
+    return E_ARG;
+  }
+  const int nwave = T >= 6 ? 1 << (T - 6) : 1;
+  shm += sizeof(float) * (size_t)nwave * prog_host[H_NGP];
+  StageArgs A{prog_dev, (const float2*)mats, n, 0, n_groups, gpw, work, stage_rows_per_wg(prog_host, gpw)};
   const int feat = prog_host[H_FLAGS] & 3;
   if (feat == 0)
-    k_stage_adj<0><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, prog_dev, (const float2*)mats,
-                                                      grad, A);
+    k_stage_adj<0><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, prog_dev, (const float2*)mats, A);
   else if (feat == FEAT_MAT2)
-    k_stage_adj<FEAT_MAT2><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, prog_dev, (const float2*)mats,
-                                                      grad, A);
+    k_stage_adj<FEAT_MAT2><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, prog_dev, (const float2*)mats, A);
   else
-    k_stage_adj<FEAT_MAT2 | FEAT_MATK><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, prog_dev, (const float2*)mats,
-                                                      grad, A);
+    k_stage_adj<FEAT_MAT2 | FEAT_MATK><<<grid, 1 << T, shm, s>>>((float2*)psi, (float2*)lam, prog_dev,
+                                                                (const float2*)mats, A);
+  if ((rc = check_launch("tqsv_stage_adjoint"))) return rc;
+  stage_adjoint_reduce(s, prog_host, prog_dev, bsz, gpw, grid, work, grad, grad_stride);
   return check_launch("tqsv_stage_adjoint");
+}
+
+// Floats of the partial buffer tqsv_stage_adjoint / tqsv_jit_stage_adjoint
+// need for this stage (same launch shape arguments).
+TQSV_API int64_t tqsv_stage_adjoint_work_floats(const int* prog_host, int n, int64_t bsz, int grid_cap) {
+  int T, grid;
+  int64_t n_groups, gpw;
+  size_t shm;
+  if (!prog_host || stage_launch_shape(prog_host, n, bsz, grid_cap, T, n_groups, grid, gpw, shm)) return -1;
+  return stage_adjoint_work(prog_host, gpw, grid);
 }
 
 // One launch over a list of (circuit, stage, group range) items; every item's

@@ -1228,6 +1228,7 @@ class Compiled:
         self._jit: Dict[str, Optional[Dict[int, int]]] = {}
         self._jit_jobs: Dict[str, object] = {}
         self._prune: Optional[tuple] = None
+        self._work_floats: Optional[int] = None
         self.mg_items = np.zeros(0, dtype=MG_ITEM)
         self.grad_width = max(self.n_slots, 1)
         if opts.mode == "fused" and n >= R_BITS:
@@ -1347,6 +1348,26 @@ class Compiled:
         table = None if funcs is None else {base: f for (base, _), f in zip(stages, funcs)}
         self._jit[kind] = table
         return table
+
+    def adjoint_work_floats(self) -> int:
+        """Floats of the partial buffer the adjoint launches of this plan need
+        (the largest over its stages and per-gate steps)."""
+        if self._work_floats is None:
+            lib = N.lib()
+            need = 0
+            if any(kind == "solo" and self.kops[p].P for kind, p in self.steps):
+                need = int(lib.tqsv_adjoint_work_floats(self.bsz, self.n))
+            for kind, payload in self.steps:
+                if kind == "stage":
+                    hp = self.prog_host[payload[0]:]
+                    w = int(lib.tqsv_stage_adjoint_work_floats(hp.ctypes.data_as(N.c_int_p), self.n, self.bsz,
+                                                               self.opts.grid_cap))
+                    if w < 0:
+                        raise RuntimeError("stage adjoint workspace query failed: "
+                                           + lib.tqsv_last_error().decode(errors="replace"))
+                    need = max(need, w)
+            self._work_floats = need
+        return self._work_floats
 
     def prune_info(self) -> tuple:
         """How the reverse sweep shrinks when no gradient with respect to the
@@ -1715,6 +1736,10 @@ def adjoint(c: Compiled, psi_src: torch.Tensor, lam_src: torch.Tensor, pool: tor
     grad = torch.zeros((bsz, c.grad_width), dtype=torch.float32, device=c.device)
     stream = N.stream_of(psi_src)
     gstride = grad.shape[1]
+    # per-workgroup gradient partials of one launch (added into grad in a
+    # fixed order by the library right after it; every launch reuses them)
+    work = torch.empty(max(c.adjoint_work_floats(), 1), dtype=torch.float32, device=c.device)
+    wptr, wlen = work.data_ptr(), work.numel()
     jit = c.jit_kernels("a") if c.prog_host is not None else None
     steps = list(reversed(c.steps))
     pj = None
@@ -1746,11 +1771,12 @@ def adjoint(c: Compiled, psi_src: torch.Tensor, lam_src: torch.Tensor, pool: tor
                 pin, lin = first_src if first_src is not None else (None, None)
                 first_src = None
                 N.call("tqsv_jit_stage_adjoint", stream, f, psi.data_ptr(), lam.data_ptr(), pin, lin, bsz, n,
-                       hp.ctypes.data_as(N.c_int_p), pool.data_ptr(), N.fptr(grad), gstride, c.opts.grid_cap)
+                       hp.ctypes.data_as(N.c_int_p), c.prog_dev.data_ptr() + 4 * base, pool.data_ptr(),
+                       N.fptr(grad), gstride, wptr, wlen, c.opts.grid_cap)
             else:
                 N.call("tqsv_stage_adjoint", stream, psi.data_ptr(), lam.data_ptr(), bsz, n,
                        hp.ctypes.data_as(N.c_int_p), c.prog_dev.data_ptr() + 4 * base, pool.data_ptr(),
-                       N.fptr(grad), gstride, c.opts.grid_cap)
+                       N.fptr(grad), gstride, wptr, wlen, c.opts.grid_cap)
             _timed_end(tok)
             continue
         i = payload
@@ -1761,16 +1787,16 @@ def adjoint(c: Compiled, psi_src: torch.Tensor, lam_src: torch.Tensor, pool: tor
             g0 = c.scatter_col.get(i, k.grad_slots[0])
             if k.kind == "mat1":
                 N.call("tqsv_adjoint_1q", stream, psi.data_ptr(), lam.data_ptr(), bsz, n, k.bits[0], k.cmask,
-                       k.cval, p, bs, k.P, N.fptr(grad), gstride, g0)
+                       k.cval, p, bs, k.P, N.fptr(grad), gstride, g0, wptr, wlen)
             elif k.kind == "mat2":
                 N.call("tqsv_adjoint_2q", stream, psi.data_ptr(), lam.data_ptr(), bsz, n, k.bits[0], k.bits[1],
-                       k.cmask, k.cval, p, bs, k.P, N.fptr(grad), gstride, g0)
+                       k.cmask, k.cval, p, bs, k.P, N.fptr(grad), gstride, g0, wptr, wlen)
             elif k.kind == "diag":
                 pm = 0
                 for b in k.bits:
                     pm |= 1 << b
                 N.call("tqsv_adjoint_diag", stream, psi.data_ptr(), lam.data_ptr(), bsz, n, k.cmask, k.cval, pm,
-                       p, bs, k.P, N.fptr(grad), gstride, g0)
+                       p, bs, k.P, N.fptr(grad), gstride, g0, wptr, wlen)
             else:
                 raise NotImplementedError(f"adjoint of trainable '{k.kind}' op")
         else:

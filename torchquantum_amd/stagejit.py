@@ -677,7 +677,7 @@ class _Gen:
             w0 = f(f"sum32({a[0]}, {a[0]})")
             u = f(f"sum16({w0}, {w0})")
             c = f(f"allsum8({u} + dpp_ror8({u}))")
-        self.line(f"if ({cond}) atomicAdd(&gacc[{w[7]} + (lane >> {sh})], {c});")
+        self.line(f"if ({cond}) gacc[wrow + {w[7]} + (lane >> {sh})] += {c};")
 
 
 def _split_addr(sp: StageProgram) -> bool:
@@ -882,21 +882,25 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
         drain += [f"{nm} = 0.f;" for nm in names]
     nt = 1 << sp.T
     ngp = sp.ngp
+    nwave = max(1, nt >> 6)
     src = []
-    if ngp:
-        src.append(f"__constant__ int {name}_gs[{ngp}] = {{{', '.join(str(int(x)) for x in sp.gslots)}}};")
     # psi / lam: where the swept states are written; psi_in / lam_in: where
     # they are read (the same buffers in place, or the forward output and
-    # the incoming gradient for the first stage of the sweep)
+    # the incoming gradient for the first stage of the sweep).  Gradient
+    # partials: each wave adds its per-op totals into its own row of gacc
+    # (no atomics); at every batch-row change the workgroup writes the sum
+    # over waves, in wave order, to part[(blockIdx.x * rpw + row - row0) *
+    # ngp + t], which a fixed-order reduction adds into the gradient.
     src.append(f'extern "C" __global__ void __launch_bounds__({nt}) {name}(float2* __restrict__ psi, '
                "float2* __restrict__ lam, const float2* psi_in, const float2* lam_in, "
-               "const float2* __restrict__ mats, float* __restrict__ grad, "
-               "long long grad_stride, long long n_groups, long long gpw) {")
+               "const float2* __restrict__ mats, float* __restrict__ part, "
+               "long long rpw, long long n_groups, long long gpw) {")
     if has_remap:
         src.append(f"  __shared__ float2 lds[{lds_words(sp.m, lds)}];")
     if ngp:
-        src.append(f"  __shared__ float gacc[{ngp}];")
-        src.append(f"  for (int t = threadIdx.x; t < {ngp}; t += {nt}) gacc[t] = 0.f;")
+        src.append(f"  __shared__ float gacc[{nwave * ngp}];")
+        src.append(f"  for (int t = threadIdx.x; t < {nwave * ngp}; t += {nt}) gacc[t] = 0.f;")
+        src.append(f"  const int wrow = (threadIdx.x >> 6) * {ngp};")
     for _, names in acc_ops:
         src.append(f"  float {', '.join(nm + ' = 0.f' for nm in names)};")
     src.append("  const int tid = threadIdx.x;")
@@ -908,6 +912,8 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
             "  long long g1 = g0 + gpw;",
             "  if (g1 > n_groups) g1 = n_groups;",
             "  long long cur_row = -1;",
+            f"  float* __restrict__ wpart = part + (long long)blockIdx.x * rpw * {max(ngp, 1)};",
+            f"  const long long row0 = g0 >> {sp.nnl};",
             "  __syncthreads();"]
     if prefetch:
         src += [f"  float2 {', '.join(f'pc{j}' for j in range(NS))};",
@@ -926,9 +932,9 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
         src += ["        " + x for x in drain]
         src += ["        __syncthreads();",
                 f"        for (int t = tid; t < {ngp}; t += {nt}) {{",
-                "          const float v = gacc[t];",
-                f"          if (v != 0.f) atomicAdd(grad + cur_row * grad_stride + {name}_gs[t], v);",
-                "          gacc[t] = 0.f;",
+                "          float v = 0.f;",
+                f"          for (int w = 0; w < {nwave}; ++w) {{ v += gacc[w * {ngp} + t]; gacc[w * {ngp} + t] = 0.f; }}",
+                f"          wpart[(cur_row - row0) * {ngp} + t] = v;",
                 "        }",
                 "        __syncthreads();",
                 "      }",
@@ -957,8 +963,9 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
         src += ["    " + x for x in drain]
         src += ["    __syncthreads();",
                 f"    for (int t = tid; t < {ngp}; t += {nt}) {{",
-                "      const float v = gacc[t];",
-                f"      if (v != 0.f) atomicAdd(grad + cur_row * grad_stride + {name}_gs[t], v);",
+                "      float v = 0.f;",
+                f"      for (int w = 0; w < {nwave}; ++w) v += gacc[w * {ngp} + t];",
+                f"      wpart[(cur_row - row0) * {ngp} + t] = v;",
                 "    }",
                 "  }"]
     src.append("}")
