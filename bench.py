@@ -494,13 +494,14 @@ def config5(args, tq):
 
     def build(mode):
         q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
-        q.engine_options = tq.EngineOptions(mode=mode)
+        q.engine_options = (tq.EngineOptions(mode=mode, fusion_width=args.c5_fusion_width) if mode == "fused"
+                            else tq.EngineOptions(mode=mode))
         for op, p in zip(ops, params):
             tqf.func_name_dict[op["name"]](q, op["wires"], params=p)
         return q
 
     res = {"workload": f"{n} qubits, bsz {bsz}, 20 x [RY all, CZ even+odd] + Toffoli/CSWAP every 5 layers",
-           "gates": len(ops)}
+           "gates": len(ops), "fusion_width": args.c5_fusion_width}
     with torch.no_grad():
         # one untimed fused pass plans the circuit and starts the background
         # compile of its stage kernels; the timed passes run the compiled ones
@@ -807,6 +808,8 @@ def main():
                     help="run only the per-gate forward of the 20-qubit RY/CZ circuit (--steps, --warmup)")
     ap.add_argument("--stage-bits", type=int, default=0,
                     help="local bits per stage for the gradient workload (default: the engine's choice)")
+    ap.add_argument("--c5-fusion-width", type=int, default=1,
+                    help="fusion width of the 30-qubit workload's fused plan (1 = merged single-wire runs)")
     ap.add_argument("--config5-plans", action="store_true",
                     help="only the 30-qubit plan choice: learned planner vs exhaustive timing")
     ap.add_argument("--only-small", action="store_true",
@@ -869,6 +872,7 @@ def main():
         amps = float(ab) * (1 << an)
         print(json.dumps({"fused_gradient_s": dt, "workload": f"{an} qubits, bsz {ab}", "loss": loss,
                           "adjoint_stage": stage_launch_stats(rec, args.steps, "a", amps, 32.0),
+                          "pruned_adjoint_stage": stage_launch_stats(rec, args.steps, "a_pruned", amps, 32.0),
                           "final_adjoint_stage": stage_launch_stats(rec, args.steps, "a_read", amps, 16.0),
                           "forward_stage": stage_launch_stats(rec, args.steps, "f", amps, 16.0)}), flush=True)
         return
@@ -896,6 +900,7 @@ def main():
     log(f"[bench] fused {dt_af:.4f} s/gradient")
     adj = stage_launch_stats(rec, args.steps, "a", amps, 32.0)
     adj_last = stage_launch_stats(rec, args.steps, "a_read", amps, 16.0)
+    adj_pruned = stage_launch_stats(rec, args.steps, "a_pruned", amps, 32.0)
     fwd = stage_launch_stats(rec, args.steps, "f", amps, 16.0)
     del rec
     torch.cuda.empty_cache()
@@ -918,6 +923,9 @@ def main():
         "fused_interpreting_stage_kernels": {"grads_per_s": 1.0 / dt_ai, "s_per_gradient": dt_ai},
         "forward_stage": None if fwd is None else {**fwd, "frac": fwd["achieved_GBs"] / HBM_PEAK_GBS,
                                                    "kernel": "compiled forward stage tqsv_jf_* (16 B/amplitude)"},
+        "pruned_adjoint_stages": None if adj_pruned is None else {
+            **adj_pruned, "kernel": "pruned compiled adjoint stages tqsv_jp_* (ops on wires no earlier trainable op "
+                                    "touches are not un-applied; psi and lam read and written: 32 B/amplitude)"},
         "final_adjoint_stage": None if adj_last is None else {
             **adj_last, "frac": adj_last["achieved_GBs"] / HBM_PEAK_GBS,
             "kernel": "last stage of the pruned reverse sweep (no input-state gradient): reads psi and lam, "

@@ -1762,11 +1762,12 @@ def adjoint(c: Compiled, psi_src: torch.Tensor, lam_src: torch.Tensor, pool: tor
             base, stg = payload
             hp = c.prog_host[base:]
             f = jit.get(base) if jit else None
-            read_only = False
+            tag = "a"
             if pj and base in pj:
                 f = pj[base]
-                read_only = base == last_base     # the pruned sweep's last stage stores nothing
-            tok = _timed("a_read" if read_only else "a", f is not None)
+                # pruned kernels: fewer un-applies; the sweep's last stage stores nothing
+                tag = "a_read" if base == last_base else "a_pruned"
+            tok = _timed(tag, f is not None)
             if f is not None:
                 pin, lin = first_src if first_src is not None else (None, None)
                 first_src = None
