@@ -79,6 +79,7 @@ def time_forward(n, bsz, ops, mode, steps, warmup, tq, jit="auto"):
 
 
 STAGE_BITS = 0      # local bits per stage of the gradient workload's plans (0 = the engine's choice)
+LOW_BITS = None     # lowest bits kept local in every stage (None = the engine's choice)
 
 
 def time_replay(n, bsz, ops, steps, warmup, tq):
@@ -122,7 +123,8 @@ def _ansatz_loss_step(n, bsz, layers, mode, tq, jit, params, phi, ops, obs, coef
     for p in params:
         p.grad = None
     q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
-    q.engine_options = tq.EngineOptions(mode=mode, jit=jit, stage_bits=STAGE_BITS)
+    q.engine_options = tq.EngineOptions(mode=mode, jit=jit, stage_bits=STAGE_BITS,
+                                        **({"low_bits": LOW_BITS} if LOW_BITS is not None else {}))
     for w in range(n):
         tqf.ry(q, w, params=phi)
     k = 0
@@ -494,8 +496,9 @@ def config5(args, tq):
 
     def build(mode):
         q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
-        q.engine_options = (tq.EngineOptions(mode=mode, fusion_width=args.c5_fusion_width) if mode == "fused"
-                            else tq.EngineOptions(mode=mode))
+        q.engine_options = (tq.EngineOptions(mode=mode, fusion_width=args.c5_fusion_width,
+                                             **({"low_bits": LOW_BITS} if LOW_BITS is not None else {}))
+                            if mode == "fused" else tq.EngineOptions(mode=mode))
         for op, p in zip(ops, params):
             tqf.func_name_dict[op["name"]](q, op["wires"], params=p)
         return q
@@ -806,6 +809,8 @@ def main():
                     help="run only the fused forward of the 20-qubit RY/CZ circuit (--steps, --warmup)")
     ap.add_argument("--only-pergate", action="store_true",
                     help="run only the per-gate forward of the 20-qubit RY/CZ circuit (--steps, --warmup)")
+    ap.add_argument("--low-bits", type=int, default=None,
+                    help="lowest state bits kept local in every stage of the gradient workload (default: the engine's)")
     ap.add_argument("--stage-bits", type=int, default=0,
                     help="local bits per stage for the gradient workload (default: the engine's choice)")
     ap.add_argument("--c5-fusion-width", type=int, default=1,
@@ -832,8 +837,9 @@ def main():
         sys.exit(2)
     import torchquantum_amd as tq
     from circuits import config2_ops
-    global STAGE_BITS
+    global STAGE_BITS, LOW_BITS
     STAGE_BITS = args.stage_bits
+    LOW_BITS = args.low_bits
 
     if args.only_roofline:
         cb, cms = copy_calibration()

@@ -209,3 +209,35 @@ def test_gradients_bitwise_reproducible(label):
     for loss, g in runs[1:]:
         assert loss == runs[0][0]
         assert np.array_equal(g, runs[0][1]), float(np.abs(g - runs[0][1]).max())
+
+
+@gpu
+def test_adjoint_rejects_short_partial_buffer():
+    """Adjoint entry points check the partial buffer against the launch
+    before launching: a short buffer is an error, not an out-of-bounds
+    write."""
+    from torchquantum_amd import _native as N, engine as E, gates as G
+    lib = N.lib()
+    n, bsz = 12, 2
+    psi = torch.zeros(bsz, 1 << n, dtype=torch.complex64, device=DEV)
+    lam = torch.zeros_like(psi)
+    grad = torch.zeros(bsz, 8, dtype=torch.float32, device=DEV)
+    mat = torch.zeros(8, dtype=torch.complex64, device=DEV)
+    work = torch.zeros(4, dtype=torch.float32, device=DEV)
+    need = lib.tqsv_adjoint_work_floats(bsz, n)
+    assert need > 4
+    s = N.stream_of(psi)
+    rc = lib.tqsv_adjoint_1q(s, psi.data_ptr(), lam.data_ptr(), bsz, n, 3, 0, 0, mat.data_ptr(), 0, 1,
+                             N.fptr(grad), 8, 0, work.data_ptr(), 4)
+    assert rc == -1 and b"partial buffer" in lib.tqsv_last_error()
+    tape = [E.TapeOp("rx", (w,), torch.full((1, 1), 0.3, device=DEV, requires_grad=True), False) for w in range(n)]
+    c = E.Compiled(tape, n, bsz, E.EngineOptions(mode="fused", stage_bits=10, jit="off"), DEV)
+    base, st = next(p for k, p in c.steps if k == "stage")
+    hp = c.prog_host[base:]
+    need_st = lib.tqsv_stage_adjoint_work_floats(hp.ctypes.data_as(N.c_int_p), n, bsz, 0)
+    assert need_st > 4
+    rc = lib.tqsv_stage_adjoint(s, psi.data_ptr(), lam.data_ptr(), bsz, n, hp.ctypes.data_as(N.c_int_p),
+                                c.prog_dev.data_ptr() + 4 * base, mat.data_ptr(), N.fptr(grad), 8,
+                                work.data_ptr(), 4, 0)
+    assert rc == -1 and b"partial buffer" in lib.tqsv_last_error()
+    torch.cuda.synchronize()

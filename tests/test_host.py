@@ -48,6 +48,11 @@ def test_abi_rejects_bad_arguments_without_gpu():
     assert lib.tqsv_apply_2q(None, 16, 1, 4, 1, 1, 0, 0, 16, 0) == -1
     assert lib.tqsv_expval_workspace_bytes(1, 40, 1) == -1
     assert lib.tqsv_sample_workspace_bytes(1, 33) == -1
+    # a trainable per-gate adjoint step without its partial buffer
+    import ctypes
+    fake_grad = ctypes.cast(16, ctypes.POINTER(ctypes.c_float))
+    assert lib.tqsv_adjoint_1q(None, 16, 16, 1, 4, 0, 0, 0, 16, 0, 1, fake_grad, 1, 0, None, 0) == -1
+    assert b"null" in lib.tqsv_last_error()
 
 
 @pytest.mark.skipif(not _native.lib_available(), reason="libtqsv.so not built")
