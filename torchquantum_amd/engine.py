@@ -98,6 +98,9 @@ class EngineOptions:
                  are cached on disk) while the interpreting kernels run, and
                  switches once they are ready (wait_for_compiled_stages()
                  blocks until then).  "on" compiles before the first pass.
+    reorder_ops: within a stage, run commuting ops in an order that needs
+                 fewer register-set changes (LDS remaps); ops sharing a wire
+                 keep their order.
     """
     mode: str = "fused"
     stage_bits: int = 0
@@ -108,10 +111,11 @@ class EngineOptions:
     adjoint_mg: bool = True
     jit: str = "auto"
     device_gates: bool = True
+    reorder_ops: bool = True
 
     def key(self):
         return (self.mode, self.stage_bits, self.low_bits, self.merge_1q, self.fusion_width, self.grid_cap,
-                self.adjoint_mg, self.jit, self.device_gates)
+                self.adjoint_mg, self.jit, self.device_gates, self.reorder_ops)
 
 
 DEFAULT_OPTIONS = EngineOptions()
@@ -965,10 +969,53 @@ def reg_targets(k: KOp) -> Tuple[int, ...]:
     return ()
 
 
+def order_for_registers(inc: List[int], kops: List[KOp], lbit: Dict[int, int], canon: List[int]) -> List[int]:
+    """Reorder a stage's ops so that runs of them share a register set (fewer
+    LDS remaps).  Ops that share a bit keep their relative order, so the
+    product is unchanged; others commute.  Greedy list scheduling: run every
+    ready op whose targets are register bits now; when none is, take the new
+    register set from the earliest ready op's targets plus those of the other
+    ready ops that fit beside them."""
+    k = len(inc)
+    bits = [kops[i].allbits for i in inc]
+    need = [frozenset(lbit[b] for b in reg_targets(kops[i])) for i in inc]
+    npred = [0] * k
+    succ: List[List[int]] = [[] for _ in range(k)]
+    for a in range(k):
+        for b in range(a):
+            if bits[a] & bits[b]:
+                npred[a] += 1
+                succ[b].append(a)
+    ready = sorted(a for a in range(k) if npred[a] == 0)
+    S = set(canon)
+    order: List[int] = []
+    while ready:
+        fit = [a for a in ready if need[a] <= S]
+        if not fit:
+            first = ready[0]
+            new = set(need[first])
+            for a in ready[1:]:
+                if len(new | need[a]) <= R_BITS:
+                    new |= need[a]
+            S = new
+            continue
+        a = fit[0]
+        ready.remove(a)
+        order.append(a)
+        for b in succ[a]:
+            npred[b] -= 1
+            if npred[b] == 0:
+                ready.append(b)
+        ready.sort()
+    assert len(order) == k
+    return [inc[a] for a in order]
+
+
 def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off: Dict[int, tuple],
                  slot_of_kop: Dict[int, List[int]], stage_base: int,
-                 mg_slot_of: Optional[Dict[int, List[int]]] = None) -> Stage:
-    """Program words of one stage (see tqsv_stage.hip for the format)."""
+                 mg_slot_of: Optional[Dict[int, List[int]]] = None, reorder: bool = False) -> Stage:
+    """Program words of one stage (see tqsv_stage.hip for the format);
+    reorder: schedule commuting ops for fewer remaps (order_for_registers)."""
     m = len(L)
     T = m - R_BITS
     lbit = {g: i for i, g in enumerate(L)}           # global position -> local bit
@@ -982,6 +1029,8 @@ def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off
     header[H_POS + m:H_POS + n] = nonlocal_pos
 
     # register-set schedule with look-ahead
+    if reorder:
+        inc = order_for_registers(inc, kops, lbit, canon)
     needs = [tuple(lbit[b] for b in reg_targets(kops[i])) for i in inc]
     S = list(canon)
     seq = []   # ("remap", old, new) | ("op", i, S)
@@ -1255,7 +1304,8 @@ class Compiled:
                     self.steps.append(("solo", payload))
                     continue
                 inc, L = payload
-                st = encode_stage(inc, L, kops, n, self.data_off, slot_of, base, mg_slot_of)
+                st = encode_stage(inc, L, kops, n, self.data_off, slot_of, base, mg_slot_of,
+                                  reorder=opts.reorder_ops)
                 for (ki, perm) in st.enc:
                     if ki >= 0 and perm is not None:
                         self.perms[ki] = perm
