@@ -58,7 +58,7 @@ def time_forward(n, bsz, ops, mode, steps, warmup, tq, jit="auto"):
 
     def step():
         q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
-        q.engine_options = tq.EngineOptions(mode=mode, jit=jit)
+        q.engine_options = tq.EngineOptions(mode=mode, jit=jit, **_plan_opts(mode))
         for op, p in zip(ops, params):
             tqf.func_name_dict[op["name"]](q, op["wires"], params=p)
         return tq.MeasureAll(tq.PauliZ)(q)
@@ -80,6 +80,17 @@ def time_forward(n, bsz, ops, mode, steps, warmup, tq, jit="auto"):
 
 STAGE_BITS = 0      # local bits per stage of the gradient workload's plans (0 = the engine's choice)
 LOW_BITS = None     # lowest bits kept local in every stage (None = the engine's choice)
+IO_LAYOUTS = True   # stages load / store in the layouts of their first / last ops where lines stay whole
+
+
+def _plan_opts(mode="fused"):
+    """EngineOptions fields set from the command line (fused plans only)."""
+    if mode != "fused":
+        return {}
+    d = {} if LOW_BITS is None else {"low_bits": LOW_BITS}
+    if not IO_LAYOUTS:
+        d["io_layouts"] = False
+    return d
 
 
 def time_replay(n, bsz, ops, steps, warmup, tq):
@@ -123,8 +134,7 @@ def _ansatz_loss_step(n, bsz, layers, mode, tq, jit, params, phi, ops, obs, coef
     for p in params:
         p.grad = None
     q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
-    q.engine_options = tq.EngineOptions(mode=mode, jit=jit, stage_bits=STAGE_BITS,
-                                        **({"low_bits": LOW_BITS} if LOW_BITS is not None else {}))
+    q.engine_options = tq.EngineOptions(mode=mode, jit=jit, stage_bits=STAGE_BITS, **_plan_opts(mode))
     for w in range(n):
         tqf.ry(q, w, params=phi)
     k = 0
@@ -496,8 +506,7 @@ def config5(args, tq):
 
     def build(mode):
         q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
-        q.engine_options = (tq.EngineOptions(mode=mode, fusion_width=args.c5_fusion_width,
-                                             **({"low_bits": LOW_BITS} if LOW_BITS is not None else {}))
+        q.engine_options = (tq.EngineOptions(mode=mode, fusion_width=args.c5_fusion_width, **_plan_opts(mode))
                             if mode == "fused" else tq.EngineOptions(mode=mode))
         for op, p in zip(ops, params):
             tqf.func_name_dict[op["name"]](q, op["wires"], params=p)
@@ -811,6 +820,8 @@ def main():
                     help="run only the per-gate forward of the 20-qubit RY/CZ circuit (--steps, --warmup)")
     ap.add_argument("--low-bits", type=int, default=None,
                     help="lowest state bits kept local in every stage of the gradient workload (default: the engine's)")
+    ap.add_argument("--canonical-io", action="store_true",
+                    help="fused plans load and store every stage in the canonical layout (remaps at both ends)")
     ap.add_argument("--stage-bits", type=int, default=0,
                     help="local bits per stage for the gradient workload (default: the engine's choice)")
     ap.add_argument("--c5-fusion-width", type=int, default=1,
@@ -837,9 +848,10 @@ def main():
         sys.exit(2)
     import torchquantum_amd as tq
     from circuits import config2_ops
-    global STAGE_BITS, LOW_BITS
+    global STAGE_BITS, LOW_BITS, IO_LAYOUTS
     STAGE_BITS = args.stage_bits
     LOW_BITS = args.low_bits
+    IO_LAYOUTS = not args.canonical_io
 
     if args.only_roofline:
         cb, cms = copy_calibration()

@@ -28,28 +28,46 @@ def _op_mat(mats, w, row):
     return mats[off:]
 
 
+def layout_offsets(words, which):
+    """(thread offsets [2^T], slot offsets [NS]) of a stage's load ("ld") or
+    store ("st") layout: the register local bits are the H_FLAGS mask
+    (0 = canonical: the top R local bits), slot bit k is the k-th of them
+    ascending, thread-index bit i the i-th other local bit."""
+    h = words
+    m, T = int(h[E.H_M]), int(h[E.H_T])
+    L = [int(v) for v in h[E.H_POS:E.H_POS + m]]     # in-row position of each local bit
+    shift = E.FL_LOAD_SHIFT if which == "ld" else E.FL_STORE_SHIFT
+    mask = (int(h[E.H_FLAGS]) >> shift) & 0x1FFF
+    regs = list(range(T, m)) if mask == 0 else [b for b in range(m) if (mask >> b) & 1]
+    thr = [b for b in range(m) if b not in regs]
+    tid = np.arange(1 << T)
+    to = np.zeros(1 << T, dtype=np.int64)
+    for i, b in enumerate(thr):
+        to |= ((tid >> i) & 1) << L[b]
+    ro = np.array([sum(((j >> k) & 1) << L[regs[k]] for k in range(R)) for j in range(NS)], dtype=np.int64)
+    return to, ro
+
+
 def stage_apply(words, st, mats, n, init_zero=False, adjoint=False, lam=None, grad=None, skip=(), no_store=False):
     """One stage over every group.  Adjoint: skip = program positions whose
     un-apply is left out (their gradient partials still count); no_store =
     psi and lam are not written back (the pruned kernels of stagejit)."""
     h = words
     n_ops, m, T, nnl = int(h[E.H_NOPS]), int(h[E.H_M]), int(h[E.H_T]), int(h[E.H_NNL])
-    tpos = [int(v) for v in h[E.H_POS:E.H_POS + T]]
-    rpos = [int(v) for v in h[E.H_POS + T:E.H_POS + m]]
     nlpos = [int(v) for v in h[E.H_POS + m:E.H_POS + n]]
     ops = h[int(h[E.H_OFF_OPS]):]
     gslot = h[int(h[E.H_OFF_GSLOT]):int(h[E.H_OFF_GSLOT]) + int(h[E.H_NGP])]
-    nthr = 1 << T
-    tid = np.arange(nthr)
-    toff = np.zeros(nthr, dtype=np.int64)
-    for i, p in enumerate(tpos):
-        toff |= ((tid >> i) & 1) << p
-    roff = np.array([sum(((j >> k) & 1) << rpos[k] for k in range(R)) for j in range(NS)], dtype=np.int64)
+    tid = np.arange(1 << T)
+    # the forward reads in the load layout and writes in the store layout;
+    # the adjoint walks the program backwards, so the other way round
+    toff_in, roff_in = layout_offsets(h, "st" if adjoint else "ld")
+    toff_out, roff_out = layout_offsets(h, "ld" if adjoint else "st")
     bsz = st.shape[0]
     for row in range(bsz):
         for rest in range(1 << nnl):
             goff = sum(((rest >> i) & 1) << p for i, p in enumerate(nlpos))
-            idx = goff + toff[:, None] + roff[None, :]
+            idx = goff + toff_in[:, None] + roff_in[None, :]
+            odx = goff + toff_out[:, None] + roff_out[None, :]
             if init_zero:
                 a = (idx == 0).astype(st.dtype)
             else:
@@ -91,10 +109,10 @@ def stage_apply(words, st, mats, n, init_zero=False, adjoint=False, lam=None, gr
                     for q in range(P):
                         gacc[w[7] + q] += g[q]
             if not no_store:
-                st[row, idx] = a
+                st[row, odx] = a
             if adjoint:
                 if not no_store:
-                    lam[row, idx] = l
+                    lam[row, odx] = l
                 for t, s in enumerate(gslot):
                     grad[row, s] += gacc[t]
 

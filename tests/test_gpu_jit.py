@@ -129,6 +129,53 @@ def test_jit_config3_family_gradient():
 
 
 @gpu
+def test_io_layouts_state_and_gradients():
+    """Stages that load in the register layout of their first ops and store
+    in that of their last (the default; 16-qubit layered ansatz, 12-bit
+    stages, where both kinds occur): compiled and interpreting kernels give
+    the oracle's state (1e-5) and the gradients of canonical loads and
+    stores (EngineOptions(io_layouts=False)) within 1e-5."""
+    import torchquantum_amd as tq
+    import torchquantum_amd.functional as tqf
+    from torchquantum_amd import engine as E
+    E._plan_cache.clear()
+    n, bsz, layers = 16, 2, 3
+    ops = config3_ops(seed=3, n=n, layers=layers)
+    obs, coef = config3_observables(n)
+    phi = np.linspace(0.2, 1.1, bsz).reshape(bsz, 1)
+    ref = O.run_ops(oracle_ops([dict(name="ry", wires=[w], params=phi, inverse=False) for w in range(n)] + ops,
+                               bsz), n, bsz)
+    out = {}
+    for label, kw in (("compiled", dict(jit="on")), ("interpreting", dict(jit="off")),
+                      ("canonical", dict(jit="off", io_layouts=False))):
+        params = [torch.tensor(op["params"], dtype=torch.float32, device=DEV, requires_grad=True) for op in ops
+                  if op["params"] is not None]
+        q = tq.QuantumDevice(n_wires=n, bsz=bsz, device=DEV)
+        q.engine_options = tq.EngineOptions(stage_bits=12, **kw)
+        for w in range(n):
+            tqf.ry(q, w, params=torch.tensor(phi, dtype=torch.float32, device=DEV))
+        k = 0
+        for op in ops:
+            if op["params"] is None:
+                tqf.func_name_dict[op["name"]](q, op["wires"])
+            else:
+                tqf.func_name_dict[op["name"]](q, op["wires"], params=params[k])
+                k += 1
+        state = q.get_states_1d().detach().cpu().numpy()
+        loss = tq.expval_pauli_sum(q, obs, coef).mean()
+        loss.backward()
+        out[label] = (state, torch.cat([p.grad.reshape(-1) for p in params]).cpu().numpy())
+    plans = [c for c in E._plan_cache.values() if c.opts.io_layouts]
+    flags = [int(st.words[E.H_FLAGS]) for c in plans for kind, (b, st) in c.steps if kind == "stage"]
+    assert any((f >> E.FL_LOAD_SHIFT) & 0x1FFF for f in flags) and any(f >> E.FL_STORE_SHIFT for f in flags)
+    assert _plans_with_jit("f") and _plans_with_jit("a"), "compiled stage kernels did not run"
+    for label in out:
+        assert np.abs(out[label][0] - ref).max() < 1e-5, label
+    for label in ("compiled", "interpreting"):
+        assert np.abs(out[label][1] - out["canonical"][1]).max() < 1e-5, label
+
+
+@gpu
 def test_jit_auto_compiles_in_background():
     """jit="auto" on a 2^22-amplitude state: the first pass runs the
     interpreting kernels while the stage kernels compile on a background

@@ -596,3 +596,46 @@ def test_reordered_stages_same_state_fewer_remaps(seed):
                            if (st.words[int(st.words[E.H_OFF_OPS]) + 8 * q_] & 15) == E.OP_REMAP))
                    for k, (b, st) in [(k, p) for k, p in c.steps if k == "stage"])
     assert remaps(True) < remaps(False)
+
+
+def test_io_layouts_same_state_fewer_remaps():
+    """Loading in the register layout of a stage's first ops and storing in
+    that of its last (EngineOptions.io_layouts, the H_FLAGS masks) leaves the
+    emulated output state and adjoint gradients unchanged and drops LDS
+    remaps on the layered ansatz; every mask names 4 register bits and keeps
+    local bits 0-3 (whole 128-byte lines per 16 lanes) on the thread index."""
+    masks = {"ld": [], "st": []}
+    for n, stage_bits in ((14, 11), (16, 12)):
+        _io_layouts_case(n, stage_bits, masks)
+    for key in ("ld", "st"):
+        assert any(masks[key]), f"no stage {key} layout other than the canonical one"
+        for mk in masks[key]:
+            assert mk == 0 or (bin(mk).count("1") == E.R_BITS and mk & 0xF == 0)
+
+
+def _io_layouts_case(n, stage_bits, masks):
+    ops = config3_ops(seed=3, n=n, layers=3)
+    bsz = 1
+    tape = _tape_from_ops(ops, grad=True)
+    res, n_remap = {}, {}
+    for io in (False, True):
+        c = E.Compiled(tape, n, bsz, E.EngineOptions(mode="fused", stage_bits=stage_bits, io_layouts=io), "cpu")
+        stages = [p[1] for k, p in c.steps if k == "stage"]
+        n_remap[io] = sum(int((st.words[int(st.words[E.H_OFF_OPS]) + 8 * np.arange(int(st.words[E.H_NOPS]))]
+                               & 15 == E.OP_REMAP).sum()) for st in stages)
+        for st in stages:
+            f = int(st.words[E.H_FLAGS])
+            for key, sh in (("ld", E.FL_LOAD_SHIFT), ("st", E.FL_STORE_SHIFT)):
+                mk = (f >> sh) & 0x1FFF
+                if io:
+                    masks[key].append(mk)
+                else:
+                    assert mk == 0
+        pool = c.pool_host(tape, True, True).astype(np.complex128)
+        psi = EM.run_compiled(c, pool, None, init_zero=True)
+        cvec = np.random.default_rng(5).normal(size=psi.shape) + 0j
+        res[io] = (psi.copy(), EM.adjoint_compiled(c, pool, psi, cvec))
+    assert n_remap[True] < n_remap[False]
+    assert np.abs(res[True][0] - res[False][0]).max() < 1e-12
+    assert np.abs(res[True][1] - res[False][1]).max() < 1e-9 * (1 + np.abs(res[False][1]).max())
+    assert np.abs(res[True][0] - O.run_ops(oracle_ops(ops, bsz), n, bsz)).max() < 1e-5

@@ -2,6 +2,7 @@
 (hipRTC cross-compiles for gfx950 without a GPU): the generated HIP compiles,
 carries no packed-FP32 instructions, and reflects the stage programs it was
 generated from.  Their numerics are checked on the GPU (test_gpu_jit.py)."""
+import re
 import shutil
 import subprocess
 from pathlib import Path
@@ -11,6 +12,8 @@ import pytest
 import torch
 
 from circuits import apply_ops_tq, random_ops
+
+NS = 16          # amplitudes per lane in a stage kernel
 
 
 def _plan(n, bsz, ops, **kw):
@@ -159,3 +162,71 @@ def test_pruned_adjoint_code():
     assert J.kernel_name("p", (st.words, skip[si], si == stop)) != J.kernel_name("a", st.words)
     code = J.compile_source(J.PRELUDE + pruned, J.COMPILE_OPTS["p"])
     assert len(code) > 1000
+
+
+def _lane_positions(src, var):
+    """Bit positions of `const unsigned <var> = ((((unsigned)tid >> i) & 1u) << p) | ...;` in thread-bit order."""
+    m = re.search(rf"const unsigned {var} = ([^;]*);", src)
+    assert m, var
+    return [int(p) for _, p in sorted((int(i), p) for i, p in re.findall(r"tid >> (\d+)\) & 1u\) << (\d+)\)", m.group(1)))]
+
+
+def _lane_offsets(positions, nthr):
+    tid = np.arange(nthr)
+    out = np.zeros(nthr, dtype=np.int64)
+    for i, p in enumerate(positions):
+        out |= ((tid >> i) & 1) << p
+    return out
+
+
+def test_load_store_layouts_follow_program():
+    """Stages that load in the layout of their first ops or store in that of
+    their last (H_FLAGS masks): the generated forward code reads in the load
+    layout and writes in the store layout, the adjoint code the other way
+    round, each lane and slot at the offsets the program emulator uses
+    (tests/emulator.py layout_offsets); the kernels compile for gfx950."""
+    import torchquantum_amd as tq
+    import torchquantum_amd.functional as tqf
+    import emulator as EM
+    from circuits import config3_ops
+    from torchquantum_amd import engine as E, stagejit as J
+    n = 16
+    q = tq.QuantumDevice(n_wires=n, bsz=1, device="cpu")
+    for op in config3_ops(seed=3, n=n, layers=3):
+        p = op["params"]
+        tqf.func_name_dict[op["name"]](q, op["wires"], params=None if p is None else
+                                       torch.tensor(np.asarray(p, dtype=np.float32).reshape(1, -1),
+                                                    requires_grad=True))
+    c = E.Compiled(q._tape, n, 1, tq.EngineOptions(stage_bits=12), "cpu")
+    picked = [s for s in _stages(c) if int(s.words[E.H_FLAGS]) >> E.FL_LOAD_SHIFT]
+    seen = set()
+    srcs = []
+    for s in picked:
+        sp = J.StageProgram(s.words)
+        seen |= {k for k, mk in (("ld", sp.lmask), ("st", sp.smask)) if mk}
+        nthr = 1 << sp.T
+        want = {k: EM.layout_offsets(s.words, k) for k in ("ld", "st")}
+        var = {"toff": "ld", "toffs": "st"}
+        fwd = J.gen_forward(s.words, f"kf{len(srcs)}")
+        adj = J.gen_adjoint(s.words, f"ka{len(srcs)}")
+        srcs += [fwd, adj]
+        lanes = {v: _lane_offsets(_lane_positions(src, v), nthr) for src in (fwd,)
+                 for v in ("toff", "toffs") if f"const unsigned {v} =" in src}
+        assert ("toffs" in lanes) == (sp.lmask != sp.smask)
+
+        def check(src, pattern, lay):
+            hits = re.findall(pattern, src)
+            assert len(hits) >= NS and len(hits) % NS == 0, pattern
+            for k in range(0, len(hits), NS):
+                for j, (v, const) in enumerate(hits[k:k + NS]):
+                    assert var[v] == lay or (v == "toff" and sp.lmask == sp.smask)
+                    assert np.array_equal(lanes[v], want[lay][0])
+                    assert int(const) == 8 * int(want[lay][1][j])
+        check(fwd, r"const float2 q\d+ = tq_ld\(r_inn, (toffs?)8, (\d+)\)", "ld")
+        check(fwd, r"tq_st\(r_out, (toffs?)8, (\d+), ", "st")
+        check(adj, r"const float2 p\d+ = tq_ld\(r_psi_in, (toffs?)8, (\d+)\)", "st")
+        check(adj, r"tq_st\(r_lam, (toffs?)8, (\d+), ", "ld")
+    assert seen == {"ld", "st"}
+    code = J.compile_source(J.PRELUDE + "".join(srcs[:4]), J.COMPILE_OPTS["a"])
+    assert len(code) > 1000
+

@@ -16,8 +16,15 @@
 //     on any local bit;
 //   * a REMAP op re-assigns which local bits are register bits by one
 //     LDS write + read of the group (XOR-swizzled to spread banks);
-//   * the canonical layout used for the global load/store puts the lowest
-//     local bits on the thread index, so lanes read consecutive addresses.
+//   * the canonical layout (register bits = the top R local bits) puts the
+//     lowest local bits on the thread index, so lanes read consecutive
+//     addresses.  A stage whose first ops need other register bits, none of
+//     them among the lowest 4 local bits, loads in that layout instead of
+//     loading canonically and remapping; likewise it stores in the layout of
+//     its last ops.  Header word H_FLAGS: bits 0-1 the op features, bits
+//     2-14 the register local bits (a mask) of the load layout, bits 15-27
+//     those of the store layout, 0 meaning canonical.  Lanes 16 apart still
+//     cover whole 128-byte lines in either.
 //
 // The adjoint kernel walks the same program backwards on two states at once
 // (psi: the forward output being un-computed, lam: d loss / d conj(psi) in
@@ -573,6 +580,38 @@ struct StageArgs {
   int64_t rpw;            // adjoint: rows per workgroup slot count of `part`
 };
 
+constexpr int FL_LOAD_SHIFT = 2, FL_STORE_SHIFT = 15, FL_MASK = 0x1FFF;
+
+// In-group offsets of a layout: L[b] is the in-row position of local bit b
+// (the header's positions from H_POS: thread bits, then register bits, of
+// the canonical layout); `mask` the layout's register local bits (0: the
+// canonical ones, local bits T..m-1).  Slot bit k is the k-th register bit
+// in ascending order, thread-index bit i the i-th other local bit.
+__device__ __forceinline__ void layout_offsets(const int* __restrict__ L, int m, int T, uint32_t mask, int tid,
+                                               uint32_t& toff, uint32_t (&roff)[NS]) {
+  if (mask == 0) mask = ((1u << m) - 1u) & ~((1u << T) - 1u);
+  uint32_t rp[R];
+  uint32_t mm = mask;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    rp[k] = (uint32_t)L[__builtin_ctz(mm)];
+    mm &= mm - 1u;
+  }
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) o |= (uint32_t)((j >> k) & 1) << rp[k];
+    roff[j] = o;
+  }
+  toff = 0;
+  uint32_t tm = ((1u << m) - 1u) & ~mask;
+  for (int i = 0; tm != 0u; ++i) {
+    toff |= (uint32_t)((tid >> i) & 1) << L[__builtin_ctz(tm)];
+    tm &= tm - 1u;
+  }
+}
+
 __device__ __forceinline__ uint32_t group_offset(const int* __restrict__ nlpos, int nnl, uint64_t rest) {
   uint32_t o = 0;
   for (int i = 0; i < nnl; ++i) o |= (uint32_t)((rest >> i) & 1ull) << nlpos[i];
@@ -594,16 +633,10 @@ __global__ __launch_bounds__(512) void k_stage_fwd(float2* __restrict__ out, con
   const int* __restrict__ nlpos = rpos + R;
   const int* __restrict__ ops = h + h[H_OFF_OPS];
   const int tid = threadIdx.x;
-  uint32_t toff = 0;
-  for (int i = 0; i < T; ++i) toff |= (uint32_t)((tid >> i) & 1) << tpos[i];
-  uint32_t roff[NS];
-#pragma unroll
-  for (int j = 0; j < NS; ++j) {
-    uint32_t o = 0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) o |= (uint32_t)((j >> k) & 1) << rpos[k];
-    roff[j] = o;
-  }
+  const int m = h[H_M];
+  uint32_t toff, roff[NS], soff, sroff[NS];     // load / store layout
+  layout_offsets(tpos, m, T, (h[H_FLAGS] >> FL_LOAD_SHIFT) & FL_MASK, tid, toff, roff);
+  layout_offsets(tpos, m, T, (h[H_FLAGS] >> FL_STORE_SHIFT) & FL_MASK, tid, soff, sroff);
   const int64_t g0 = (int64_t)blockIdx.x * A.groups_per_wg;
   int64_t g1 = g0 + A.groups_per_wg;
   if (g1 > A.n_groups) g1 = A.n_groups;
@@ -611,13 +644,13 @@ __global__ __launch_bounds__(512) void k_stage_fwd(float2* __restrict__ out, con
   for (int64_t g = g0; g < g1; ++g) {
     const uint64_t row = (uint64_t)g >> nnl;
     const uint32_t goff = group_offset(nlpos, nnl, (uint64_t)g & rest_mask);
-    const uint64_t base = (row << A.n) + goff + toff;
+    const uint64_t gbase = (row << A.n) + goff;
     float2 a[NS];
     if (A.init_zero) {
 #pragma unroll
       for (int j = 0; j < NS; ++j) a[j] = make_float2((goff + toff + roff[j]) == 0 ? 1.f : 0.f, 0.f);
     } else {
-      const float2* src = in + base;
+      const float2* src = in + gbase + toff;
 #pragma unroll
       for (int j = 0; j < NS; ++j) a[j] = src[roff[j]];
     }
@@ -631,10 +664,10 @@ __global__ __launch_bounds__(512) void k_stage_fwd(float2* __restrict__ out, con
       const float2* M = mats + w[1] + ((w[0] & F_BATCHED) ? (int64_t)row * w[2] : 0);
       fwd_op<FEAT>(a, w, M, tid);
     }
-    float2* dst = out + base;
+    float2* dst = out + gbase + soff;
     asm volatile("" : "+v"(dst));
 #pragma unroll
-    for (int j = 0; j < NS; ++j) dst[roff[j]] = a[j];
+    for (int j = 0; j < NS; ++j) dst[sroff[j]] = a[j];
   }
 }
 
@@ -663,16 +696,10 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
   const int wwidth = blockDim.x < 64 ? (int)blockDim.x : 64;
   float* __restrict__ wacc = gacc + (tid >> 6) * n_gp;   // this wave's row
   for (int t = tid; t < nwave * n_gp; t += blockDim.x) gacc[t] = 0.f;
-  uint32_t toff = 0;
-  for (int i = 0; i < T; ++i) toff |= (uint32_t)((tid >> i) & 1) << tpos[i];
-  uint32_t roff[NS];
-#pragma unroll
-  for (int j = 0; j < NS; ++j) {
-    uint32_t o = 0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) o |= (uint32_t)((j >> k) & 1) << rpos[k];
-    roff[j] = o;
-  }
+  // the program runs backwards: read in its store layout, write in its load layout
+  uint32_t toff, roff[NS], soff, sroff[NS];
+  layout_offsets(tpos, h[H_M], T, (h[H_FLAGS] >> FL_STORE_SHIFT) & FL_MASK, tid, toff, roff);
+  layout_offsets(tpos, h[H_M], T, (h[H_FLAGS] >> FL_LOAD_SHIFT) & FL_MASK, tid, soff, sroff);
   const int64_t g0 = (int64_t)blockIdx.x * A.groups_per_wg;
   int64_t g1 = g0 + A.groups_per_wg;
   if (g1 > A.n_groups) g1 = A.n_groups;
@@ -700,6 +727,7 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
     }
     const uint32_t goff = group_offset(nlpos, nnl, (uint64_t)g & rest_mask);
     const uint64_t base = (row << A.n) + goff + toff;
+    const uint64_t sbase = (row << A.n) + goff + soff;
     float2 p[NS], l[NS];
     {
       const float2* sp = psi + base;
@@ -753,14 +781,14 @@ __global__ __launch_bounds__(512) void k_stage_adj(float2* __restrict__ psi, flo
       }
     }
     {
-      float2* dp = psi + base;
+      float2* dp = psi + sbase;
       asm volatile("" : "+v"(dp));
 #pragma unroll
-      for (int j = 0; j < NS; ++j) dp[roff[j]] = p[j];
-      float2* dl = lam + base;
+      for (int j = 0; j < NS; ++j) dp[sroff[j]] = p[j];
+      float2* dl = lam + sbase;
       asm volatile("" : "+v"(dl));
 #pragma unroll
-      for (int j = 0; j < NS; ++j) dl[roff[j]] = l[j];
+      for (int j = 0; j < NS; ++j) dl[sroff[j]] = l[j];
     }
   }
   if (n_gp > 0 && cur_row >= 0) {
@@ -825,27 +853,19 @@ __global__ __launch_bounds__(BATCH_THREADS) void k_batch_fwd(float2* __restrict_
     const int* __restrict__ nlpos = rpos + R;
     const int* __restrict__ ops = h + h[H_OFF_OPS];
     const bool active = tid < (1 << T);
-    uint32_t toff = 0;
-    for (int i = 0; i < T; ++i) toff |= (uint32_t)((tid >> i) & 1) << tpos[i];
-    uint32_t roff[NS];
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      uint32_t o = 0;
-#pragma unroll
-      for (int k = 0; k < R; ++k) o |= (uint32_t)((j >> k) & 1) << rpos[k];
-      roff[j] = o;
-    }
+    uint32_t toff, roff[NS], soff, sroff[NS];   // load / store layout
+    layout_offsets(tpos, h[H_M], T, (h[H_FLAGS] >> FL_LOAD_SHIFT) & FL_MASK, tid, toff, roff);
+    layout_offsets(tpos, h[H_M], T, (h[H_FLAGS] >> FL_STORE_SHIFT) & FL_MASK, tid, soff, sroff);
     const uint64_t rest_mask = (1ull << nnl) - 1ull;
     float2* st = states + item.state_off;
     for (int64_t g = item.g0; g < item.g1; ++g) {
       const uint32_t goff = group_offset(nlpos, nnl, (uint64_t)g & rest_mask);
-      const uint64_t base = (uint64_t)goff + toff;
       float2 a[NS];
       if (item.init_zero) {
 #pragma unroll
         for (int j = 0; j < NS; ++j) a[j] = make_float2((goff + toff + roff[j]) == 0 ? 1.f : 0.f, 0.f);
       } else if (active) {
-        const float2* src = st + base;
+        const float2* src = st + (uint64_t)goff + toff;
 #pragma unroll
         for (int j = 0; j < NS; ++j) a[j] = src[roff[j]];
       }
@@ -859,10 +879,10 @@ __global__ __launch_bounds__(BATCH_THREADS) void k_batch_fwd(float2* __restrict_
         fwd_op<FEAT_MAT2 | FEAT_MATK>(a, w, mats + w[1], tid);
       }
       if (active) {
-        float2* dst = st + base;
+        float2* dst = st + (uint64_t)goff + soff;
         asm volatile("" : "+v"(dst));
 #pragma unroll
-        for (int j = 0; j < NS; ++j) dst[roff[j]] = a[j];
+        for (int j = 0; j < NS; ++j) dst[sroff[j]] = a[j];
       }
     }
     __syncthreads();

@@ -57,6 +57,9 @@ DEV_OP = np.dtype([("dst", "<i8"), ("bstride", "<i8"), ("adj", "<i8"), ("abstrid
 DEV_MEMBER = np.dtype([("params", "<u8"), ("rstride", "<i8"), ("cstride", "<i8"), ("code", "<i4"), ("flags", "<i4"),
                        ("fixed", "<f4", (8,))])
 FEAT_MAT2, FEAT_MATK = 1, 2
+# H_FLAGS: bits 0-1 features; bits 2-14 / 15-27: register local bits (mask)
+# of the layout the stage loads in / stores in (0 = the canonical layout)
+FL_LOAD_SHIFT, FL_STORE_SHIFT = 2, 15
 
 
 # ----------------------------------------------------------------------------
@@ -101,6 +104,10 @@ class EngineOptions:
     reorder_ops: within a stage, run commuting ops in an order that needs
                  fewer register-set changes (LDS remaps); ops sharing a wire
                  keep their order.
+    io_layouts:  a stage may load in the register layout of its first ops
+                 and store in that of its last (when the lowest 4 local bits
+                 stay thread bits, so accesses still cover whole 128-byte
+                 lines) instead of remapping from and to the canonical one.
     """
     mode: str = "fused"
     stage_bits: int = 0
@@ -112,10 +119,11 @@ class EngineOptions:
     jit: str = "auto"
     device_gates: bool = True
     reorder_ops: bool = True
+    io_layouts: bool = True
 
     def key(self):
         return (self.mode, self.stage_bits, self.low_bits, self.merge_1q, self.fusion_width, self.grid_cap,
-                self.adjoint_mg, self.jit, self.device_gates, self.reorder_ops)
+                self.adjoint_mg, self.jit, self.device_gates, self.reorder_ops, self.io_layouts)
 
 
 DEFAULT_OPTIONS = EngineOptions()
@@ -1013,9 +1021,12 @@ def order_for_registers(inc: List[int], kops: List[KOp], lbit: Dict[int, int], c
 
 def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off: Dict[int, tuple],
                  slot_of_kop: Dict[int, List[int]], stage_base: int,
-                 mg_slot_of: Optional[Dict[int, List[int]]] = None, reorder: bool = False) -> Stage:
+                 mg_slot_of: Optional[Dict[int, List[int]]] = None, reorder: bool = False,
+                 io_layouts: bool = False) -> Stage:
     """Program words of one stage (see tqsv_stage.hip for the format);
-    reorder: schedule commuting ops for fewer remaps (order_for_registers)."""
+    reorder: schedule commuting ops for fewer remaps (order_for_registers);
+    io_layouts: load / store in the layouts of the first / last ops where
+    they keep accesses on whole lines (H_FLAGS masks)."""
     m = len(L)
     T = m - R_BITS
     lbit = {g: i for i, g in enumerate(L)}           # global position -> local bit
@@ -1044,7 +1055,10 @@ def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off
                         new.append(b)
                 if len(new) >= R_BITS:
                     break
-            for b in S + list(range(m)):
+            # fill from the current set, then from the high local bits: the
+            # lowest local bits stay thread bits where possible, so the
+            # stage can load and store in these layouts (below)
+            for b in S + list(range(m - 1, -1, -1)):
                 if len(new) >= R_BITS:
                     break
                 if b not in new:
@@ -1053,7 +1067,19 @@ def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off
             seq.append(("remap", list(S), new))
             S = new
         seq.append(("op", i, list(S)))
-    if S != canon:
+    # A layout whose register bits leave the lowest 4 local bits as thread
+    # bits 0..3 reads and writes whole 128-byte lines like the canonical one,
+    # so the stage loads in the layout of its first ops and stores in the
+    # layout of its last instead of remapping to and from the canonical one.
+    low = set(range(4))
+    load_regs = store_regs = canon
+    if io_layouts and T >= 4:
+        if seq and seq[0][0] == "remap" and not set(seq[0][2]) & low:
+            load_regs = seq[0][2]
+            seq = seq[1:]
+        if S != canon and not set(S) & low:
+            store_regs = S
+    if S != store_regs:
         seq.append(("remap", list(S), list(canon)))
 
     def layout_vals(regs):
@@ -1163,8 +1189,11 @@ def encode_stage(inc: List[int], L: List[int], kops: List[KOp], n: int, data_off
     for w in ops_words:
         if w[0] == OP_REMAP:
             w[7] += off_remap
+    def regmask(regs):
+        return 0 if list(regs) == canon else sum(1 << b for b in regs)
+
     header[H_NOPS] = n_ops
-    header[H_FLAGS] = feat
+    header[H_FLAGS] = feat | (regmask(load_regs) << FL_LOAD_SHIFT) | (regmask(store_regs) << FL_STORE_SHIFT)
     header[H_OFF_OPS] = off_ops
     header[H_NGP] = len(gslots)
     header[H_OFF_GSLOT] = off_gslot
@@ -1305,7 +1334,7 @@ class Compiled:
                     continue
                 inc, L = payload
                 st = encode_stage(inc, L, kops, n, self.data_off, slot_of, base, mg_slot_of,
-                                  reorder=opts.reorder_ops)
+                                  reorder=opts.reorder_ops, io_layouts=opts.io_layouts)
                 for (ki, perm) in st.enc:
                     if ki >= 0 and perm is not None:
                         self.perms[ki] = perm

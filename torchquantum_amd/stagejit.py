@@ -33,8 +33,9 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from .engine import (H_NOPS, H_M, H_T, H_NNL, H_OFF_OPS, H_NGP, H_OFF_GSLOT, H_POS, OP_MAT1, OP_MAT2, OP_DIAG,
-                     OP_MATK, OP_REMAP, F_BATCHED, F_NOCTRL, F_REAL, F_XPERM, F_SIGN, F_MG, R_BITS)
+from .engine import (H_NOPS, H_M, H_T, H_NNL, H_OFF_OPS, H_NGP, H_OFF_GSLOT, H_FLAGS, H_POS, OP_MAT1, OP_MAT2,
+                     OP_DIAG, OP_MATK, OP_REMAP, F_BATCHED, F_NOCTRL, F_REAL, F_XPERM, F_SIGN, F_MG, R_BITS,
+                     FL_LOAD_SHIFT, FL_STORE_SHIFT)
 
 NS = 1 << R_BITS
 MIN_T = 6            # compiled stages need full 64-lane waves
@@ -172,13 +173,32 @@ class StageProgram:
         self.ops = [h[off + 8 * i:off + 8 * i + 8] for i in range(self.n_ops)]
         self.ngp = h[H_NGP]
         self.gslots = h[h[H_OFF_GSLOT]:h[H_OFF_GSLOT] + self.ngp]
-        self.roff = []
-        for j in range(NS):
-            o = 0
-            for k in range(R_BITS):
-                if (j >> k) & 1:
-                    o |= 1 << self.rpos[k]
-            self.roff.append(o)
+        self.roff = self._slot_offsets(self.rpos)
+        # load / store layouts (register local bits as a mask; 0 = canonical)
+        flags = h[H_FLAGS]
+        self.lmask = (flags >> FL_LOAD_SHIFT) & 0x1FFF
+        self.smask = (flags >> FL_STORE_SHIFT) & 0x1FFF
+        self.lay = {"ld": self._layout(self.lmask), "st": self._layout(self.smask)}
+
+    @staticmethod
+    def _slot_offsets(rp):
+        return [sum(1 << rp[k] for k in range(R_BITS) if (j >> k) & 1) for j in range(NS)]
+
+    def _layout(self, mask):
+        """(thread-bit positions, slot offsets) of the layout whose register
+        local bits are `mask`: slot bit k = the k-th register bit ascending,
+        thread-index bit i = the i-th other local bit."""
+        if mask == 0:
+            return list(self.tpos), list(self.roff)
+        L = list(self.tpos) + list(self.rpos)
+        regs = [b for b in range(self.m) if (mask >> b) & 1]
+        thr = [b for b in range(self.m) if not (mask >> b) & 1]
+        return [L[b] for b in thr], self._slot_offsets([L[b] for b in regs])
+
+    def lane_var(self, lay: str) -> str:
+        """Name of the lane-offset variable of layout lay ("ld"/"st"); the
+        store layout shares the load layout's when they are the same."""
+        return "toffs" if lay == "st" and self.smask != self.lmask else "toff"
 
     def remap_layouts(self, w):
         rec = self.words[w[7]:w[7] + 2 * (self.T + R_BITS)]
@@ -682,9 +702,9 @@ class _Gen:
 
 def _split_addr(sp: StageProgram) -> bool:
     """Amplitude addresses as (wave-uniform 64-bit base + constant) plus one
-    32-bit per-lane byte offset (toff * 8), so that every load and store of a
-    group shares one offset register; needs toff * 8 < 2^32."""
-    return max(sp.tpos, default=0) <= 28
+    32-bit per-lane byte offset (toff * 8, one per layout), so that every
+    load (store) of a group shares one offset register; needs toff * 8 < 2^32."""
+    return max(sp.lay["ld"][0] + sp.lay["st"][0], default=0) <= 28
 
 
 def _buf_ok(sp: StageProgram) -> bool:
@@ -715,36 +735,47 @@ def _group_vars(sp: StageProgram, gv: str, sfx: str, mode: str, arrays: Sequence
     ]
     if mode == "buf":
         out += [f"const __amdgpu_buffer_rsrc_t r_{a}{sfx} = tq_rsrc({a} + gbase{sfx});" for a in arrays]
-    elif not _split_addr(sp):
-        out.append(f"const unsigned long long base{sfx} = gbase{sfx} + toff;")
     return out
 
 
-def _ptr(sp: StageProgram, arr: str, j: int, const: bool, sfx: str = "") -> str:
-    """Address of amplitude slot j of this lane's group in array arr."""
+def _ptr(sp: StageProgram, arr: str, j: int, const: bool, sfx: str, lay: str) -> str:
+    """Address of amplitude slot j of this lane's group in array arr, in
+    layout lay ("ld": the stage's load layout, "st": its store layout)."""
     c = "const " if const else ""
+    tv, ro = sp.lane_var(lay), sp.lay[lay][1][j]
     if _split_addr(sp):
-        return f"({c}float2*)(({c}char*)({arr} + gbase{sfx}) + {8 * sp.roff[j]}ull + toff8)"
-    return f"({arr} + base{sfx} + {sp.roff[j]})"
+        return f"({c}float2*)(({c}char*)({arr} + gbase{sfx}) + {8 * ro}ull + {tv}8)"
+    return f"({arr} + gbase{sfx} + {tv} + {ro})"
 
 
-def _load(sp: StageProgram, arr: str, j: int, mode: str, sfx: str = "") -> str:
+def _load(sp: StageProgram, arr: str, j: int, mode: str, sfx: str = "", lay: str = "ld") -> str:
     if mode == "buf":
-        return f"tq_ld(r_{arr}{sfx}, toff8, {8 * sp.roff[j]})"
-    p = _ptr(sp, arr, j, True, sfx)
+        return f"tq_ld(r_{arr}{sfx}, {sp.lane_var(lay)}8, {8 * sp.lay[lay][1][j]})"
+    p = _ptr(sp, arr, j, True, sfx, lay)
     return f"ldnt({p})" if mode == "nt" else f"*{p}"
 
 
-def _store(sp: StageProgram, arr: str, j: int, val: str, mode: str, sfx: str = "") -> str:
+def _store(sp: StageProgram, arr: str, j: int, val: str, mode: str, sfx: str = "", lay: str = "st") -> str:
     if mode == "buf":
-        return f"tq_st(r_{arr}{sfx}, toff8, {8 * sp.roff[j]}, {val});"
-    p = _ptr(sp, arr, j, False, sfx)
+        return f"tq_st(r_{arr}{sfx}, {sp.lane_var(lay)}8, {8 * sp.lay[lay][1][j]}, {val});"
+    p = _ptr(sp, arr, j, False, sfx, lay)
     return f"stnt({p}, {val});" if mode == "nt" else f"*{p} = {val};"
 
 
-def _toff(sp: StageProgram) -> str:
-    terms = [f"((((unsigned)tid >> {i}) & 1u) << {sp.tpos[i]})" for i in range(sp.T)]
+def _toff(tpos: Sequence[int]) -> str:
+    terms = [f"((((unsigned)tid >> {i}) & 1u) << {p})" for i, p in enumerate(tpos)]
     return " | ".join(terms) if terms else "0u"
+
+
+def _lane_offsets(sp: StageProgram) -> List[str]:
+    """Per-lane element and byte offsets of the load layout (toff, toff8)
+    and, when it differs, of the store layout (toffs, toffs8)."""
+    out = [f"  const unsigned toff = {_toff(sp.lay['ld'][0])};",
+           "  const unsigned toff8 = toff << 3;"]
+    if sp.lane_var("st") != "toff":
+        out += [f"  const unsigned toffs = {_toff(sp.lay['st'][0])};",
+                "  const unsigned toffs8 = toffs << 3;"]
+    return out
 
 
 def _codegen(buf: Optional[bool], prefetch: Optional[bool], kind: str, lds: Optional[str] = None,
@@ -781,7 +812,7 @@ def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
         else:
             g.fwd_op(w, a)
     body = ["  " + s for s in g.lines]
-    stores = [f"      {_store(sp, 'out', j, a[j], mode)}" for j in range(NS)]
+    stores = [f"      {_store(sp, 'out', j, a[j], mode, lay='st')}" for j in range(NS)]
     nt = 1 << sp.T
     src = [f'extern "C" __global__ void __launch_bounds__({nt}) {name}(float2* __restrict__ out, '
            "const float2* __restrict__ in, const float2* __restrict__ mats, int init_zero, long long n_groups, "
@@ -789,8 +820,7 @@ def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
     if has_remap:
         src.append(f"  __shared__ float2 lds[{lds_words(sp.m, lds)}];")
     src.append("  const int tid = threadIdx.x;")
-    src.append(f"  const unsigned toff = {_toff(sp)};")
-    src.append("  const unsigned toff8 = toff << 3;")
+    src += _lane_offsets(sp)
     src += g.hoisted()
     src += ["  const long long g0 = (long long)blockIdx.x * gpw;",
             "  long long g1 = g0 + gpw;",
@@ -799,7 +829,7 @@ def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
     src += ["  if (init_zero) {",
             "    for (long long g = g0; g < g1; ++g) {"]
     src += ["      " + s for s in _group_vars(sp, "g", "", mode, ["out"])]
-    src += [f"      const float2 i{j} = make_float2((goff + toff + {sp.roff[j]}u) == 0u ? 1.f : 0.f, 0.f);"
+    src += [f"      const float2 i{j} = make_float2((goff + toff + {sp.lay['ld'][1][j]}u) == 0u ? 1.f : 0.f, 0.f);"
             for j in range(NS)]
     src += body + stores + ["    }", "    return;", "  }"]
     if prefetch:
@@ -905,8 +935,7 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
         src.append(f"  float {', '.join(nm + ' = 0.f' for nm in names)};")
     src.append("  const int tid = threadIdx.x;")
     src.append("  const int lane = tid & 63;")
-    src.append(f"  const unsigned toff = {_toff(sp)};")
-    src.append("  const unsigned toff8 = toff << 3;")
+    src += _lane_offsets(sp)
     src += g.hoisted()
     src += ["  const long long g0 = (long long)blockIdx.x * gpw;",
             "  long long g1 = g0 + gpw;",
@@ -920,8 +949,8 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
                 f"  float2 {', '.join(f'lc{j}' for j in range(NS))};",
                 "  if (g0 < g1) {"]
         src += ["    " + s for s in _group_vars(sp, "g0", "n", mode, ["psi_in", "lam_in"])]
-        src += [f"    pc{j} = {_load(sp, 'psi_in', j, mode, 'n')};" for j in range(NS)]
-        src += [f"    lc{j} = {_load(sp, 'lam_in', j, mode, 'n')};" for j in range(NS)]
+        src += [f"    pc{j} = {_load(sp, 'psi_in', j, mode, 'n', 'st')};" for j in range(NS)]
+        src += [f"    lc{j} = {_load(sp, 'lam_in', j, mode, 'n', 'st')};" for j in range(NS)]
         src += ["  }"]
     src += ["  for (long long g = g0; g < g1; ++g) {"]
     src += ["    " + s for s in _group_vars(sp, "g", "", mode, ["psi", "lam"] if prefetch else
@@ -943,17 +972,17 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
     if prefetch:
         src += ["    const long long gn = g + 1 < g1 ? g + 1 : g;"]
         src += ["    " + s for s in _group_vars(sp, "gn", "n", mode, ["psi_in", "lam_in"])]
-        src += [f"    const float2 pq{j} = {_load(sp, 'psi_in', j, mode, 'n')};" for j in range(NS)]
-        src += [f"    const float2 lq{j} = {_load(sp, 'lam_in', j, mode, 'n')};" for j in range(NS)]
+        src += [f"    const float2 pq{j} = {_load(sp, 'psi_in', j, mode, 'n', 'st')};" for j in range(NS)]
+        src += [f"    const float2 lq{j} = {_load(sp, 'lam_in', j, mode, 'n', 'st')};" for j in range(NS)]
         src += [f"    const float2 p{j} = pc{j};" for j in range(NS)]
         src += [f"    const float2 l{j} = lc{j};" for j in range(NS)]
     else:
-        src += [f"    const float2 p{j} = {_load(sp, 'psi_in', j, mode)};" for j in range(NS)]
-        src += [f"    const float2 l{j} = {_load(sp, 'lam_in', j, mode)};" for j in range(NS)]
+        src += [f"    const float2 p{j} = {_load(sp, 'psi_in', j, mode, '', 'st')};" for j in range(NS)]
+        src += [f"    const float2 l{j} = {_load(sp, 'lam_in', j, mode, '', 'st')};" for j in range(NS)]
     src += g.lines
     if not no_store:
-        src += [f"    {_store(sp, 'psi', j, p[j], mode)}" for j in range(NS)]
-        src += [f"    {_store(sp, 'lam', j, l[j], mode)}" for j in range(NS)]
+        src += [f"    {_store(sp, 'psi', j, p[j], mode, lay='ld')}" for j in range(NS)]
+        src += [f"    {_store(sp, 'lam', j, l[j], mode, lay='ld')}" for j in range(NS)]
     if prefetch:
         src += [f"    pc{j} = pq{j};" for j in range(NS)]
         src += [f"    lc{j} = lq{j};" for j in range(NS)]
