@@ -99,3 +99,41 @@ def test_replay_inputs_and_gradients():
         rep()
     with pytest.raises(ValueError):
         rep(torch.rand(bsz, n + 1, device=DEV))
+
+
+@gpu
+def test_replay_general_encoder_training():
+    """GeneralEncoder (its input columns are copies: advanced indexing) and a
+    trainable RXYZCXLayer0: the replay re-gathers the encoder's columns on
+    every call, so for new inputs its <Z_i> and gradients equal the
+    functional path's (|d| <= 1e-5 (1 + |g|)), and an Adam step on the
+    ansatz between calls is followed."""
+    import torchquantum_amd as tq
+    n, bsz = 10, 8
+    enc = tq.GeneralEncoder([{"input_idx": [(3 * i) % n], "func": "ry", "wires": [i]} for i in range(n)])
+    ansatz = tq.RXYZCXLayer0({"n_wires": n, "n_blocks": 2}).to(DEV)
+    opt = torch.optim.Adam(ansatz.parameters(), lr=0.05)
+
+    def record(q, x):
+        enc(q, x)
+        ansatz(q)
+
+    rep = tq.CircuitReplay(record, n, bsz, example_inputs=(torch.rand(bsz, n, device=DEV),))
+    assert rep._gathers, "encoder columns were not bound"
+    meas = tq.MeasureAll(tq.PauliZ)
+    w = torch.linspace(-1.0, 1.0, n, device=DEV)
+    for seed in (1, 2, 3):
+        x = torch.rand(bsz, n, device=DEV, generator=torch.Generator(device=DEV).manual_seed(seed)) * np.pi
+        opt.zero_grad(set_to_none=True)
+        out = meas(rep(x))
+        (out * w).sum().backward()
+        g_rep = torch.cat([p.grad.reshape(-1) for p in ansatz.parameters()]).double().cpu().numpy()
+        opt.zero_grad(set_to_none=True)
+        q = tq.QuantumDevice(n_wires=n, bsz=bsz, device=DEV)
+        record(q, x)
+        ref = meas(q)
+        (ref * w).sum().backward()
+        g_ref = torch.cat([p.grad.reshape(-1) for p in ansatz.parameters()]).double().cpu().numpy()
+        assert (out - ref).abs().max().item() <= 1e-5, seed
+        assert np.all(np.abs(g_rep - g_ref) <= 1e-5 * (1 + np.abs(g_ref))), float(np.abs(g_rep - g_ref).max())
+        opt.step()
