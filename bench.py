@@ -81,6 +81,7 @@ def time_forward(n, bsz, ops, mode, steps, warmup, tq, jit="auto"):
 STAGE_BITS = 0      # local bits per stage of the gradient workload's plans (0 = the engine's choice)
 LOW_BITS = None     # lowest bits kept local in every stage (None = the engine's choice)
 IO_LAYOUTS = True   # stages load / store in the layouts of their first / last ops where lines stay whole
+GRAD_CAP = 0        # ops with gradient parameters per stage (0 = no limit)
 
 
 def _plan_opts(mode="fused"):
@@ -90,6 +91,8 @@ def _plan_opts(mode="fused"):
     d = {} if LOW_BITS is None else {"low_bits": LOW_BITS}
     if not IO_LAYOUTS:
         d["io_layouts"] = False
+    if GRAD_CAP:
+        d["grad_cap"] = GRAD_CAP
     return d
 
 
@@ -388,22 +391,40 @@ def encoder_training(tq, n, bsz, blocks, steps, warmup):
         opt.step()
         return loss
 
-    for i in range(warmup):
-        step(i)
-    tq.wait_for_compiled_stages()
-    step(0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(steps):
-        loss = step(i)
-    t_host = time.perf_counter() - t0
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
+    def timed(step):
+        for i in range(warmup):
+            step(i)
+        tq.wait_for_compiled_stages()
+        step(0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            loss = step(i)
+        t_host = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        return {"ms_per_step": dt * 1e3, "steps_per_s": 1.0 / dt, "host_issue_ms_per_step": t_host / steps * 1e3,
+                "loss_last": float(loss.detach())}
+
+    res = timed(step)
+    # the same training loop on a circuit recorded once (tq.CircuitReplay):
+    # the encoder's input columns are gathered on the GPU per call, the
+    # ansatz parameters are read after every Adam step
+    rep = tq.CircuitReplay(lambda q, x: (enc(q, x), ansatz(q)), n_wires=n, bsz=bsz, example_inputs=(xs[0],))
+
+    def step_replay(i):
+        logits = head(tq.MeasureAll(tq.PauliZ)(rep(xs[i % 4])))
+        loss = torch.nn.functional.cross_entropy(logits, ys[i % 4])
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        return loss
+
+    res["replay"] = timed(step_replay)
     n_par = sum(p.numel() for p in ansatz.parameters())
     return {"workload": f"{n} qubits, bsz {bsz}: RY encoder from a [bsz, {n}] input + RXYZCXLayer0 x {blocks} "
                         f"({n_par} params) + <Z_i> + linear head, cross entropy, Adam step; new inputs each step",
-            "ms_per_step": dt * 1e3, "steps_per_s": 1.0 / dt, "host_issue_ms_per_step": t_host / steps * 1e3,
-            "loss_last": float(loss.detach())}
+            **res}
 
 
 def _replay_ms(cb, steps=5):
@@ -820,6 +841,8 @@ def main():
                     help="run only the per-gate forward of the 20-qubit RY/CZ circuit (--steps, --warmup)")
     ap.add_argument("--low-bits", type=int, default=None,
                     help="lowest state bits kept local in every stage of the gradient workload (default: the engine's)")
+    ap.add_argument("--grad-cap", type=int, default=0,
+                    help="at most this many ops with gradient parameters per stage (0 = no limit)")
     ap.add_argument("--canonical-io", action="store_true",
                     help="fused plans load and store every stage in the canonical layout (remaps at both ends)")
     ap.add_argument("--stage-bits", type=int, default=0,
@@ -848,7 +871,8 @@ def main():
         sys.exit(2)
     import torchquantum_amd as tq
     from circuits import config2_ops
-    global STAGE_BITS, LOW_BITS, IO_LAYOUTS
+    global STAGE_BITS, LOW_BITS, IO_LAYOUTS, GRAD_CAP
+    GRAD_CAP = args.grad_cap
     STAGE_BITS = args.stage_bits
     LOW_BITS = args.low_bits
     IO_LAYOUTS = not args.canonical_io
