@@ -188,6 +188,17 @@ int tqsv_expval_pauli(tqsv_stream_t s, const void* state, int64_t bsz, int n, in
  * measurements.py:271-279 and :299-312). */
 int tqsv_pauli_apply(tqsv_stream_t s, void* out, const void* in, int64_t bsz, int n, int nterms,
                      const uint64_t* xmask, const uint64_t* zmask, const void* coef, int accumulate);
+/* A Pauli sum with its state gradient in the same passes: out = sum_t
+ * coef[t] X^x_t Z^z_t psi (coef: device complex64 [nterms], shared by all
+ * rows) and dot[row] = Re <psi_row|out_row> (device float [bsz]).  Replaces
+ * the forward and the autograd backward of a weighted sum of
+ * expval_joint_analytical values (measurements.py:232-279, their backward
+ * :271-279 and :299-312) when the loss is differentiated.  workspace:
+ * tqsv_pauli_sum_grad_workspace_floats(bsz, n) floats. */
+int tqsv_pauli_sum_grad(tqsv_stream_t s, void* out, const void* in, int64_t bsz, int n, int nterms,
+                        const uint64_t* xmask, const uint64_t* zmask, const void* coef, float* dot,
+                        float* workspace, int64_t workspace_floats);
+int64_t tqsv_pauli_sum_grad_workspace_floats(int64_t bsz, int n);
 
 /* ---- shot sampling: replaces measure()'s host-side random.choices over
  *      |psi|^2 (torchquantum/measurement/measurements.py:37-74), also behind

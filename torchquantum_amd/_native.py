@@ -52,6 +52,9 @@ SIGNATURES = {
                                   c_void_p, c_int64]),
     "tqsv_pauli_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int, c_u64_p, c_u64_p, c_void_p,
                                  c_int]),
+    "tqsv_pauli_sum_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int, c_u64_p, c_u64_p,
+                                    c_void_p, c_float_p, c_float_p, c_int64]),
+    "tqsv_pauli_sum_grad_workspace_floats": (c_int64, [c_int64, c_int]),
     "tqsv_batch_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int]),
     "tqsv_batch_item_bytes": (c_int, []),
     "tqsv_batch_zcirc_bytes": (c_int, []),

@@ -154,6 +154,15 @@ def pauli_masks(obs: str, n: int) -> Tuple[int, int, int]:
     return x, z, ny
 
 
+def _expval_terms(st: torch.Tensor, n: int, xs: tuple, zs: tuple) -> torch.Tensor:
+    bsz, T = st.shape[0], len(xs)
+    out = torch.empty((bsz, T), dtype=torch.float32, device=st.device)
+    ws = _workspace(st, bsz, n, T)
+    N.call("tqsv_expval_pauli", N.stream_of(st), st.data_ptr(), bsz, n, T, N.u64_array(xs), N.u64_array(zs),
+           N.fptr(out), T, ws.data_ptr(), ws.numel() * 4)
+    return out
+
+
 class ExpvalPauli(torch.autograd.Function):
     """Re<psi|P_t|psi> for a list of Pauli strings -> [bsz, T]."""
 
@@ -161,11 +170,7 @@ class ExpvalPauli(torch.autograd.Function):
     def forward(ctx, state: torch.Tensor, n: int, xs: tuple, zs: tuple, nys: tuple):
         bsz = state.shape[0]
         st = _rows(state, bsz)
-        T = len(xs)
-        out = torch.empty((bsz, T), dtype=torch.float32, device=state.device)
-        ws = _workspace(st, bsz, n, T)
-        N.call("tqsv_expval_pauli", N.stream_of(st), st.data_ptr(), bsz, n, T, N.u64_array(xs), N.u64_array(zs),
-               N.fptr(out), T, ws.data_ptr(), ws.numel() * 4)
+        out = _expval_terms(st, n, xs, zs)
         ctx.save_for_backward(st)
         ctx.meta = (n, xs, zs, nys, state.shape)
         return out
@@ -178,6 +183,54 @@ class ExpvalPauli(torch.autograd.Function):
         coef = 2.0 * g.to(torch.complex64) * phase[None, :]
         lam = _pauli_apply(st, n, xs, zs, coef)
         return lam.reshape(shape), None, None, None, None
+
+
+class ExpvalPauliSum(torch.autograd.Function):
+    """sum_t c_t Re<psi|P_t|psi> per row -> [bsz] (c: real weights).
+
+    When the state needs a gradient, the forward runs the passes that build
+    W = 2 sum_t c_t i^{nY_t} X^x_t Z^z_t psi and reads each row's value off
+    Re<psi|W> / 2 (tqsv_pauli_sum_grad); the backward is then g_row W_row,
+    one elementwise pass, instead of the expectation passes plus the
+    backward's own passes over the state.  Without a gradient the read-only
+    expectation kernels run."""
+
+    @staticmethod
+    def forward(ctx, state: torch.Tensor, n: int, xs: tuple, zs: tuple, nys: tuple, coef: tuple):
+        bsz = state.shape[0]
+        st = _rows(state, bsz)
+        if not ctx.needs_input_grad[0]:
+            c = torch.tensor(coef, dtype=torch.float32, device=st.device)
+            return _expval_terms(st, n, xs, zs) @ c
+        T = len(xs)
+        cw = torch.tensor([2.0 * float(c) * (1j) ** (k % 4) for c, k in zip(coef, nys)], dtype=torch.complex64,
+                          device=st.device)
+        w = torch.empty_like(st)
+        dot = torch.empty(bsz, dtype=torch.float32, device=st.device)
+        nws = int(N.lib().tqsv_pauli_sum_grad_workspace_floats(bsz, n))
+        if nws < 0:
+            raise ValueError(f"Pauli sums support 1 <= n_wires <= 32 (got {n})")
+        ws = torch.empty(max(nws, 1), dtype=torch.float32, device=st.device)
+        N.call("tqsv_pauli_sum_grad", N.stream_of(st), w.data_ptr(), st.data_ptr(), bsz, n, T, N.u64_array(xs),
+               N.u64_array(zs), cw.data_ptr(), N.fptr(dot), N.fptr(ws), ws.numel())
+        ctx.w = w
+        ctx.shape = state.shape
+        return dot * 0.5
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        lam = ctx.w * g.to(torch.float32)[:, None]
+        return lam.reshape(ctx.shape), None, None, None, None, None
+
+
+def expval_pauli_sum_state(state: torch.Tensor, n: int, observables: Sequence[str],
+                           coeffs: Sequence[float]) -> torch.Tensor:
+    """sum_t c_t <P_t> -> [bsz] for a [bsz, 2^n] state."""
+    if len(observables) != len(coeffs):
+        raise ValueError(f"{len(observables)} observables but {len(coeffs)} coefficients")
+    masks = [pauli_masks(o, n) for o in observables]
+    return ExpvalPauliSum.apply(state, n, tuple(m[0] for m in masks), tuple(m[1] for m in masks),
+                                tuple(m[2] for m in masks), tuple(float(c) for c in coeffs))
 
 
 def expval_pauli_strings(state: torch.Tensor, n: int, observables: Sequence[str]) -> torch.Tensor:

@@ -242,6 +242,24 @@ __global__ __launch_bounds__(EV_THREADS) void k_ex_part(const float2* __restrict
   if (threadIdx.x == 0) out[1] = si;
 }
 
+// Partials of Re <a_row|b_row>: block = (row, chunk of 2^lg amplitudes),
+// part[row * nchunks + chunk].
+__global__ __launch_bounds__(EV_THREADS) void k_rowdot_part(const float2* __restrict__ a,
+                                                            const float2* __restrict__ b, int n, int lg_chunk,
+                                                            float* __restrict__ part) {
+  __shared__ float red[EV_THREADS / 64];
+  const uint64_t row = blockIdx.x >> (n - lg_chunk);
+  const uint64_t chunk = blockIdx.x & ((1u << (n - lg_chunk)) - 1u);
+  const uint64_t base = (row << n) + (chunk << lg_chunk);
+  float d = 0.f;
+  for (uint64_t i = threadIdx.x; i < (1ull << lg_chunk); i += blockDim.x) {
+    const float2 x = a[base + i], y = b[base + i];
+    d = fmaf(x.x, y.x, fmaf(x.y, y.y, d));
+  }
+  const float t = block_sum(d, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // Sum per-chunk partials of one term: partial of (row, chunk) sits at
 // part[(row * nchunks + chunk) * cstride] (complex: two floats re, im).
 // out[row * ostride + ooff] = scale_re * Re(sum) - scale_im * Im(sum).
@@ -344,13 +362,20 @@ __device__ __forceinline__ void pg_load(const PGroup& g, const float2* __restric
 // and a term's coefficient (with the group's non-local phase folded in) is
 // wave-uniform, so a real or imaginary coefficient -- the usual case, i^{nY}
 // times a real weight -- costs two FMA per amplitude.
+//
+// dotpart != null: the workgroup also adds Re conj(psi_l) out_l over its
+// groups (all in one row) and writes the sum to dotpart[blockIdx.x]; on the
+// last pass over out this is the workgroup's part of Re <psi|out>.
 template <int NR>
 __global__ __launch_bounds__(PG_THREADS) void k_pauli_group_apply(float2* __restrict__ out,
                                                                   const float2* __restrict__ in, int n,
                                                                   PGroup g, const float2* __restrict__ coef,
                                                                   int64_t coef_stride, int accumulate,
-                                                                  int64_t n_groups, int64_t gpw) {
+                                                                  int64_t n_groups, int64_t gpw,
+                                                                  float* __restrict__ dotpart) {
   __shared__ float2 buf[NR * PG_THREADS];
+  __shared__ float red[PG_THREADS / 64];
+  float dsum = 0.f;
   const int tid = threadIdx.x;
   const uint32_t size = NR * PG_THREADS;
   // local index l = tid + 256 r: its bit positions are those of tid | those of r << 8
@@ -415,6 +440,17 @@ __global__ __launch_bounds__(PG_THREADS) void k_pauli_group_apply(float2* __rest
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) rout[tsc | rsc[r]] = acc[r];
+    if (dotpart) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const float2 a = buf[tid + r * PG_THREADS];
+        dsum = fmaf(a.x, acc[r].x, fmaf(a.y, acc[r].y, dsum));
+      }
+    }
+  }
+  if (dotpart) {
+    const float t = block_sum(dsum, red);
+    if (tid == 0) dotpart[blockIdx.x] = t;
   }
 }
 
@@ -478,15 +514,29 @@ __global__ __launch_bounds__(PG_THREADS) void k_pauli_group_expval(const float2*
 
 constexpr int PG_WG_PER_CU = 4;   // workgroups per CU of the grouped-term launches
 
+// Workgroups per row of the grouped-term launches: a power of two dividing
+// the 2^nnl groups of a row, about PG_WG_PER_CU per CU over the batch.
+static int64_t pg_wg_per_row(int64_t bsz, int nnl) {
+  const int64_t per_row = 1ll << nnl;
+  int64_t wpr = ((int64_t)num_cu() * PG_WG_PER_CU) / bsz;
+  if (wpr < 1) wpr = 1;
+  if (wpr > per_row) wpr = per_row;
+  while (per_row % wpr) --wpr;        // per_row is a power of two: wpr becomes one too
+  return wpr;
+}
+
+// One pass over the terms of a local set; every workgroup stays in one row
+// (wpr workgroups per row), so with dotpart each writes one row partial:
+// dotpart[row * wpr + k].
 template <int NR>
 static int pg_apply_launch(hipStream_t s, float2* out, const float2* in, int n, const PGroup& g, const float2* coef,
-                           int64_t coef_stride, int acc, int64_t bsz) {
-  const int64_t n_groups = bsz << g.nnl;
-  int64_t grid = (int64_t)num_cu() * PG_WG_PER_CU;
-  if (grid > n_groups) grid = n_groups;
-  const int64_t gpw = (n_groups + grid - 1) / grid;
-  grid = (n_groups + gpw - 1) / gpw;
-  k_pauli_group_apply<NR><<<(unsigned)grid, PG_THREADS, 0, s>>>(out, in, n, g, coef, coef_stride, acc, n_groups, gpw);
+                           int64_t coef_stride, int acc, int64_t bsz, float* dotpart) {
+  const int64_t wpr = pg_wg_per_row(bsz, g.nnl);
+  const int64_t gpw = (1ll << g.nnl) / wpr;
+  const int64_t grid = bsz * wpr;
+  if (grid > 0x7fffffffll) { set_error("tqsv_pauli_apply: batch too large"); return E_LIMIT; }
+  k_pauli_group_apply<NR><<<(unsigned)grid, PG_THREADS, 0, s>>>(out, in, n, g, coef, coef_stride, acc,
+                                                                bsz << g.nnl, gpw, dotpart);
   return check_launch("tqsv_pauli_apply/group");
 }
 
@@ -494,10 +544,7 @@ template <int NR>
 static int pg_expval_launch(hipStream_t s, const float2* in, int n, const PGroup& g, int64_t bsz, void* ws,
                             int64_t ws_bytes, int64_t& wpr) {
   const int64_t per_row = 1ll << g.nnl;
-  wpr = ((int64_t)num_cu() * PG_WG_PER_CU) / bsz;
-  if (wpr < 1) wpr = 1;
-  if (wpr > per_row) wpr = per_row;
-  while (per_row % wpr) --wpr;        // per_row is a power of two: wpr becomes one too
+  wpr = pg_wg_per_row(bsz, g.nnl);
   const int64_t gpw = per_row / wpr;
   const int64_t grid = bsz * wpr;
   if (grid > 0x7fffffffll || ws_bytes < grid * 2 * g.nt * (int64_t)sizeof(float)) return 1;   // caller falls back
@@ -767,37 +814,41 @@ TQSV_API int tqsv_expval_pauli(hipStream_t s, const void* state, int64_t bsz, in
   return OK;
 }
 
-// out (+)= sum_t coef[row, t] P_t psi   (coef: device complex64 [bsz, nterms],
-// already including i^{nY}); used for d/d psi of Pauli expectations.
-TQSV_API int tqsv_pauli_apply(hipStream_t s, void* out, const void* in, int64_t bsz, int n, int nterms,
-                              const uint64_t* xmask, const uint64_t* zmask, const void* coef, int accumulate) {
-  if (!out || !in || !coef || (nterms > 0 && (!xmask || !zmask))) { set_error("tqsv_pauli_apply: null pointer"); return E_ARG; }
-  if (out == in) { set_error("tqsv_pauli_apply: out must not alias in"); return E_ARG; }
-  if (bsz < 1 || n < 1 || n > 32 || nterms < 1) { set_error("tqsv_pauli_apply: bad sizes"); return E_ARG; }
-  for (int t = 0; t < nterms; ++t)
-    if ((xmask[t] | zmask[t]) >> n) { set_error("tqsv_pauli_apply: mask beyond n bits"); return E_ARG; }
+// out (+)= sum_t coef[row * coef_stride + t] P_t psi.  dotpart != null: the
+// last pass also leaves the row partials of Re <psi|out> there and sets
+// *dot_wpr to their count per row (dotpart[row * wpr + k]); 0 when the last
+// pass was a per-term launch, which computes no partials.
+static int pauli_apply_impl(hipStream_t s, void* out, const void* in, int64_t bsz, int n, int nterms,
+                            const uint64_t* xmask, const uint64_t* zmask, const void* coef, int64_t coef_stride,
+                            int accumulate, float* dotpart, int64_t* dot_wpr) {
   const uint64_t total = (uint64_t)bsz << n;
   int acc = accumulate ? 1 : 0;
+  if (dot_wpr) *dot_wpr = 0;
   // terms grouped by local set: one LDS-staged pass per set
   std::vector<char> done(nterms, 0);
-  if (n >= 8) {
-    const PGPlan P = pg_plan(n, nterms, xmask, zmask, true, PG_MAXT);
-    for (const PGroup& g : P.sets) {
-      float2* o = (float2*)out;
-      const float2* x = (const float2*)in;
-      const float2* c = (const float2*)coef;
-      int rc;
-      switch (g.m) {
-        case 8: rc = pg_apply_launch<1>(s, o, x, n, g, c, nterms, acc, bsz); break;
-        case 9: rc = pg_apply_launch<2>(s, o, x, n, g, c, nterms, acc, bsz); break;
-        case 10: rc = pg_apply_launch<4>(s, o, x, n, g, c, nterms, acc, bsz); break;
-        case 11: rc = pg_apply_launch<8>(s, o, x, n, g, c, nterms, acc, bsz); break;
-        default: rc = pg_apply_launch<16>(s, o, x, n, g, c, nterms, acc, bsz); break;
-      }
-      if (rc) return rc;
-      acc = 1;
-      for (int q = 0; q < g.nt; ++q) done[g.col[q]] = 1;
+  std::vector<PGroup> sets;
+  if (n >= 8) sets = pg_plan(n, nterms, xmask, zmask, true, PG_MAXT).sets;
+  int n_rest = 0;
+  for (const PGroup& g : sets)
+    for (int q = 0; q < g.nt; ++q) done[g.col[q]] = 1;
+  for (int t = 0; t < nterms; ++t) n_rest += !done[t];
+  for (size_t i = 0; i < sets.size(); ++i) {
+    const PGroup& g = sets[i];
+    float2* o = (float2*)out;
+    const float2* x = (const float2*)in;
+    const float2* c = (const float2*)coef;
+    float* dp = (dotpart && n_rest == 0 && i + 1 == sets.size()) ? dotpart : nullptr;
+    int rc;
+    switch (g.m) {
+      case 8: rc = pg_apply_launch<1>(s, o, x, n, g, c, coef_stride, acc, bsz, dp); break;
+      case 9: rc = pg_apply_launch<2>(s, o, x, n, g, c, coef_stride, acc, bsz, dp); break;
+      case 10: rc = pg_apply_launch<4>(s, o, x, n, g, c, coef_stride, acc, bsz, dp); break;
+      case 11: rc = pg_apply_launch<8>(s, o, x, n, g, c, coef_stride, acc, bsz, dp); break;
+      default: rc = pg_apply_launch<16>(s, o, x, n, g, c, coef_stride, acc, bsz, dp); break;
     }
+    if (rc) return rc;
+    if (dp) *dot_wpr = pg_wg_per_row(bsz, g.nnl);
+    acc = 1;
   }
   // the rest: gathered per amplitude, EV_MAXT terms per launch
   std::vector<int> rest;
@@ -812,10 +863,71 @@ TQSV_API int tqsv_pauli_apply(hipStream_t s, void* out, const void* in, int64_t 
       pt.col[q] = rest[t0 + q];
     }
     k_pauli_apply<<<grid_cap(total), EV_THREADS, 0, s>>>((float2*)out, (const float2*)in, n, total, pt,
-                                                         (const float2*)coef, nterms, acc);
+                                                         (const float2*)coef, coef_stride, acc);
     int rc = check_launch("tqsv_pauli_apply");
     if (rc) return rc;
     acc = 1;
   }
   return OK;
+}
+
+static int pauli_args_ok(const char* who, const void* out, const void* in, int64_t bsz, int n, int nterms,
+                         const uint64_t* xmask, const uint64_t* zmask, const void* coef) {
+  if (!out || !in || !coef || (nterms > 0 && (!xmask || !zmask))) { set_error("%s: null pointer", who); return E_ARG; }
+  if (out == in) { set_error("%s: out must not alias in", who); return E_ARG; }
+  if (bsz < 1 || n < 1 || n > 32 || nterms < 1) { set_error("%s: bad sizes", who); return E_ARG; }
+  for (int t = 0; t < nterms; ++t)
+    if ((xmask[t] | zmask[t]) >> n) { set_error("%s: mask beyond n bits", who); return E_ARG; }
+  return OK;
+}
+
+// out (+)= sum_t coef[row, t] P_t psi   (coef: device complex64 [bsz, nterms],
+// already including i^{nY}); used for d/d psi of Pauli expectations.
+TQSV_API int tqsv_pauli_apply(hipStream_t s, void* out, const void* in, int64_t bsz, int n, int nterms,
+                              const uint64_t* xmask, const uint64_t* zmask, const void* coef, int accumulate) {
+  int rc = pauli_args_ok("tqsv_pauli_apply", out, in, bsz, n, nterms, xmask, zmask, coef);
+  if (rc) return rc;
+  return pauli_apply_impl(s, out, in, bsz, n, nterms, xmask, zmask, coef, nterms, accumulate, nullptr, nullptr);
+}
+
+// Floats of workspace tqsv_pauli_sum_grad needs (row partials of the dot).
+TQSV_API int64_t tqsv_pauli_sum_grad_workspace_floats(int64_t bsz, int n) {
+  if (bsz < 1 || n < 1 || n > 32) return -1;
+  const int64_t fused = n >= 8 ? bsz * pg_wg_per_row(bsz, n - (n < PG_MAXM ? n : PG_MAXM)) : 0;
+  const int64_t plain = bsz << (n - ev_chunk_log2(bsz, n));
+  return fused > plain ? fused : plain;
+}
+
+// A Pauli sum and its state gradient in one go: out = sum_t coef[t] P_t psi
+// (coef: device complex64 [nterms], the same for every row, including
+// i^{nY}) and dot[row] = Re <psi_row|out_row> (device float [bsz]).  With
+// coef = 2 c_t i^{nY_t}, dot / 2 is the row's sum_t c_t <P_t> and
+// g_row * out is d(sum_row g_row value_row) / d conj(psi) -- what a
+// differentiated Pauli-sum loss needs, for the cost of the backward's passes
+// alone.  workspace: tqsv_pauli_sum_grad_workspace_floats floats.
+TQSV_API int tqsv_pauli_sum_grad(hipStream_t s, void* out, const void* in, int64_t bsz, int n, int nterms,
+                                 const uint64_t* xmask, const uint64_t* zmask, const void* coef, float* dot,
+                                 float* workspace, int64_t workspace_floats) {
+  int rc = pauli_args_ok("tqsv_pauli_sum_grad", out, in, bsz, n, nterms, xmask, zmask, coef);
+  if (rc) return rc;
+  if (!dot || !workspace) { set_error("tqsv_pauli_sum_grad: null pointer"); return E_ARG; }
+  if ((uintptr_t)in & 15) { set_error("tqsv_pauli_sum_grad: state must be 16-byte aligned"); return E_ARG; }
+  const int64_t need = tqsv_pauli_sum_grad_workspace_floats(bsz, n);
+  if (workspace_floats < need) {
+    set_error("tqsv_pauli_sum_grad: workspace of %lld floats, need %lld", (long long)workspace_floats,
+              (long long)need);
+    return E_ARG;
+  }
+  int64_t wpr = 0;
+  rc = pauli_apply_impl(s, out, in, bsz, n, nterms, xmask, zmask, coef, 0, 0, workspace, &wpr);
+  if (rc) return rc;
+  if (wpr == 0) {      // the last pass computed no partials: one read of psi and out
+    const int lg = ev_chunk_log2(bsz, n);
+    k_rowdot_part<<<(unsigned)(bsz << (n - lg)), EV_THREADS, 0, s>>>((const float2*)in, (const float2*)out, n, lg,
+                                                                      workspace);
+    if ((rc = check_launch("tqsv_pauli_sum_grad/dot"))) return rc;
+    wpr = 1ll << (n - lg);
+  }
+  k_sum_chunks<<<sum_grid(bsz), 256, 0, s>>>(workspace, (int)wpr, 1, 0, bsz, 1.f, 0.f, dot, 1, 0);
+  return check_launch("tqsv_pauli_sum_grad/sum");
 }

@@ -104,6 +104,8 @@ class EngineOptions:
     reorder_ops: within a stage, run commuting ops in an order that needs
                  fewer register-set changes (LDS remaps); ops sharing a wire
                  keep their order.
+    grad_cap:    at most this many ops with gradient parameters per stage
+                 (0: no limit); evens out the adjoint stages' arithmetic.
     io_layouts:  a stage may load in the register layout of its first ops
                  and store in that of its last (when the lowest 4 local bits
                  stay thread bits, so accesses still cover whole 128-byte
@@ -120,10 +122,11 @@ class EngineOptions:
     device_gates: bool = True
     reorder_ops: bool = True
     io_layouts: bool = True
+    grad_cap: int = 0
 
     def key(self):
         return (self.mode, self.stage_bits, self.low_bits, self.merge_1q, self.fusion_width, self.grid_cap,
-                self.adjoint_mg, self.jit, self.device_gates, self.reorder_ops, self.io_layouts)
+                self.adjoint_mg, self.jit, self.device_gates, self.reorder_ops, self.io_layouts, self.grad_cap)
 
 
 DEFAULT_OPTIONS = EngineOptions()
@@ -926,8 +929,10 @@ def stageable(k: KOp, m: int) -> bool:
     return True
 
 
-def partition(kops: List[KOp], n: int, m: int, low_bits: int) -> List[tuple]:
-    """Greedy DAG partition into stages of <= m local bits (+ solo ops)."""
+def partition(kops: List[KOp], n: int, m: int, low_bits: int, grad_cap: int = 0) -> List[tuple]:
+    """Greedy DAG partition into stages of <= m local bits (+ solo ops).
+    grad_cap > 0: at most that many ops with gradient parameters per stage
+    (their adjoint arithmetic, not memory, bounds a stage)."""
     steps = []
     remaining = list(range(len(kops)))
     forced = set(range(min(low_bits, n)))
@@ -941,9 +946,11 @@ def partition(kops: List[KOp], n: int, m: int, low_bits: int) -> List[tuple]:
         L = set(forced) if len(set(forced) | kops[first].allbits) <= m else set()
         blocked = set()
         inc, skip = [], []
+        n_grad = 0
         for idx in remaining:
             b = kops[idx].allbits
-            if b & blocked or not stageable(kops[idx], m):
+            capped = grad_cap > 0 and kops[idx].P > 0 and n_grad >= grad_cap and idx != first
+            if b & blocked or not stageable(kops[idx], m) or capped:
                 skip.append(idx)
                 blocked |= b
                 continue
@@ -951,6 +958,7 @@ def partition(kops: List[KOp], n: int, m: int, low_bits: int) -> List[tuple]:
             if len(nl) <= m:
                 L = nl
                 inc.append(idx)
+                n_grad += kops[idx].P > 0
             else:
                 skip.append(idx)
                 blocked |= b
@@ -1328,7 +1336,7 @@ class Compiled:
             self.grad_width = max(self.n_slots + MG_SUMS * len(items), 1)
             words = []
             base = 0
-            for kind, payload in partition(kops, n, m, opts.low_bits):
+            for kind, payload in partition(kops, n, m, opts.low_bits, opts.grad_cap):
                 if kind == "solo":
                     self.steps.append(("solo", payload))
                     continue

@@ -25,7 +25,7 @@ import torch
 
 from . import _native as N
 from . import operator as op
-from .autograd import ExpvalZAll, expval_pauli_strings
+from .autograd import ExpvalZAll, expval_pauli_strings, expval_pauli_sum_state
 from .macro import F_DTYPE
 from .module import QuantumModule
 from .noisedevice import NoiseDevice, density_expval_strings
@@ -137,9 +137,12 @@ def expval_joint_analytical_batch(qdev, observables: Sequence[str]) -> torch.Ten
 
 def expval_pauli_sum(qdev, observables: Sequence[str], coeffs: Sequence[float]) -> torch.Tensor:
     """sum_t c_t <P_t> -> [bsz]."""
-    ev = expval_joint_analytical_batch(qdev, observables)
-    c = torch.as_tensor(np.asarray(coeffs, dtype=np.float32), device=ev.device)
-    return ev @ c
+    if isinstance(qdev, NoiseDevice):
+        ev = expval_joint_analytical_batch(qdev, observables)
+        c = torch.as_tensor(np.asarray(coeffs, dtype=np.float32), device=ev.device)
+        return ev @ c
+    return expval_pauli_sum_state(qdev.get_states_1d(), qdev.n_wires, [o.upper() for o in observables],
+                                  [float(c) for c in coeffs])
 
 
 class MeasureMultiPauliSum(QuantumModule):
