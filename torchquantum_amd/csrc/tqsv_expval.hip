@@ -386,18 +386,32 @@ __global__ __launch_bounds__(PG_THREADS) void k_pauli_group_apply(float2* __rest
   const int64_t g0 = (int64_t)blockIdx.x * gpw;
   const int64_t g1 = g0 + gpw < n_groups ? g0 + gpw : n_groups;
   const uint64_t rmask = (1ull << g.nnl) - 1ull;
+  // this group's psi in registers (loaded one group ahead); it goes through
+  // LDS for the flipped terms, diagonal terms read it from registers
+  float2 a[NR];
+  if (g0 < g1) {
+    const float2* rin = in + (((uint64_t)g0 >> g.nnl) << n) + pg_group_offset(g, (uint64_t)g0 & rmask);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) a[r] = rin[tsc | rsc[r]];
+  }
   for (int64_t grp = g0; grp < g1; ++grp) {
     const uint64_t row = (uint64_t)grp >> g.nnl;
     const uint32_t goff = pg_group_offset(g, (uint64_t)grp & rmask);
-    const float2* rin = in + (row << n) + goff;
     float2* rout = out + (row << n) + goff;
-    __syncthreads();   // the previous group's LDS reads are done
-#pragma unroll
-    for (int r = 0; r < NR; ++r) buf[tid + r * PG_THREADS] = rin[tsc | rsc[r]];
-    __syncthreads();
     float2 acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = accumulate ? rout[tsc | rsc[r]] : make_float2(0.f, 0.f);
+    __syncthreads();   // the previous group's LDS reads are done
+#pragma unroll
+    for (int r = 0; r < NR; ++r) buf[tid + r * PG_THREADS] = a[r];
+    __syncthreads();
+    float2 an[NR];
+    if (grp + 1 < g1) {
+      const int64_t gn = grp + 1;
+      const float2* rin = in + (((uint64_t)gn >> g.nnl) << n) + pg_group_offset(g, (uint64_t)gn & rmask);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) an[r] = rin[tsc | rsc[r]];
+    }
     const float2* cf = coef + row * coef_stride;
     for (int t = 0; t < g.nt; ++t) {
       float2 c = cf[g.col[t]];
@@ -410,7 +424,14 @@ __global__ __launch_bounds__(PG_THREADS) void k_pauli_group_apply(float2* __rest
       const uint32_t tk = (uint32_t)tid ^ (xl & (PG_THREADS - 1));
       const uint32_t xh = xl >> 8, zh = zl >> 8;
       if (__popc(tk & zl & (PG_THREADS - 1)) & 1) c = make_float2(-c.x, -c.y);
-      if (c.y == 0.f) {          // real coefficient (wave-uniform branch)
+      if (xl == 0u && c.y == 0.f) {   // diagonal, real coefficient: own amplitudes
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const float cr = (__popc((uint32_t)r & zh) & 1) ? -c.x : c.x;
+          acc[r].x = fmaf(cr, a[r].x, acc[r].x);
+          acc[r].y = fmaf(cr, a[r].y, acc[r].y);
+        }
+      } else if (c.y == 0.f) {        // real coefficient (wave-uniform branch)
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
           const uint32_t rr = (uint32_t)r ^ xh;                   // wave-uniform
@@ -442,10 +463,11 @@ __global__ __launch_bounds__(PG_THREADS) void k_pauli_group_apply(float2* __rest
     for (int r = 0; r < NR; ++r) rout[tsc | rsc[r]] = acc[r];
     if (dotpart) {
 #pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const float2 a = buf[tid + r * PG_THREADS];
-        dsum = fmaf(a.x, acc[r].x, fmaf(a.y, acc[r].y, dsum));
-      }
+      for (int r = 0; r < NR; ++r) dsum = fmaf(a[r].x, acc[r].x, fmaf(a[r].y, acc[r].y, dsum));
+    }
+    if (grp + 1 < g1) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) a[r] = an[r];
     }
   }
   if (dotpart) {
