@@ -301,8 +301,9 @@ def dense_block_bench(n=26, bsz=16, iters=6):
 
 
 def time_batch(n_circ, steps, tq):
-    """256 circuits, n ~ U{8..22}, depth ~ U{4..40}: level-synchronous batch
-    launches captured in a HIP graph vs one engine call per circuit."""
+    """256 circuits, n ~ U{8..22}, depth ~ U{4..40}: the batch as one
+    persistent work-queue launch (and, for comparison, one launch per level),
+    captured in a HIP graph, vs one engine call per circuit."""
     from torchquantum_amd import batch as B
     import torchquantum_amd.functional as tqf
     circs = B.random_circuits(n_circ=n_circ, n_range=(8, 22), depth_range=(4, 40), seed=0)
@@ -325,6 +326,11 @@ def time_batch(n_circ, steps, tq):
         out = cb.run()
     torch.cuda.synchronize()
     t_graph = (time.perf_counter() - t0) / steps
+    cb.check()
+    cl = B.CircuitBatch(circs, device="cuda", mode="levels")
+    t_levels = _replay_ms(cl, steps)
+    same_as_levels = bool(torch.equal(cl.states, cb.states))
+    del cl
     # one engine call per circuit (fused plans, the same interpreting op code as
     # the batch kernel, Python loop over circuits)
     def per_circuit():
@@ -357,8 +363,11 @@ def time_batch(n_circ, steps, tq):
                          "pergate_equivalent_bytes": pergate_bytes(circs),
                          "note": "16 B x 2^n per executed stage of each circuit + the <Z_0> read, per graph "
                                  "replay; states of <= 64 MiB are cache-resident, so this is not an HBM figure"},
+            "mode": "one persistent work-queue launch (items level-major, each waiting for its circuit's "
+                    "previous stage)",
             "levels": len(cb.levels), "launches_per_batch": cb.n_launches,
             "ms_graph_replay": t_graph * 1e3, "ms_eager_batch": t_eager * 1e3, "ms_per_circuit_loop": t_loop * 1e3,
+            "ms_graph_replay_launch_per_level": t_levels, "states_equal_launch_per_level": same_as_levels,
             "amp_updates_per_s_graph": upd / t_graph, "speedup_vs_per_circuit_loop": t_loop / t_graph,
             "build_s": t_build, "z0_max_abs_diff_vs_per_circuit": float((out - ref).abs().max()),
             "plans": tuned}, \
@@ -435,7 +444,9 @@ def _replay_ms(cb, steps=5):
     for _ in range(steps):
         cb.run()
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps * 1e3
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    cb.check()
+    return ms
 
 
 def tune_batch(circs, tq):

@@ -225,6 +225,19 @@ int tqsv_mg_to_grad(tqsv_stream_t s, float* grad, int64_t grad_stride, int64_t b
 int tqsv_batch_item_bytes(void);
 int tqsv_batch_forward(tqsv_stream_t s, void* states, const void* items_dev, int n_items, const int* progs_dev,
                        const void* mats, int max_m, int grid);
+/* Every level of every circuit in ONE persistent launch: items (the layout
+ * of tqsv_batch_queue_item_bytes: a batch item plus its circuit index and
+ * the number of that circuit's items in earlier levels) listed level by
+ * level; workgroups take them in order from a device counter and wait, at
+ * most spin_limit polls, for the circuit's earlier levels.  ctrl: 2 device
+ * ints (ctrl[1] != 0 afterwards: a wait ran out and the results are not
+ * valid), done: n_circ device ints; both are reset by the call's first
+ * kernel.  Replaces, like tqsv_batch_forward, a Python loop of per-circuit
+ * gate_wrapper calls (gate_wrapper.py:324-446 per gate of every circuit),
+ * without a launch per level. */
+int tqsv_batch_queue(tqsv_stream_t s, void* states, const void* items, int n_items, const int* progs,
+                     const void* mats, int max_m, int* ctrl, int* done, int n_circ, int64_t spin_limit, int grid);
+int tqsv_batch_queue_item_bytes(void);
 /* per-circuit Z-string expectation (ZCirc array, tqsv_batch_zcirc_bytes()). */
 int tqsv_batch_zcirc_bytes(void);
 int tqsv_batch_zexpval(tqsv_stream_t s, const void* states, const void* circ_dev, int n_circ,

@@ -38,3 +38,28 @@ def test_batch_graph_replay():
         cb.states.zero_()          # eager work between replays
     refs = np.array([O.expval_z_all(O.run_ops(ops, n, 1), n)[0, 0] for n, ops in circs])
     assert np.abs(eager.cpu().numpy() - refs).max() < 1e-5
+
+
+@gpu
+def test_batch_queue_matches_levels():
+    """One persistent launch (mode "queue": items taken from a device counter,
+    each waiting for its circuit's previous stage) gives bitwise the states
+    of one launch per level, with no wait running out, eagerly and in a
+    replayed graph."""
+    from torchquantum_amd import batch as B
+    circs = B.random_circuits(n_circ=24, n_range=(6, 16), depth_range=(2, 12), seed=11)
+    cq = B.CircuitBatch(circs, device="cuda", mode="queue")
+    cl = B.CircuitBatch(circs, device="cuda", mode="levels")
+    zq, zl = cq.run().clone(), cl.run().clone()
+    torch.cuda.synchronize()
+    cq.check()
+    assert torch.equal(cq.states, cl.states)
+    assert torch.equal(zq, zl)
+    cq.capture()
+    for _ in range(3):
+        cq.states.zero_()
+        out = cq.run()
+        torch.cuda.synchronize()
+        cq.check()
+        assert torch.equal(cq.states, cl.states)
+        assert torch.equal(out, zl)

@@ -639,3 +639,26 @@ def _io_layouts_case(n, stage_bits, masks):
     assert np.abs(res[True][0] - res[False][0]).max() < 1e-12
     assert np.abs(res[True][1] - res[False][1]).max() < 1e-9 * (1 + np.abs(res[False][1]).max())
     assert np.abs(res[True][0] - O.run_ops(oracle_ops(ops, bsz), n, bsz)).max() < 1e-5
+
+
+def test_batch_queue_order():
+    """The uneven batch's single-launch queue (CircuitBatch mode "queue"):
+    items are the per-level work lists in level-major order, and each item's
+    `need` is the number of its circuit's items in earlier levels, all of
+    which precede it -- so the earliest unfinished item never waits and the
+    persistent kernel's queue drains."""
+    from torchquantum_amd import batch as B
+    circs = B.random_circuits(n_circ=12, n_range=(5, 14), depth_range=(2, 9), seed=4)
+    cb = B.CircuitBatch(circs, device="cpu")
+    q = np.frombuffer(cb.queue.numpy().tobytes(), dtype=B.QUEUE_ITEM)
+    lv = [np.frombuffer(items.numpy().tobytes(), dtype=B.BATCH_ITEM) for items, _ in cb.levels]
+    assert len(q) == cb.n_queue == sum(len(x) for x in lv)
+    assert np.array_equal(q["b"], np.concatenate(lv))
+    level = np.concatenate([np.full(len(x), L) for L, x in enumerate(lv)])
+    for k in range(len(q)):
+        c = q["circ"][k]
+        earlier = np.nonzero((q["circ"] == c) & (level < level[k]))[0]
+        assert q["need"][k] == len(earlier)
+        assert earlier.size == 0 or earlier.max() < k
+        # the item belongs to circuit c's state
+        assert q["b"]["state_off"][k] == cb.state_off[c]

@@ -56,6 +56,9 @@ SIGNATURES = {
                                     c_void_p, c_float_p, c_float_p, c_int64]),
     "tqsv_pauli_sum_grad_workspace_floats": (c_int64, [c_int64, c_int]),
     "tqsv_batch_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int]),
+    "tqsv_batch_queue": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                 c_int, c_int64, c_int]),
+    "tqsv_batch_queue_item_bytes": (c_int, []),
     "tqsv_batch_item_bytes": (c_int, []),
     "tqsv_batch_zcirc_bytes": (c_int, []),
     "tqsv_batch_zexpval": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int64, c_float_p, c_float_p]),

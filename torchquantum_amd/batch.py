@@ -1,12 +1,17 @@
-"""Many independent circuits of uneven width and depth in a few launches.
+"""Many independent circuits of uneven width and depth in one launch.
 
-Each circuit (bsz = 1) is compiled by the fused planner into stages.  Launch
-L of ``CircuitBatch.launch`` applies stage L of every circuit that has one,
-as one ``tqsv_batch_forward`` over a work list of (circuit, stage, group
-range) items, so the whole batch costs (max stages over circuits) launches
-plus one reduction, independent of the number of circuits.  All device
-buffers (states, programs, matrix pool, work lists) are built once, so the
-launch sequence can be captured in a HIP graph (``capture`` / ``replay``).
+Each circuit (bsz = 1) is compiled by the fused planner into stages, and
+every stage into (circuit, stage, group range) work items.  By default
+(``mode="queue"``) all items of all circuits run in ONE persistent launch
+(``tqsv_batch_queue``): listed level by level, taken in order from a device
+counter, each waiting only for its own circuit's previous stage, so a
+circuit's next stage starts as soon as its last one is done rather than when
+the slowest circuit's is.  ``mode="levels"`` launches one
+``tqsv_batch_forward`` per level instead (max stages over circuits launches,
+each ending when its slowest item does).  Both end with one reduction.  All
+device buffers (states, programs, matrix pool, work lists, queue counters)
+are built once, so the launch sequence can be captured in a HIP graph
+(``capture`` / ``replay``).
 
 The reference runs such a batch as one Python loop per circuit, each gate a
 separate permute + bmm (torchquantum/functional/gate_wrapper.py:96-140).
@@ -25,9 +30,11 @@ from . import gates as G
 
 BATCH_ITEM = np.dtype([("state_off", "<i8"), ("prog_off", "<i4"), ("n", "<i4"), ("init_zero", "<i4"),
                        ("pad", "<i4"), ("g0", "<i8"), ("g1", "<i8")])
+QUEUE_ITEM = np.dtype([("b", BATCH_ITEM), ("circ", "<i4"), ("need", "<i4")])
 ZCIRC = np.dtype([("state_off", "<i8"), ("n", "<i4"), ("zmask", "<u4"), ("part_off", "<i8")])
 ZCHUNK_LG = 14
 MAX_M = 12
+SPIN_LIMIT = 1 << 20      # polls of a waiting workgroup (each a short sleep): about a second, far above any stage
 
 
 def ops_to_tape(ops: Sequence[dict]) -> List[E.TapeOp]:
@@ -46,12 +53,17 @@ def ops_to_tape(ops: Sequence[dict]) -> List[E.TapeOp]:
 class CircuitBatch:
     def __init__(self, circuits: Sequence[tuple], device="cuda",
                  options: Optional[Sequence[E.EngineOptions]] = None, groups_per_item: int = 4,
-                 zwire: int = 0):
+                 zwire: int = 0, mode: str = "queue"):
         """circuits: [(n_wires, ops)], ops as {name, wires, params, inverse} dicts.
-        options: per-circuit EngineOptions (mode is forced to 'fused')."""
+        options: per-circuit EngineOptions (mode is forced to 'fused').
+        mode: "queue" (one persistent launch) or "levels" (a launch per level)."""
         lib = N.lib()
-        if lib.tqsv_batch_item_bytes() != BATCH_ITEM.itemsize or lib.tqsv_batch_zcirc_bytes() != ZCIRC.itemsize:
-            raise RuntimeError("BatchItem / ZCirc layout mismatch with libtqsv.so")
+        if (lib.tqsv_batch_item_bytes() != BATCH_ITEM.itemsize or lib.tqsv_batch_zcirc_bytes() != ZCIRC.itemsize
+                or lib.tqsv_batch_queue_item_bytes() != QUEUE_ITEM.itemsize):
+            raise RuntimeError("BatchItem / QueueItem / ZCirc layout mismatch with libtqsv.so")
+        if mode not in ("queue", "levels"):
+            raise ValueError(f"unknown batch mode {mode!r}")
+        self.mode = mode
         self.device = torch.device(device)
         self.n_circ = len(circuits)
         self.ns = [int(n) for n, _ in circuits]
@@ -101,21 +113,34 @@ class CircuitBatch:
         self.states = torch.zeros(tot, dtype=torch.complex64, device=self.device)
         self.progs = torch.from_numpy(np.concatenate(progs).astype(np.int32)).to(self.device)
         self.mats = torch.from_numpy(np.concatenate(pools).astype(np.complex64)).to(self.device)
-        # work lists per level
+        # work lists per level, and the same items as one level-major queue
         n_levels = max(len(o) for o in self.stage_offsets)
         self.levels = []
+        queue = []
+        seen = [0] * self.n_circ          # items of each circuit in earlier levels
         for L in range(n_levels):
             items = []
+            per_circ = [0] * self.n_circ
             for i in range(self.n_circ):
                 if L >= len(self.stage_offsets[i]):
                     continue
                 m = self.stage_m[i][L]
                 ng = 1 << (self.ns[i] - m)
                 for g0 in range(0, ng, groups_per_item):
-                    items.append((self.state_off[i], self.stage_offsets[i][L], self.ns[i], 1 if L == 0 else 0, 0,
-                                  g0, min(g0 + groups_per_item, ng)))
+                    b = (self.state_off[i], self.stage_offsets[i][L], self.ns[i], 1 if L == 0 else 0, 0,
+                         g0, min(g0 + groups_per_item, ng))
+                    items.append(b)
+                    queue.append((b, i, seen[i]))
+                    per_circ[i] += 1
+            for i in range(self.n_circ):
+                seen[i] += per_circ[i]
             arr = np.array(items, dtype=BATCH_ITEM)
             self.levels.append((torch.from_numpy(arr.view(np.uint8).copy()).to(self.device), len(items)))
+        qa = np.array(queue, dtype=QUEUE_ITEM)
+        self.queue = torch.from_numpy(qa.view(np.uint8).copy()).to(self.device)
+        self.n_queue = len(queue)
+        self.ctrl = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self.done = torch.zeros(self.n_circ, dtype=torch.int32, device=self.device)
         # <Z_zwire> per circuit
         zc = np.zeros(self.n_circ, dtype=ZCIRC)
         chunk_circ = []
@@ -134,14 +159,20 @@ class CircuitBatch:
 
     @property
     def n_launches(self) -> int:
-        return len(self.levels) + 2
+        return (2 if self.mode == "queue" else len(self.levels)) + 2
 
     def launch(self):
-        """Enqueue every level and the <Z> read-out on the current stream."""
+        """Enqueue every stage of every circuit and the <Z> read-out on the
+        current stream."""
         stream = N.stream_of(self.states)
-        for items, cnt in self.levels:
-            N.call("tqsv_batch_forward", stream, self.states.data_ptr(), items.data_ptr(), cnt,
-                   self.progs.data_ptr(), self.mats.data_ptr(), self.max_m, 0)
+        if self.mode == "queue":
+            N.call("tqsv_batch_queue", stream, self.states.data_ptr(), self.queue.data_ptr(), self.n_queue,
+                   self.progs.data_ptr(), self.mats.data_ptr(), self.max_m, self.ctrl.data_ptr(),
+                   self.done.data_ptr(), self.n_circ, SPIN_LIMIT, 0)
+        else:
+            for items, cnt in self.levels:
+                N.call("tqsv_batch_forward", stream, self.states.data_ptr(), items.data_ptr(), cnt,
+                       self.progs.data_ptr(), self.mats.data_ptr(), self.max_m, 0)
         N.call("tqsv_batch_zexpval", stream, self.states.data_ptr(), self.zcirc.data_ptr(), self.n_circ,
                self.chunk_circ.data_ptr(), self.n_chunks, N.fptr(self.part), N.fptr(self.out))
         return self.out
@@ -162,7 +193,14 @@ class CircuitBatch:
         self.graph = g
         return g
 
+    def check(self):
+        """Raise if a wait of the last queue launch ran out (synchronizes)."""
+        if self.mode == "queue" and int(self.ctrl[1].item()) != 0:
+            raise RuntimeError("batch queue: a workgroup's wait for its circuit's previous stage ran out; "
+                               "the batch results are not valid")
+
     def state(self, i: int) -> torch.Tensor:
+        self.check()
         o = int(self.state_off[i])
         return self.states[o:o + (1 << self.ns[i])]
 

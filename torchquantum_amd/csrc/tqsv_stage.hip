@@ -838,54 +838,144 @@ struct BatchItem {
   int64_t g0, g1;      // group range of this item
 };
 
+// One item: a stage program over groups [g0, g1) of one circuit's state.
+__device__ __forceinline__ void batch_item(float2* __restrict__ states, const BatchItem& item,
+                                           const int* __restrict__ progs, const float2* __restrict__ mats,
+                                           float2* lds, int tid) {
+  const int* __restrict__ h = progs + item.prog_off;
+  const int n_ops = h[H_NOPS], T = h[H_T], nnl = h[H_NNL];
+  const int* __restrict__ tpos = h + H_POS;
+  const int* __restrict__ rpos = tpos + T;
+  const int* __restrict__ nlpos = rpos + R;
+  const int* __restrict__ ops = h + h[H_OFF_OPS];
+  const bool active = tid < (1 << T);
+  uint32_t toff, roff[NS], soff, sroff[NS];   // load / store layout
+  layout_offsets(tpos, h[H_M], T, (h[H_FLAGS] >> FL_LOAD_SHIFT) & FL_MASK, tid, toff, roff);
+  layout_offsets(tpos, h[H_M], T, (h[H_FLAGS] >> FL_STORE_SHIFT) & FL_MASK, tid, soff, sroff);
+  const uint64_t rest_mask = (1ull << nnl) - 1ull;
+  float2* st = states + item.state_off;
+  for (int64_t g = item.g0; g < item.g1; ++g) {
+    const uint32_t goff = group_offset(nlpos, nnl, (uint64_t)g & rest_mask);
+    float2 a[NS];
+    if (item.init_zero) {
+#pragma unroll
+      for (int j = 0; j < NS; ++j) a[j] = make_float2((goff + toff + roff[j]) == 0 ? 1.f : 0.f, 0.f);
+    } else if (active) {
+      const float2* src = st + (uint64_t)goff + toff;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) a[j] = src[roff[j]];
+    }
+    for (int o = 0; o < n_ops; ++o) {
+      const int* __restrict__ w = ops + 8 * o;
+      if ((w[0] & 15) == OP_REMAP) {
+        const int* rec = h + w[7];
+        remap(a, lds, rec, rec + T + R, T, tid, active);
+        continue;
+      }
+      fwd_op<FEAT_MAT2 | FEAT_MATK>(a, w, mats + w[1], tid);
+    }
+    if (active) {
+      float2* dst = st + (uint64_t)goff + soff;
+      asm volatile("" : "+v"(dst));
+#pragma unroll
+      for (int j = 0; j < NS; ++j) dst[sroff[j]] = a[j];
+    }
+  }
+}
+
 __global__ __launch_bounds__(BATCH_THREADS) void k_batch_fwd(float2* __restrict__ states,
                                                              const BatchItem* __restrict__ items, int n_items,
                                                              const int* __restrict__ progs,
                                                              const float2* __restrict__ mats) {
   extern __shared__ float2 lds[];
-  const int tid = threadIdx.x;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    const BatchItem item = items[it];
-    const int* __restrict__ h = progs + item.prog_off;
-    const int n_ops = h[H_NOPS], T = h[H_T], nnl = h[H_NNL];
-    const int* __restrict__ tpos = h + H_POS;
-    const int* __restrict__ rpos = tpos + T;
-    const int* __restrict__ nlpos = rpos + R;
-    const int* __restrict__ ops = h + h[H_OFF_OPS];
-    const bool active = tid < (1 << T);
-    uint32_t toff, roff[NS], soff, sroff[NS];   // load / store layout
-    layout_offsets(tpos, h[H_M], T, (h[H_FLAGS] >> FL_LOAD_SHIFT) & FL_MASK, tid, toff, roff);
-    layout_offsets(tpos, h[H_M], T, (h[H_FLAGS] >> FL_STORE_SHIFT) & FL_MASK, tid, soff, sroff);
-    const uint64_t rest_mask = (1ull << nnl) - 1ull;
-    float2* st = states + item.state_off;
-    for (int64_t g = item.g0; g < item.g1; ++g) {
-      const uint32_t goff = group_offset(nlpos, nnl, (uint64_t)g & rest_mask);
-      float2 a[NS];
-      if (item.init_zero) {
-#pragma unroll
-        for (int j = 0; j < NS; ++j) a[j] = make_float2((goff + toff + roff[j]) == 0 ? 1.f : 0.f, 0.f);
-      } else if (active) {
-        const float2* src = st + (uint64_t)goff + toff;
-#pragma unroll
-        for (int j = 0; j < NS; ++j) a[j] = src[roff[j]];
-      }
-      for (int o = 0; o < n_ops; ++o) {
-        const int* __restrict__ w = ops + 8 * o;
-        if ((w[0] & 15) == OP_REMAP) {
-          const int* rec = h + w[7];
-          remap(a, lds, rec, rec + T + R, T, tid, active);
-          continue;
-        }
-        fwd_op<FEAT_MAT2 | FEAT_MATK>(a, w, mats + w[1], tid);
-      }
-      if (active) {
-        float2* dst = st + (uint64_t)goff + soff;
-        asm volatile("" : "+v"(dst));
-#pragma unroll
-        for (int j = 0; j < NS; ++j) dst[sroff[j]] = a[j];
-      }
+    batch_item(states, items[it], progs, mats, lds, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+// ---- one persistent launch for every level of every circuit ---------------
+// Items are listed level by level; item i of circuit c needs the `need`
+// items of c's earlier levels to be done.  Workgroups take items in list
+// order from a global counter (ctrl[0]), so whenever a workgroup waits, every
+// item it waits for is already held by a running workgroup, and the earliest
+// unfinished item never waits: the queue drains with any grid size.  A
+// finished item is published with a device-scope release (its stores, then
+// done[c] += 1); a waiting workgroup polls done[c] with acquire loads, at
+// most spin_limit times: past that it sets ctrl[1] (the results are then
+// wrong and the host reports it) and goes on, as does every later wait once
+// ctrl[1] is set, so every wave reaches the queue's end.
+struct QueueItem {
+  BatchItem b;
+  int32_t circ;   // circuit index (done[] slot)
+  int32_t need;   // items of this circuit in earlier levels
+};
+
+__global__ void k_queue_reset(int* __restrict__ ctrl, int* __restrict__ done, int n_circ) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2) ctrl[i] = 0;
+  if (i < n_circ) done[i] = 0;
+}
+
+// Control flow is uniform for the compiler throughout: per-wave choices test
+// the wave index taken through readfirstlane, never a per-lane value.  (With
+// "if (tid == 0)" around the claim and the release, the compiler treats the
+// loop as divergent and may move lane 0's part of the iteration boundary out
+// of the loop that the other lanes keep running -- past their barrier -- so
+// the loop never ends.)  Only wave 0 claims, waits and publishes:
+//   claim    one atomic add on the queue counter;
+//   wait     relaxed polls of done[c], then one device-scope acquire fence
+//            (it invalidates this CU's L1 and this XCD's L2, which the
+//            workgroup's other waves, ordered after it by the barrier, share);
+//   publish  every wave waits for its stores (workgroup-scope release), the
+//            barrier, then wave 0's device-scope release (L2 write-back) and
+//            a relaxed add to done[c].
+__global__ __launch_bounds__(BATCH_THREADS) void k_batch_queue(float2* __restrict__ states,
+                                                               const QueueItem* __restrict__ items, int n_items,
+                                                               const int* __restrict__ progs,
+                                                               const float2* __restrict__ mats,
+                                                               int* __restrict__ ctrl, int* __restrict__ done,
+                                                               int64_t spin_limit) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_item;
+  const int tid = threadIdx.x;
+  const bool wave0 = (__builtin_amdgcn_readfirstlane(tid) >> 6) == 0;   // wave-uniform
+  const int one = tid == 0 ? 1 : 0;
+  for (;;) {
+    if (wave0) {
+      const int v = atomicAdd(&ctrl[0], one);
+      s_item = __builtin_amdgcn_readfirstlane(v);
     }
     __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= n_items) break;
+    const QueueItem q = items[it];
+    if (wave0 && q.need > 0) {
+      int64_t spins = 0;
+      for (;;) {
+        const int d = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&done[q.circ], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (d >= q.need) break;
+        if (++spins > spin_limit) {
+          atomicExch(&ctrl[1], 1);
+          break;
+        }
+        // once any wait has run out, the others stop waiting too
+        if ((spins & 255) == 0 &&
+            __builtin_amdgcn_readfirstlane(__hip_atomic_load(&ctrl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+          break;
+        __builtin_amdgcn_s_sleep(8);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();                           // the wait is over, the previous item's LDS use too
+    batch_item(states, q.b, progs, mats, lds, tid);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // this wave's stores are done
+    __syncthreads();                           // ... in every wave; s_item has been read
+    if (wave0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_fetch_add(&done[q.circ], one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -1135,6 +1225,33 @@ TQSV_API int tqsv_batch_forward(hipStream_t s, void* states, const void* items_d
   return check_launch("tqsv_batch_forward");
 }
 
+// Every level of every circuit in one persistent launch (k_batch_queue).
+// items_dev: QueueItem array, level-major; ctrl_dev: 2 ints (queue counter,
+// error flag), done_dev: n_circ ints, both reset here by a small kernel (no
+// memset, so the pair can be captured in a HIP graph).  After the stream
+// reaches this point ctrl_dev[1] != 0 means a bounded wait ran out.
+TQSV_API int tqsv_batch_queue(hipStream_t s, void* states, const void* items_dev, int n_items,
+                              const int* progs_dev, const void* mats, int max_m, int* ctrl_dev, int* done_dev,
+                              int n_circ, int64_t spin_limit, int grid) {
+  if (!states || !items_dev || !progs_dev || !mats || !ctrl_dev || !done_dev || n_items < 0 || n_circ < 1) {
+    set_error("tqsv_batch_queue: null pointer or bad count");
+    return E_ARG;
+  }
+  if (max_m < R || max_m > 12) { set_error("tqsv_batch_queue: max_m must be in [4, 12]"); return E_LIMIT; }
+  if (spin_limit < 1) { set_error("tqsv_batch_queue: spin_limit must be positive"); return E_ARG; }
+  const int nr = n_circ > 2 ? n_circ : 2;
+  k_queue_reset<<<(nr + 255) / 256, 256, 0, s>>>(ctrl_dev, done_dev, n_circ);
+  int rc = check_launch("tqsv_batch_queue/reset");
+  if (rc || n_items == 0) return rc;
+  if (grid <= 0) grid = num_cu() * 4;
+  if (grid > n_items) grid = n_items;
+  const size_t shm = sizeof(float2) << max_m;
+  k_batch_queue<<<grid, BATCH_THREADS, shm, s>>>((float2*)states, (const QueueItem*)items_dev, n_items, progs_dev,
+                                                 (const float2*)mats, ctrl_dev, done_dev, spin_limit);
+  return check_launch("tqsv_batch_queue");
+}
+
+TQSV_API int tqsv_batch_queue_item_bytes(void) { return (int)sizeof(QueueItem); }
 TQSV_API int tqsv_batch_item_bytes(void) { return (int)sizeof(BatchItem); }
 TQSV_API int tqsv_batch_zcirc_bytes(void) { return (int)sizeof(ZCirc); }
 
