@@ -230,3 +230,33 @@ def test_load_store_layouts_follow_program():
     code = J.compile_source(J.PRELUDE + "".join(srcs[:4]), J.COMPILE_OPTS["a"])
     assert len(code) > 1000
 
+
+
+def test_matrix_core_products_in_forward_code():
+    """Forward stages run the 2x2 products of single-wire ops without
+    thread-bit controls on the matrix cores (tq_mm, one lane row each):
+    present for the merged RX.RY.RZ rotations, absent with mfma=False and
+    for ops with thread-bit controls; the code compiles for gfx950."""
+    import torchquantum_amd as tq
+    import torchquantum_amd.functional as tqf
+    from torchquantum_amd import engine as E, stagejit as J
+    n = 12
+    q = tq.QuantumDevice(n_wires=n, bsz=1, device="cpu")
+    for w in range(n):
+        for g in ("rx", "ry", "rz"):
+            tqf.func_name_dict[g](q, w, params=torch.tensor([[0.1 * (w + 1)]]))
+    c = E.Compiled(q._tape, n, 1, tq.EngineOptions(), "cpu")
+    (s,) = _stages(c)
+    on = J.gen_forward(s.words, "kon", mfma=True)
+    off = J.gen_forward(s.words, "koff", mfma=False)
+    n_mat1 = sum(1 for w in J.StageProgram(s.words).ops if (w[0] & 15) == E.OP_MAT1)
+    # 8 pairs per op; the body appears twice (the |0..0>-start loop and the loop that reads)
+    assert on.count("tq_mm(") == 16 * n_mat1 and "tq_mm(" not in off
+    assert "tq_row_u(" in on and "mhi" in on
+    # a controlled rotation whose control is a thread bit: VALU code
+    q = tq.QuantumDevice(n_wires=n, bsz=1, device="cpu")
+    tqf.crx(q, [n - 1, 0], params=torch.tensor([[0.3]]))
+    c = E.Compiled(q._tape, n, 1, tq.EngineOptions(merge_1q=False), "cpu")
+    (s2,) = _stages(c)
+    assert "tq_mm(" not in J.gen_forward(s2.words, "kc", mfma=True)
+    assert len(J.compile_source(J.PRELUDE + on)) > 1000

@@ -29,7 +29,7 @@ import os
 import queue
 import threading
 from pathlib import Path
-from typing import Dict, List, Optional, Sequence
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -55,7 +55,14 @@ COMPILE_OPTS["p"] = COMPILE_OPTS["a"]     # pruned adjoint stages (gen_adjoint w
 # more waves per SIMD), next-group loads issued ahead in forward stages only
 # (in adjoint stages their 64 registers cost more occupancy than the overlap
 # gains), no register accumulation of gradient partials (same trade).
-CODEGEN = {"buf": True, "prefetch_f": True, "prefetch_a": False, "lds": "pad4", "racc": 0}
+# mfma_f / mfma_a: the 2x2 complex products of single-wire ops without
+# thread-bit controls (forward U, adjoint U^dagger on psi and lam) run on the
+# matrix cores (tq_mm).  Forward stages: 18.7 -> 18.0 ms over five stages of
+# the gradient workload.  Adjoint stages: half the vector instructions, but
+# 176 VGPRs + 64 AGPRs against 106 (or spills when held to 128), and
+# 42.1 -> 44.2 ms; they stay on the vector ALUs.
+CODEGEN = {"buf": True, "prefetch_f": True, "prefetch_a": False, "lds": "pad4", "racc": 0,
+           "mfma_f": True, "mfma_a": False}
 
 PRELUDE = r"""
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -93,6 +100,43 @@ __device__ __forceinline__ void tq_st(__amdgpu_buffer_rsrc_t r, unsigned vo, int
   __builtin_amdgcn_raw_buffer_store_b64(v, r, vo, so, 0);
 }
 __device__ __forceinline__ float redot(float2 a, float2 b) { return fmaf(a.x, b.x, a.y * b.y); }
+// A 2x2 complex matrix times this lane's amplitude pair on the matrix cores.
+// v_mfma_f32_4x4x1_16b (16 blocks of 4 lanes): output register i of lane
+// 4b + j accumulates A(lane 4b + i) * B(lane 4b + j) -- a layout checked on
+// MI355X -- so with A = row (lane & 3) of the 4x4 real form of the matrix
+// (one k-step per column) and B = component k of this lane's own pair
+// (Re x0, Im x0, Re x1, Im x1), the lane receives its pair's product.  The
+// four lanes of a block must share the matrix (no per-lane controls).
+typedef float tq_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ tq_f4 tq_mm(float m0, float m1, float m2, float m3, float2 x0, float2 x1) {
+  tq_f4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m0, x0.x, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m1, x0.y, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m2, x1.x, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m3, x1.y, acc, 0, 0, 0);
+  return acc;
+}
+// Row (2 hi + odd) of the real form of U = [[u0, u1], [u2, u3]] acting on
+// (Re x0, Im x0, Re x1, Im x1): rows Re y0, Im y0, Re y1, Im y1.
+__device__ __forceinline__ tq_f4 tq_row_u(float2 u0, float2 u1, float2 u2, float2 u3, bool hi, bool odd) {
+  const float2 p = hi ? u2 : u0, q = hi ? u3 : u1;
+  tq_f4 r;
+  r[0] = odd ? p.y : p.x;
+  r[1] = odd ? p.x : -p.y;
+  r[2] = odd ? q.y : q.x;
+  r[3] = odd ? q.x : -q.y;
+  return r;
+}
+// The same for U^dagger: y0 = conj(u0) x0 + conj(u2) x1, y1 = conj(u1) x0 + conj(u3) x1.
+__device__ __forceinline__ tq_f4 tq_row_udag(float2 u0, float2 u1, float2 u2, float2 u3, bool hi, bool odd) {
+  const float2 p = hi ? u1 : u0, q = hi ? u3 : u2;
+  tq_f4 r;
+  r[0] = odd ? -p.y : p.x;
+  r[1] = odd ? p.x : p.y;
+  r[2] = odd ? -q.y : q.x;
+  r[3] = odd ? q.x : q.y;
+  return r;
+}
 __device__ __forceinline__ float2 csel(bool c, float2 a, float2 b) {
   return make_float2(c ? a.x : b.x, c ? a.y : b.y);
 }
@@ -219,8 +263,10 @@ def supported(words: np.ndarray) -> bool:
 
 
 class _Gen:
-    def __init__(self, sp: StageProgram, lds_mode: str = "xor"):
+    def __init__(self, sp: StageProgram, lds_mode: str = "xor", mfma: bool = False):
         self.sp = sp
+        self.mfma = mfma          # 2x2 products of uncontrolled single-wire ops on the matrix cores
+        self.used_mfma = False
         self.lines: List[str] = []
         self.k = 0
         self.ind = 2
@@ -346,12 +392,17 @@ class _Gen:
                                  f"fmaf({r[2]}, {x0}.y, {r[3]} * {x1}.y))")
             return
         u = [self.var(f"{M}[{i}]") for i in range(4)]
-        u = self.select_identity(self.thread_ok(w), u, 2)
+        tk = self.thread_ok(w)
+        u = self.select_identity(tk, u, 2)
+        row = self.mm_row(u, adjoint=False) if self.mfma and tk is None else None
         for j in range(NS):
             if j & (1 << K) or (j & rm) != rv:
                 continue
             j1 = j | (1 << K)
             x0, x1 = a[j], a[j1]
+            if row is not None:
+                a[j], a[j1] = self.mm_pair(row, x0, x1)
+                continue
             a[j] = self.var(f"cfma({u[1]}, {x1}, cmul({u[0]}, {x0}))")
             a[j1] = self.var(f"cfma({u[3]}, {x1}, cmul({u[2]}, {x0}))")
 
@@ -485,6 +536,17 @@ class _Gen:
             raise ValueError(f"unknown stage op {t}")
 
     # ---- adjoint ops (k_stage_adj's bwd_op); each returns per-lane partials
+    def mm_row(self, u: List[str], adjoint: bool) -> str:
+        """This lane's row of the real 4x4 form of U (or U^dagger)."""
+        self.used_mfma = True
+        fn = "tq_row_udag" if adjoint else "tq_row_u"
+        return self.var(f"{fn}({u[0]}, {u[1]}, {u[2]}, {u[3]}, mhi, modd)", "tq_f4")
+
+    def mm_pair(self, row: str, x0: str, x1: str) -> Tuple[str, str]:
+        """The lane's pair (x0, x1) times the matrix whose row is `row`."""
+        v = self.var(f"tq_mm({row}[0], {row}[1], {row}[2], {row}[3], {x0}, {x1})", "tq_f4")
+        return self.var(f"make_float2({v}[0], {v}[1])"), self.var(f"make_float2({v}[2], {v}[3])")
+
     def acc(self, cur: Optional[str], term: str) -> str:
         return self.var(term if cur is None else f"{cur} + {term}", "float")
 
@@ -505,6 +567,7 @@ class _Gen:
         M = self.mat_base(w)
         tk = self.thread_ok(w)
         u = self.select_identity(tk, [self.var(f"{M}[{i}]") for i in range(4)], 2)
+        row = self.mm_row(u, adjoint=True) if self.mfma and tk is None else None
         Dm = [[f"{M}[{4 * (q + 1) + i}]" for i in range(4)] for q in range(P)]
         g: List[Optional[str]] = [None] * P
         for j in range(NS):
@@ -512,8 +575,11 @@ class _Gen:
                 continue
             j1 = j | (1 << K)
             x0, x1 = p[j], p[j1]
-            y0 = self.var(f"cfmac({u[2]}, {x1}, cmulc({u[0]}, {x0}))")
-            y1 = self.var(f"cfmac({u[3]}, {x1}, cmulc({u[1]}, {x0}))")
+            if row is not None:
+                y0, y1 = self.mm_pair(row, x0, x1)
+            else:
+                y0 = self.var(f"cfmac({u[2]}, {x1}, cmulc({u[0]}, {x0}))")
+                y1 = self.var(f"cfmac({u[3]}, {x1}, cmulc({u[1]}, {x0}))")
             l0, l1 = l[j], l[j1]
             for q in range(P):
                 D = Dm[q]
@@ -521,8 +587,11 @@ class _Gen:
                 v1 = self.var(f"cfma({D[3]}, {y1}, cmul({D[2]}, {y0}))")
                 g[q] = self.acc(g[q], f"(redot({l0}, {v0}) + redot({l1}, {v1}))")
             if update:
-                n0 = self.var(f"cfmac({u[2]}, {l1}, cmulc({u[0]}, {l0}))")
-                n1 = self.var(f"cfmac({u[3]}, {l1}, cmulc({u[1]}, {l0}))")
+                if row is not None:
+                    n0, n1 = self.mm_pair(row, l0, l1)
+                else:
+                    n0 = self.var(f"cfmac({u[2]}, {l1}, cmulc({u[0]}, {l0}))")
+                    n1 = self.var(f"cfmac({u[3]}, {l1}, cmulc({u[1]}, {l0}))")
                 p[j], p[j1], l[j], l[j1] = y0, y1, n0, n1
         return self.lane_gate(tk, g)
 
@@ -536,6 +605,7 @@ class _Gen:
         M = self.mat_base(w)
         tk = None if nc else self.thread_ok(w)
         u = self.select_identity(tk, [self.var(f"{M}[{i}]") for i in range(4)], 2)
+        row = self.mm_row(u, adjoint=True) if self.mfma and tk is None and update else None
         a = b = c = d = None
         for j in range(NS):
             if j & (1 << K) or (j & rm) != rv:
@@ -550,10 +620,14 @@ class _Gen:
             d = self.var(f"cfmac({l1}, {x0}, {d if d else ZERO})")
             if not update:
                 continue
-            y0 = self.var(f"cfmac({u[2]}, {x1}, cmulc({u[0]}, {x0}))")
-            y1 = self.var(f"cfmac({u[3]}, {x1}, cmulc({u[1]}, {x0}))")
-            n0 = self.var(f"cfmac({u[2]}, {l1}, cmulc({u[0]}, {l0}))")
-            n1 = self.var(f"cfmac({u[3]}, {l1}, cmulc({u[1]}, {l0}))")
+            if row is not None:
+                y0, y1 = self.mm_pair(row, x0, x1)
+                n0, n1 = self.mm_pair(row, l0, l1)
+            else:
+                y0 = self.var(f"cfmac({u[2]}, {x1}, cmulc({u[0]}, {x0}))")
+                y1 = self.var(f"cfmac({u[3]}, {x1}, cmulc({u[1]}, {x0}))")
+                n0 = self.var(f"cfmac({u[2]}, {l1}, cmulc({u[0]}, {l0}))")
+                n1 = self.var(f"cfmac({u[3]}, {l1}, cmulc({u[1]}, {l0}))")
             p[j], p[j1], l[j], l[j1] = y0, y1, n0, n1
         if a is None:
             return self.lane_gate(tk, [None] * 4)
@@ -778,6 +852,17 @@ def _lane_offsets(sp: StageProgram) -> List[str]:
     return out
 
 
+def _mfma(kind: str, mfma: Optional[bool] = None) -> bool:
+    kind = "a" if kind == "p" else kind
+    return CODEGEN["mfma_" + kind] if mfma is None else bool(mfma)
+
+
+def _lane_bits(g: "_Gen") -> List[str]:
+    """Lane-position flags the matrix-core rows need (tq_row_u / tq_row_udag)."""
+    return (["  const bool mhi = (threadIdx.x & 2u) != 0u;", "  const bool modd = (threadIdx.x & 1u) != 0u;"]
+            if g.used_mfma else [])
+
+
 def _codegen(buf: Optional[bool], prefetch: Optional[bool], kind: str, lds: Optional[str] = None,
              racc: Optional[int] = None):
     kind = "a" if kind == "p" else kind
@@ -788,7 +873,7 @@ def _codegen(buf: Optional[bool], prefetch: Optional[bool], kind: str, lds: Opti
 
 
 def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Optional[bool] = None,
-                prefetch: Optional[bool] = None, lds: Optional[str] = None) -> str:
+                prefetch: Optional[bool] = None, lds: Optional[str] = None, mfma: Optional[bool] = None) -> str:
     """HIP source of one forward stage kernel (k_stage_fwd semantics).
     nontemporal: the state is read and written with non-temporal hints;
     buf: buffer instructions for the state where the offsets allow;
@@ -801,7 +886,7 @@ def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
         raise ValueError("compiled stages need T >= 6")
     buf, prefetch, lds, _ = _codegen(buf, prefetch, "f", lds)
     mode = _mem_mode(sp, nontemporal, buf)
-    g = _Gen(sp, lds)
+    g = _Gen(sp, lds, _mfma("f", mfma))
     a = [f"i{j}" for j in range(NS)]
     has_remap = False
     for w in sp.ops:
@@ -821,6 +906,7 @@ def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
         src.append(f"  __shared__ float2 lds[{lds_words(sp.m, lds)}];")
     src.append("  const int tid = threadIdx.x;")
     src += _lane_offsets(sp)
+    src += _lane_bits(g)
     src += g.hoisted()
     src += ["  const long long g0 = (long long)blockIdx.x * gpw;",
             "  long long g1 = g0 + gpw;",
@@ -858,7 +944,7 @@ def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
 
 def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Optional[bool] = None,
                 prefetch: Optional[bool] = None, lds: Optional[str] = None, racc: Optional[int] = None,
-                skip: Sequence[int] = (), no_store: bool = False) -> str:
+                skip: Sequence[int] = (), no_store: bool = False, mfma: Optional[bool] = None) -> str:
     """HIP source of one adjoint stage kernel (k_stage_adj semantics);
     nontemporal, buf, prefetch and lds as for gen_forward (prefetch loads
     psi and lam of the next group).  racc: when the stage has at most this
@@ -876,7 +962,7 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
         raise ValueError("compiled stages need T >= 6")
     buf, prefetch, lds, racc = _codegen(buf, prefetch, "a", lds, racc)
     mode = _mem_mode(sp, nontemporal, buf)
-    g = _Gen(sp, lds)
+    g = _Gen(sp, lds, _mfma("a", mfma))
     p = [f"p{j}" for j in range(NS)]
     l = [f"l{j}" for j in range(NS)]
     has_remap = False
@@ -936,6 +1022,7 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
     src.append("  const int tid = threadIdx.x;")
     src.append("  const int lane = tid & 63;")
     src += _lane_offsets(sp)
+    src += _lane_bits(g)
     src += g.hoisted()
     src += ["  const long long g0 = (long long)blockIdx.x * gpw;",
             "  long long g1 = g0 + gpw;",
@@ -1019,7 +1106,7 @@ def kernel_name(kind: str, prog, nt: bool = False) -> str:
     if extra:
         hs.update(repr(sorted(extra.items())).encode())
     hs.update(COMPILE_OPTS[kind].encode())
-    hs.update(repr(_codegen(None, None, kind)).encode())
+    hs.update(repr((_codegen(None, None, kind), _mfma(kind))).encode())
     if nt:
         hs.update(b"nt")
     h = hs.hexdigest()[:20]
