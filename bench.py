@@ -78,7 +78,7 @@ def time_forward(n, bsz, ops, mode, steps, warmup, tq, jit="auto"):
     return dt, out
 
 
-STAGE_BITS = 0      # local bits per stage of the gradient workload's plans (0 = the engine's choice)
+STAGE_BITS = 0      # local bits per stage of the fused plans (0 = the engine's choice)
 LOW_BITS = None     # lowest bits kept local in every stage (None = the engine's choice)
 IO_LAYOUTS = True   # stages load / store in the layouts of their first / last ops where lines stay whole
 GRAD_CAP = 0        # ops with gradient parameters per stage (0 = no limit)
@@ -89,6 +89,8 @@ def _plan_opts(mode="fused"):
     if mode != "fused":
         return {}
     d = {} if LOW_BITS is None else {"low_bits": LOW_BITS}
+    if STAGE_BITS:
+        d["stage_bits"] = STAGE_BITS
     if not IO_LAYOUTS:
         d["io_layouts"] = False
     if GRAD_CAP:
@@ -137,7 +139,7 @@ def _ansatz_loss_step(n, bsz, layers, mode, tq, jit, params, phi, ops, obs, coef
     for p in params:
         p.grad = None
     q = tq.QuantumDevice(n_wires=n, bsz=bsz, device="cuda")
-    q.engine_options = tq.EngineOptions(mode=mode, jit=jit, stage_bits=STAGE_BITS, **_plan_opts(mode))
+    q.engine_options = tq.EngineOptions(mode=mode, jit=jit, **_plan_opts(mode))
     for w in range(n):
         tqf.ry(q, w, params=phi)
     k = 0
@@ -857,7 +859,7 @@ def main():
     ap.add_argument("--canonical-io", action="store_true",
                     help="fused plans load and store every stage in the canonical layout (remaps at both ends)")
     ap.add_argument("--stage-bits", type=int, default=0,
-                    help="local bits per stage for the gradient workload (default: the engine's choice)")
+                    help="local bits per stage of the fused plans (default: the engine's choice)")
     ap.add_argument("--c5-fusion-width", type=int, default=1,
                     help="fusion width of the 30-qubit workload's fused plan (1 = merged single-wire runs)")
     ap.add_argument("--config5-plans", action="store_true",
