@@ -63,3 +63,33 @@ def test_batch_queue_matches_levels():
         cq.check()
         assert torch.equal(cq.states, cl.states)
         assert torch.equal(out, zl)
+
+
+@gpu
+def test_batch_queue_wait_that_runs_out_is_reported():
+    """The queue's bounded wait, forced to run out from the host: one item is
+    given a wait count its circuit never reaches and the poll limit is made
+    small, so that wait ends by its limit; the launch still finishes, the
+    error flag is set and check() raises. Restored, the next launch is valid
+    again (the flag is reset by the launch's first kernel)."""
+    from torchquantum_amd import batch as B
+    circs = B.random_circuits(n_circ=6, n_range=(6, 12), depth_range=(2, 6), seed=5)
+    cb = B.CircuitBatch(circs, device="cuda")
+    ref = B.CircuitBatch(circs, device="cuda", mode="levels")
+    ref.run()
+    q = np.frombuffer(cb.queue.cpu().numpy().tobytes(), dtype=B.QUEUE_ITEM).copy()
+    last = len(q) - 1
+    saved = int(q["need"][last])
+    q["need"][last] = 1 << 30
+    cb.queue.copy_(torch.from_numpy(q.view(np.uint8).copy()))
+    cb.spin_limit = 1000
+    cb.run()
+    torch.cuda.synchronize()
+    with pytest.raises(RuntimeError, match="ran out"):
+        cb.check()
+    q["need"][last] = saved
+    cb.queue.copy_(torch.from_numpy(q.view(np.uint8).copy()))
+    cb.run()
+    torch.cuda.synchronize()
+    cb.check()
+    assert torch.equal(cb.states, ref.states)

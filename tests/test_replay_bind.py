@@ -66,3 +66,17 @@ def test_structure_change_rejected():
         xb = xa.clone()
         xa[0, 1], xb[0, 1] = 0.9, 0.1
         bind_inputs(_tape(record, xa), _tape(record, xb), [xa], [xb], BSZ)
+
+
+def test_many_encoder_columns_bind_quickly():
+    """An encoder with hundreds of input columns (an image-sized input):
+    each gate's column is found by a host-side index, not by comparing
+    against every column."""
+    import time
+    n_feat = 400
+    enc = tq.GeneralEncoder([{"input_idx": [(7 * i) % n_feat], "func": "ry", "wires": [i % N]}
+                             for i in range(n_feat)])
+    t0 = time.perf_counter()
+    binds = _two(lambda q, x: enc(q, x), [(BSZ, n_feat)], seed=2)
+    assert time.perf_counter() - t0 < 20.0
+    assert binds == [(i, 0, [(7 * i) % n_feat]) for i in range(n_feat)]

@@ -148,6 +148,7 @@ class CircuitBatch:
         self.n_queue = len(queue)
         self.ctrl = torch.zeros(2, dtype=torch.int32, device=self.device)
         self.done = torch.zeros(self.n_circ, dtype=torch.int32, device=self.device)
+        self.spin_limit = SPIN_LIMIT
         # <Z_zwire> per circuit
         zc = np.zeros(self.n_circ, dtype=ZCIRC)
         chunk_circ = []
@@ -175,7 +176,7 @@ class CircuitBatch:
         if self.mode == "queue":
             N.call("tqsv_batch_queue", stream, self.states.data_ptr(), self.queue.data_ptr(), self.n_queue,
                    self.progs.data_ptr(), self.mats.data_ptr(), self.max_m, self.ctrl.data_ptr(),
-                   self.done.data_ptr(), self.n_circ, SPIN_LIMIT, 0)
+                   self.done.data_ptr(), self.n_circ, self.spin_limit, 0)
         else:
             for items, cnt in self.levels:
                 N.call("tqsv_batch_forward", stream, self.states.data_ptr(), items.data_ptr(), cnt,

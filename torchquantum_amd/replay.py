@@ -42,6 +42,7 @@ from __future__ import annotations
 
 from typing import Callable, List, Optional, Sequence, Tuple
 
+import numpy as np
 import torch
 
 from . import _native as N
@@ -75,6 +76,25 @@ def bind_inputs(tape_a, tape_b, inputs_a: Sequence[torch.Tensor], inputs_b: Sequ
                                          for a, b in zip(tape_a, tape_b)):
         raise ValueError("the recorded circuit depends on the input values (gate sequence or wires changed "
                          "between two recordings); a replay needs a fixed structure")
+    cache = {}
+
+    def columns(k, dtype):
+        """Input k as host columns of `dtype` in recording a, and a map from
+        the bytes of each column in recording b to its index (first wins)."""
+        key = (k, dtype)
+        if key not in cache:
+            xa, xb = inputs_a[k], inputs_b[k]
+            if xa.dim() == 0 or xa.shape[0] != bsz:
+                cache[key] = None
+            else:
+                fa = xa.detach().reshape(bsz, -1).to(dtype).cpu().numpy()
+                fb = xb.detach().reshape(bsz, -1).to(dtype).cpu().numpy()
+                index = {}
+                for c in range(fb.shape[1]):
+                    index.setdefault(np.ascontiguousarray(fb[:, c]).tobytes(), c)
+                cache[key] = (fa, index)
+        return cache[key]
+
     gathers = []
     for i, (a, b) in enumerate(zip(tape_a, tape_b)):
         pa, pb = a.params, b.params
@@ -92,19 +112,20 @@ def bind_inputs(tape_a, tape_b, inputs_a: Sequence[torch.Tensor], inputs_b: Sequ
                f"the input")
         if pa.is_complex() or pa.dim() == 0 or pa.shape[0] != bsz or pa.shape != pb.shape:
             raise ValueError(msg)
-        ca, cb = pa.reshape(bsz, -1), pb.reshape(bsz, -1)
+        ca = pa.detach().reshape(bsz, -1).cpu().numpy()
+        cb = pb.detach().reshape(bsz, -1).cpu().numpy()
         found = None
-        for k, (xa, xb) in enumerate(zip(inputs_a, inputs_b)):
-            if xa.dim() == 0 or xa.shape[0] != bsz:
+        for k in range(len(inputs_a)):
+            cols_of = columns(k, pa.dtype)
+            if cols_of is None:
                 continue
-            fa, fb = xa.reshape(bsz, -1).to(pa.dtype), xb.reshape(bsz, -1).to(pb.dtype)
+            fa = cols_of[0]
             cols = []
             for j in range(ca.shape[1]):
-                hit = [c for c in range(fb.shape[1]) if torch.equal(fb[:, c], cb[:, j]) and
-                       torch.equal(fa[:, c], ca[:, j])]
-                if not hit:
+                hit = cols_of[1].get(cb[:, j].tobytes())
+                if hit is None or not np.array_equal(fa[:, hit], ca[:, j]):
                     break
-                cols.append(hit[0])
+                cols.append(hit)
             if len(cols) == ca.shape[1]:
                 found = (i, k, cols)
                 break
