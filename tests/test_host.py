@@ -662,3 +662,16 @@ def test_batch_queue_order():
         assert earlier.size == 0 or earlier.max() < k
         # the item belongs to circuit c's state
         assert q["b"]["state_off"][k] == cb.state_off[c]
+
+
+def test_default_stage_width_follows_state_size():
+    """Automatic stage width: 13 local bits for states of more than 512 MiB
+    (HBM-resident: fewer passes win), 12 below (cache-resident), never more
+    than the qubit count; an explicit stage_bits wins."""
+    o = E.EngineOptions()
+    assert E.choose_stage_bits(24, o, 64) == 13          # 8 GiB
+    assert E.choose_stage_bits(30, o, 4) == 13           # 32 GiB
+    assert E.choose_stage_bits(26, o, 1) == 12           # 512 MiB: not above
+    assert E.choose_stage_bits(20, o, 16) == 12          # 128 MiB
+    assert E.choose_stage_bits(10, o, 1 << 20) == 10     # many rows, few qubits
+    assert E.choose_stage_bits(24, E.EngineOptions(stage_bits=11), 64) == 11
