@@ -260,3 +260,112 @@ def test_matrix_core_products_in_forward_code():
     (s2,) = _stages(c)
     assert "tq_mm(" not in J.gen_forward(s2.words, "kc", mfma=True)
     assert len(J.compile_source(J.PRELUDE + on)) > 1000
+
+
+def _xl_model(b, x0, x1):
+    """What tq_xl<b> leaves in each of the 64 lanes (the semantics the
+    permlane / DPP helpers implement): a lane with bit b set takes x1 of
+    lane l - 2^b into x0, a lane with bit b clear takes x0 of lane l + 2^b
+    into x1."""
+    n0, n1 = x0.copy(), x1.copy()
+    for l in range(64):
+        if (l >> b) & 1:
+            n0[l] = x1[l - (1 << b)]
+        else:
+            n1[l] = x0[l + (1 << b)]
+    return n0, n1
+
+
+def _local_index(layout, T, tid, j):
+    return (sum(layout[i] for i in range(T) if (tid >> i) & 1)
+            + sum(layout[T + k] for k in range(4) if (j >> k) & 1))
+
+
+def _lane_plan_holds(T, F, G):
+    """Run lane_plan's exchanges on every wave of a 2^T-thread group of local
+    indices; True if each (thread, slot) ends holding the local index that
+    layout G assigns it (what the LDS remap produces)."""
+    from torchquantum_amd import stagejit as J
+    steps, perm = J.lane_plan(T, F, G)
+    for w in range(1 << (T - 6)):
+        tids = [64 * w + l for l in range(64)]
+        regs = [np.array([_local_index(F, T, t, j) for t in tids]) for j in range(NS)]
+        for b, sb in steps:
+            for j0 in range(NS):
+                if not j0 & (1 << sb):
+                    regs[j0], regs[j0 | (1 << sb)] = _xl_model(b, regs[j0], regs[j0 | (1 << sb)])
+        want = [np.array([_local_index(G, T, t, j) for t in tids]) for j in range(NS)]
+        if not all((regs[perm[j]] == want[j]).all() for j in range(NS)):
+            return False
+    return True
+
+
+def test_lane_remap_plan_matches_lds_remap():
+    """Remaps that keep the wave bits and only exchange lane bits with
+    register slots run as cross-lane moves in compiled adjoint stages; the
+    exchange sequence reproduces the LDS remap's permutation on random
+    layouts, and every other remap is left to the LDS."""
+    import random
+    from torchquantum_amd import stagejit as J
+    rnd = random.Random(1)
+    seen, lane_bits = 0, set()
+    for _ in range(600):
+        T = rnd.choice([6, 7, 9])
+        m = T + 4
+        F = [1 << i for i in range(m)]
+        rnd.shuffle(F)
+        G = list(F)
+        lanes = rnd.sample(range(6), rnd.randint(1, 4))
+        for b, r in zip(lanes, rnd.sample(range(T, m), len(lanes))):
+            G[b], G[r] = G[r], G[b]
+        regs = G[T:]
+        rnd.shuffle(regs)
+        G[T:] = regs
+        wave_moved = T > 6 and rnd.random() < 0.2
+        if wave_moved:
+            i = rnd.randrange(6, T)
+            r = rnd.randrange(T, m)
+            G[i], G[r] = G[r], G[i]
+        plan = J.lane_plan(T, F, G)
+        if wave_moved or all(F[i] == G[i] for i in range(6)):
+            assert plan is None
+            continue
+        assert plan is not None
+        assert _lane_plan_holds(T, F, G)
+        seen += 1
+        lane_bits |= {b for b, _ in plan[0]}
+    assert seen > 300 and lane_bits == set(range(6))
+
+
+def test_lane_remaps_in_adjoint_code():
+    """Adjoint stages of the 24-qubit gradient ansatz (13-bit stages): the
+    wave-local remaps become permlane / DPP exchanges (fewer barriers than
+    remaps through the LDS), while the forward kernels keep the LDS."""
+    import torchquantum_amd as tq
+    import torchquantum_amd.functional as tqf
+    from torchquantum_amd import engine as E, stagejit as J
+    from circuits import config3_ops
+    n = 24
+    q = tq.QuantumDevice(n_wires=n, bsz=1, device="cpu")
+    for w in range(n):
+        tqf.ry(q, w, params=torch.full((1, 1), 0.3))
+    for op in config3_ops(seed=0, n=n, layers=2):
+        p = op["params"]
+        tqf.func_name_dict[op["name"]](q, op["wires"], params=None if p is None else
+                                       torch.tensor(np.asarray(p, dtype=np.float32), requires_grad=True))
+    c = E.Compiled(q._tape, n, 64, tq.EngineOptions(stage_bits=13), "cpu")
+    found = 0
+    for st in _stages(c):
+        sp = J.StageProgram(st.words)
+        lane = [w for w in sp.ops if (w[0] & 15) == E.OP_REMAP
+                and J.lane_plan(sp.T, *reversed(sp.remap_layouts(w))) is not None]
+        if not lane:
+            continue
+        found += 1
+        adj = J.gen_adjoint(st.words, "k")
+        adj_lds = J.gen_adjoint(st.words, "k", lane_remap=False)
+        assert "tq_xl" in adj and "tq_xl" not in adj_lds
+        # an LDS remap has two barriers per array (psi, lam); a lane remap none
+        assert adj.count("__syncthreads();") == adj_lds.count("__syncthreads();") - 4 * len(lane)
+        assert "tq_xl" not in J.gen_forward(st.words, "k")
+    assert found >= 2

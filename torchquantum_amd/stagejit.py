@@ -61,8 +61,14 @@ COMPILE_OPTS["p"] = COMPILE_OPTS["a"]     # pruned adjoint stages (gen_adjoint w
 # the gradient workload.  Adjoint stages: half the vector instructions, but
 # 176 VGPRs + 64 AGPRs against 106 (or spills when held to 128), and
 # 42.1 -> 44.2 ms; they stay on the vector ALUs.
+# lane_remap_f / _a: a remap that leaves the wave bits alone and only
+# exchanges lane bits with register slots runs as cross-lane register moves
+# (permlane swaps for lane bits 4 and 5, DPP for 0-3): no LDS traffic and no
+# barrier.  Adjoint stages of the gradient workload (13-bit stages, six
+# stages): 62.1 -> 60.8 ms; forward stages, which remap one array instead of
+# two: 25.1 -> 25.4 ms, so they keep the LDS.
 CODEGEN = {"buf": True, "prefetch_f": True, "prefetch_a": False, "lds": "pad4", "racc": 0,
-           "mfma_f": True, "mfma_a": False}
+           "mfma_f": True, "mfma_a": False, "lane_remap_f": False, "lane_remap_a": True}
 
 PRELUDE = r"""
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -162,6 +168,46 @@ __device__ __forceinline__ float allsum8(float v) {
   v += __builtin_amdgcn_update_dpp(0.f, v, 0xB1, 0xf, 0xf, false);    // quad_perm [1,0,3,2]: l ^ 1
   return v;
 }
+// Exchange of lane bit b with a register slot bit.  (x0, x1) are the two
+// slots that differ in that slot bit; afterwards lane bit b indexes what the
+// slot bit did and vice versa: a lane with bit b set takes x1 of lane
+// l - 2^b into x0, a lane with bit b clear takes x0 of lane l + 2^b into x1.
+// b = 5 / 4: v_permlane32_swap / v_permlane16_swap exchange exactly these
+// halves; b = 3 / 2: DPP row_shr / row_shl by 2^b with a bank mask keeping
+// the lanes that stay; b = 1 / 0: quad_perm (l ^ 2^b) and a select.
+struct tq_p2 { float2 a, b; };
+__device__ __forceinline__ tq_p2 tq_xl5(float2 x0, float2 x1) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x0.x), __float_as_uint(x1.x), false, false);
+  const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(x0.y), __float_as_uint(x1.y), false, false);
+  return {make_float2(__uint_as_float(r[0]), __uint_as_float(q[0])), make_float2(__uint_as_float(r[1]), __uint_as_float(q[1]))};
+}
+__device__ __forceinline__ tq_p2 tq_xl4(float2 x0, float2 x1) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x0.x), __float_as_uint(x1.x), false, false);
+  const auto q = __builtin_amdgcn_permlane16_swap(__float_as_uint(x0.y), __float_as_uint(x1.y), false, false);
+  return {make_float2(__uint_as_float(r[0]), __uint_as_float(q[0])), make_float2(__uint_as_float(r[1]), __uint_as_float(q[1]))};
+}
+__device__ __forceinline__ tq_p2 tq_xl3(float2 x0, float2 x1) {
+  return {make_float2(__builtin_amdgcn_update_dpp(x0.x, x1.x, 0x118, 0xf, 0xc, false),
+                      __builtin_amdgcn_update_dpp(x0.y, x1.y, 0x118, 0xf, 0xc, false)),
+          make_float2(__builtin_amdgcn_update_dpp(x1.x, x0.x, 0x108, 0xf, 0x3, false),
+                      __builtin_amdgcn_update_dpp(x1.y, x0.y, 0x108, 0xf, 0x3, false))};
+}
+__device__ __forceinline__ tq_p2 tq_xl2(float2 x0, float2 x1) {
+  return {make_float2(__builtin_amdgcn_update_dpp(x0.x, x1.x, 0x114, 0xf, 0xa, false),
+                      __builtin_amdgcn_update_dpp(x0.y, x1.y, 0x114, 0xf, 0xa, false)),
+          make_float2(__builtin_amdgcn_update_dpp(x1.x, x0.x, 0x104, 0xf, 0x5, false),
+                      __builtin_amdgcn_update_dpp(x1.y, x0.y, 0x104, 0xf, 0x5, false))};
+}
+template <int ctrl>
+__device__ __forceinline__ tq_p2 tq_xlq(float2 x0, float2 x1, bool hi) {
+  const float2 d1 = make_float2(__builtin_amdgcn_update_dpp(0.f, x1.x, ctrl, 0xf, 0xf, false),
+                                __builtin_amdgcn_update_dpp(0.f, x1.y, ctrl, 0xf, 0xf, false));
+  const float2 d0 = make_float2(__builtin_amdgcn_update_dpp(0.f, x0.x, ctrl, 0xf, 0xf, false),
+                                __builtin_amdgcn_update_dpp(0.f, x0.y, ctrl, 0xf, 0xf, false));
+  return {hi ? d1 : x0, hi ? x1 : d0};
+}
+__device__ __forceinline__ tq_p2 tq_xl1(float2 x0, float2 x1, bool hi) { return tq_xlq<0x4E>(x0, x1, hi); }
+__device__ __forceinline__ tq_p2 tq_xl0(float2 x0, float2 x1, bool hi) { return tq_xlq<0xB1>(x0, x1, hi); }
 """
 
 NT_HELPERS = r"""
@@ -263,10 +309,13 @@ def supported(words: np.ndarray) -> bool:
 
 
 class _Gen:
-    def __init__(self, sp: StageProgram, lds_mode: str = "xor", mfma: bool = False):
+    def __init__(self, sp: StageProgram, lds_mode: str = "xor", mfma: bool = False, lane_remap: bool = False):
         self.sp = sp
+        self.lane_remap = lane_remap   # wave-local remaps as cross-lane moves
         self.mfma = mfma          # 2x2 products of uncontrolled single-wire ops on the matrix cores
         self.used_mfma = False
+        self.used_lds = False     # some remap goes through the LDS buffer
+        self.lane_flags = set()   # lane bits (0, 1) whose flags lane remaps select on
         self.lines: List[str] = []
         self.k = 0
         self.ind = 2
@@ -340,6 +389,10 @@ class _Gen:
         return f"{ra} ^ {swz(lr)}u"
 
     def remap(self, a: List[str], frm: Sequence[int], to: Sequence[int], lds: str = "lds") -> List[str]:
+        plan = lane_plan(self.sp.T, frm, to) if self.lane_remap else None
+        if plan is not None:
+            return self.remap_lanes(a, plan)
+        self.used_lds = True
         T = self.sp.T
         ra, rb = self.lane_index(frm), self.lane_index(to)
         self.line("__syncthreads();")
@@ -358,6 +411,23 @@ class _Gen:
                     lr |= to[T + k]
             out.append(self.var(f"{lds}[{self._slot(rb, lr)}]"))
         return out
+
+    def remap_lanes(self, a: List[str], plan) -> List[str]:
+        """A remap within each wave as lane-bit / slot-bit exchanges
+        (tq_xl<b>), then a renaming of the registers."""
+        steps, perm = plan
+        cur = list(a)
+        for b, sb in steps:
+            if b < 2:
+                self.lane_flags.add(b)
+            for j0 in range(NS):
+                if j0 & (1 << sb):
+                    continue
+                j1 = j0 | (1 << sb)
+                extra = f", lb{b}" if b < 2 else ""
+                r = self.var(f"tq_xl{b}({cur[j0]}, {cur[j1]}{extra})", "tq_p2")
+                cur[j0], cur[j1] = f"{r}.a", f"{r}.b"
+        return [cur[perm[j]] for j in range(NS)]
 
     # ---- forward ops (k_stage_fwd's fwd_op)
     def f_xperm(self, w, a: List[str]):
@@ -857,10 +927,45 @@ def _mfma(kind: str, mfma: Optional[bool] = None) -> bool:
     return CODEGEN["mfma_" + kind] if mfma is None else bool(mfma)
 
 
+def _lane(kind: str, lane_remap: Optional[bool] = None) -> bool:
+    kind = "a" if kind == "p" else kind
+    return CODEGEN["lane_remap_" + kind] if lane_remap is None else bool(lane_remap)
+
+
 def _lane_bits(g: "_Gen") -> List[str]:
-    """Lane-position flags the matrix-core rows need (tq_row_u / tq_row_udag)."""
-    return (["  const bool mhi = (threadIdx.x & 2u) != 0u;", "  const bool modd = (threadIdx.x & 1u) != 0u;"]
-            if g.used_mfma else [])
+    """Lane-position flags the matrix-core rows (tq_row_u / tq_row_udag) and
+    the lane remaps on lane bits 0 and 1 need."""
+    out = (["  const bool mhi = (threadIdx.x & 2u) != 0u;", "  const bool modd = (threadIdx.x & 1u) != 0u;"]
+           if g.used_mfma else [])
+    return out + [f"  const bool lb{b} = (threadIdx.x & {1 << b}u) != 0u;" for b in sorted(g.lane_flags)]
+
+
+def lane_plan(T: int, frm: Sequence[int], to: Sequence[int]):
+    """How a remap from layout `frm` to layout `to` (local-bit values of the
+    T thread bits, then the R_BITS slot bits) runs inside each wave: None
+    unless every wave bit (thread bit >= 6) keeps its local bit and every lane
+    bit that changes takes a local bit from a register slot and gives its own
+    to one.  Otherwise ([(lane bit b, slot bit s), ...], perm): exchange lane
+    bit b with slot bit s in turn (slot bit s indexing the slots of the layout
+    at that point), then slot j of `to` is slot perm[j] of the result."""
+    F, G = [int(x) for x in frm], [int(x) for x in to]
+    if T < 6 or any(F[i] != G[i] for i in range(6, T)):
+        return None
+    reg, regs_to = F[T:T + R_BITS], set(G[T:T + R_BITS])
+    steps = []
+    for b in range(6):
+        if F[b] == G[b]:
+            continue
+        if G[b] not in reg or F[b] not in regs_to:
+            return None
+        sb = reg.index(G[b])
+        steps.append((b, sb))
+        reg[sb] = F[b]
+    if not steps:
+        return None
+    where = [reg.index(G[T + k]) for k in range(R_BITS)]
+    perm = [sum(1 << where[k] for k in range(R_BITS) if (j >> k) & 1) for j in range(NS)]
+    return steps, perm
 
 
 def _codegen(buf: Optional[bool], prefetch: Optional[bool], kind: str, lds: Optional[str] = None,
@@ -873,7 +978,8 @@ def _codegen(buf: Optional[bool], prefetch: Optional[bool], kind: str, lds: Opti
 
 
 def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Optional[bool] = None,
-                prefetch: Optional[bool] = None, lds: Optional[str] = None, mfma: Optional[bool] = None) -> str:
+                prefetch: Optional[bool] = None, lds: Optional[str] = None, mfma: Optional[bool] = None,
+                lane_remap: Optional[bool] = None) -> str:
     """HIP source of one forward stage kernel (k_stage_fwd semantics).
     nontemporal: the state is read and written with non-temporal hints;
     buf: buffer instructions for the state where the offsets allow;
@@ -886,14 +992,12 @@ def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
         raise ValueError("compiled stages need T >= 6")
     buf, prefetch, lds, _ = _codegen(buf, prefetch, "f", lds)
     mode = _mem_mode(sp, nontemporal, buf)
-    g = _Gen(sp, lds, _mfma("f", mfma))
+    g = _Gen(sp, lds, _mfma("f", mfma), _lane("f", lane_remap))
     a = [f"i{j}" for j in range(NS)]
-    has_remap = False
     for w in sp.ops:
         if (w[0] & 15) == OP_REMAP:
             frm, to = sp.remap_layouts(w)
             a = g.remap(a, frm, to)
-            has_remap = True
         else:
             g.fwd_op(w, a)
     body = ["  " + s for s in g.lines]
@@ -902,7 +1006,7 @@ def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
     src = [f'extern "C" __global__ void __launch_bounds__({nt}) {name}(float2* __restrict__ out, '
            "const float2* __restrict__ in, const float2* __restrict__ mats, int init_zero, long long n_groups, "
            "long long gpw) {"]
-    if has_remap:
+    if g.used_lds:
         src.append(f"  __shared__ float2 lds[{lds_words(sp.m, lds)}];")
     src.append("  const int tid = threadIdx.x;")
     src += _lane_offsets(sp)
@@ -944,7 +1048,8 @@ def gen_forward(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
 
 def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Optional[bool] = None,
                 prefetch: Optional[bool] = None, lds: Optional[str] = None, racc: Optional[int] = None,
-                skip: Sequence[int] = (), no_store: bool = False, mfma: Optional[bool] = None) -> str:
+                skip: Sequence[int] = (), no_store: bool = False, mfma: Optional[bool] = None,
+                lane_remap: Optional[bool] = None) -> str:
     """HIP source of one adjoint stage kernel (k_stage_adj semantics);
     nontemporal, buf, prefetch and lds as for gen_forward (prefetch loads
     psi and lam of the next group).  racc: when the stage has at most this
@@ -962,10 +1067,9 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
         raise ValueError("compiled stages need T >= 6")
     buf, prefetch, lds, racc = _codegen(buf, prefetch, "a", lds, racc)
     mode = _mem_mode(sp, nontemporal, buf)
-    g = _Gen(sp, lds, _mfma("a", mfma))
+    g = _Gen(sp, lds, _mfma("a", mfma), _lane("a", lane_remap))
     p = [f"p{j}" for j in range(NS)]
     l = [f"l{j}" for j in range(NS)]
-    has_remap = False
     in_regs = 0 < sp.ngp <= racc
     acc_ops = []            # (op words, register accumulators of its partials)
     skip = frozenset(int(x) for x in skip)
@@ -979,7 +1083,6 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
             frm, to = sp.remap_layouts(w)
             p = g.remap(p, to, frm)
             l = g.remap(l, to, frm)
-            has_remap = True
             continue
         parts = g.bwd_op(w, p, l, update=pos not in skip)
         if w[7] >= 0 and parts is not None:
@@ -1011,7 +1114,7 @@ def gen_adjoint(words: np.ndarray, name: str, nontemporal: bool = False, buf: Op
                "float2* __restrict__ lam, const float2* psi_in, const float2* lam_in, "
                "const float2* __restrict__ mats, float* __restrict__ part, "
                "long long rpw, long long n_groups, long long gpw) {")
-    if has_remap:
+    if g.used_lds:
         src.append(f"  __shared__ float2 lds[{lds_words(sp.m, lds)}];")
     if ngp:
         src.append(f"  __shared__ float gacc[{nwave * ngp}];")
@@ -1106,7 +1209,7 @@ def kernel_name(kind: str, prog, nt: bool = False) -> str:
     if extra:
         hs.update(repr(sorted(extra.items())).encode())
     hs.update(COMPILE_OPTS[kind].encode())
-    hs.update(repr((_codegen(None, None, kind), _mfma(kind))).encode())
+    hs.update(repr((_codegen(None, None, kind), _mfma(kind), _lane(kind))).encode())
     if nt:
         hs.update(b"nt")
     h = hs.hexdigest()[:20]
