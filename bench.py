@@ -607,7 +607,7 @@ def config5_plans(args, tq):
     per candidate).  The learned planner -- the MLP trained by this package's
     kernels on exhaustive timings of six smaller circuits of the same family
     (16, 20 and 24 qubits, bsz 1) -- chooses from plan features alone;
-    exhaustive timing of every candidate at full size judges the choice."""
+    exhaustive timing of every candidate at full size scores the choice."""
     from torchquantum_amd import batch as B
     from torchquantum_amd.autotune import Autotuner, LearnedPlanner, circuit_key, time_plan
     from torchquantum_amd.costmodel import MLPCostModel
@@ -639,7 +639,7 @@ def config5_plans(args, tq):
         measured.append(time_plan(tape, n, bsz, o, reps=1))
         torch.cuda.empty_cache()
         log(f"[config5-plans] fusion width {o.fusion_width}: {measured[-1]:.0f} ms")
-    t_judge = time.perf_counter() - t0
+    t_exhaustive = time.perf_counter() - t0
     best = int(np.argmin(measured))
     ip = opts.index(pick)
     return {"workload": f"{n} qubits, bsz {bsz}, the config-5 circuit, forward",
@@ -650,7 +650,7 @@ def config5_plans(args, tq):
             "predicted_ms_by_fusion_width": {str(o.fusion_width): float(p) for o, p in zip(opts, pred)},
             "learned_pick": pick.fusion_width, "exhaustive_pick": opts[best].fusion_width,
             "learned_regret_vs_exhaustive": measured[ip] / measured[best] - 1,
-            "default_plan_ms": measured[0], "learned_selection_s": t_pick, "exhaustive_timing_s": t_judge}
+            "default_plan_ms": measured[0], "learned_selection_s": t_pick, "exhaustive_timing_s": t_exhaustive}
 
 
 def cpu_threads():
