@@ -259,8 +259,8 @@ def test_partition_properties():
     ops = random_ops(n, 300, 1, rng, names=["h", "rx", "cnot", "cz", "toffoli", "swap", "rzz", "c4x", "multirz"])
     tape = _tape_from_ops(ops)
     kops = E.lower(tape, n, 1, [0] * len(tape))
-    for m in (5, 8, 12):
-        steps = E.partition(kops, n, m, 3)
+    for m, lookahead in ((5, False), (8, False), (12, False), (5, True), (8, True), (12, True)):
+        steps = E.partition(kops, n, m, 3, lookahead=lookahead)
         seen = []
         for kind, payload in steps:
             if kind == "solo":
@@ -280,6 +280,67 @@ def test_partition_properties():
                 if b in last:
                     assert pos[last[b]] < pos[i]
                 last[b] = i
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_lookahead_partition_emulated(seed):
+    """Lookahead partition (local bits chosen by how many ops they admit):
+    emulated forward == oracle, and the adjoint sweep's parameter gradients
+    == central differences of the oracle."""
+    rng = np.random.default_rng(300 + seed)
+    n, bsz = 7, 2
+    names = ["rx", "ry", "rz", "u3", "cnot", "cz", "crx", "rzz", "h", "swap", "toffoli", "rot", "cu3"]
+    ops = random_ops(n, 40, bsz, rng, names=names)
+    tape = _tape_from_ops(ops, grad=True)
+    opts = E.EngineOptions(mode="fused", stage_bits=5, partition="lookahead")
+    c = E.Compiled(tape, n, bsz, opts, "cpu")
+    assert sum(1 for k, _ in c.steps if k == "stage") > 1
+    pool = c.pool_host(tape, True, True).astype(np.complex128)
+    init = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+    psi = EM.run_compiled(c, pool, init)
+    assert np.abs(psi - O.run_ops(oracle_ops(ops, bsz), n, bsz, state=init)).max() < 5e-6
+    cvec = rng.normal(size=(bsz, 2 ** n)) + 1j * rng.normal(size=(bsz, 2 ** n))
+    grad = EM.adjoint_compiled(c, pool, psi, cvec.copy())
+
+    def L(flat):
+        k = 0
+        o2 = []
+        for op in ops:
+            p = op["params"]
+            if p is not None and not np.iscomplexobj(p):
+                p = flat[k:k + p.size].reshape(p.shape)
+                k += p.size
+            o2.append(dict(op, params=p))
+        return float(np.real(np.sum(np.conj(cvec) * O.run_ops(oracle_ops(o2, bsz), n, bsz, state=init))))
+
+    flat = np.concatenate([op["params"].reshape(-1) for op in ops
+                           if op["params"] is not None and not np.iscomplexobj(op["params"])])
+    fd = O.central_fd(L, flat.copy(), h=1e-5)
+    got = []
+    for ti, op in enumerate(tape):
+        if op.params is None or op.is_matrix_param:
+            continue
+        off = c.param_offsets[ti]
+        g = grad[:, off:off + op.n_params()]
+        got.append(g.sum(0) if op.params.shape[0] == 1 else g.reshape(-1))
+    got = np.concatenate(got)
+    assert np.all(np.abs(got - fd) < 1e-4 * (1 + np.abs(fd)))
+
+
+def test_lookahead_partition_default_for_large_states():
+    """The 24-qubit bsz-64 gradient ansatz (8 GiB state) is partitioned by
+    lookahead by default, into fewer 13-bit stages than the greedy pass;
+    states of up to 2^22 amplitudes keep the greedy pass (milliseconds)."""
+    tape = _tape_from_ops(config3_ops(seed=0, n=24, layers=10))
+    count = {}
+    for part in ("auto", "greedy"):
+        c = E.Compiled(tape, 24, 64, E.EngineOptions(partition=part), "cpu")
+        count[part] = sum(1 for k, _ in c.steps if k == "stage")
+    assert count["auto"] < count["greedy"]
+    assert E.use_lookahead(24, E.EngineOptions(), 64) and E.use_lookahead(20, E.EngineOptions(), 16)
+    assert not E.use_lookahead(22, E.EngineOptions(), 1)
+    with pytest.raises(ValueError):
+        E.use_lookahead(24, E.EngineOptions(partition="best"), 64)
 
 
 def test_benchmark_circuits_fuse_into_few_stages():
